@@ -1,0 +1,637 @@
+// sp_api.hip — C ABI of libspiral_hip.so (declared in include/spiral_pir.h).
+//
+// Host side of the MI355X Spiral PIR answer path: context and device tables,
+// reference-surface operators (NTT_R, INTT_R, from_ntt, to_ntt, automorph,
+// inv_gadget_ntt_mult, db_mul_kernel, FOLD_CIPHERTEXTS,
+// coefficient_expansion_r), the resident database and the per-query answer
+// pipeline.  Every function returns a status code; nothing exits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spiral_pir.h"
+#include "sp_kernels.hpp"
+
+using namespace sp;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(SP_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                     \
+  } while (0)
+
+extern "C" const char *sp_last_error(void) { return g_err.c_str(); }
+extern "C" const char *sp_version(void) {
+  return "spiral_hip 0.1 gfx950: wave64 NTT (Shoup/Harvey), db_mul u64-lazy VALU, fold fast+general";
+}
+
+// ---------------------------------------------------------------------------
+// host arithmetic for table construction
+// ---------------------------------------------------------------------------
+typedef unsigned __int128 u128;
+static uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+  uint64_t r = 1 % m;
+  b %= m;
+  while (e) {
+    if (e & 1) r = (uint64_t)((u128)r * b % m);
+    b = (uint64_t)((u128)b * b % m);
+    e >>= 1;
+  }
+  return r;
+}
+static uint32_t shoup_companion(uint32_t w, uint32_t q) { return (uint32_t)(((uint64_t)w << 32) / q); }
+static uint32_t brv11_h(uint32_t x) {
+  uint32_t r = 0;
+  for (int i = 0; i < LOGN; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+  return r;
+}
+static const uint32_t kQ[2] = {Q0, Q1};
+static const uint32_t kPsi[2] = {66687u, 158221u};  // gpu-pir/params.cuh:26,29
+
+static ModConst make_modconst(int m) {
+  ModConst c{};
+  const uint32_t q = kQ[m], qo = kQ[1 - m];
+  c.q = q;
+  uint32_t inv = 1;  // q^-1 mod 2^32 by Newton iteration
+  for (int i = 0; i < 6; i++) inv *= 2u - q * inv;
+  c.qinv_neg = (uint32_t)(0u - inv);
+  c.r2 = (uint32_t)(((u128)1 << 64) % q);
+  const uint64_t ninv = powmod(N, q - 2, q);
+  const uint64_t oinv = powmod(qo, q - 2, q);
+  c.crt_c = (uint32_t)(ninv * oinv % q);
+  c.crt_cp = shoup_companion(c.crt_c, q);
+  c.ninv = (uint32_t)ninv;
+  c.ninv_p = shoup_companion(c.ninv, q);
+  c.two32 = (uint32_t)((1ull << 32) % q);
+  c.two32_p = shoup_companion(c.two32, q);
+  c.mu1 = (uint32_t)((1ull << 32) / q);
+  return c;
+}
+
+// Twiddle table for (m, inverse): see the TW_* layout in sp_device.hpp.
+static void build_tw(uint32_t *out, int m, int inv) {
+  const uint32_t q = kQ[m];
+  const uint64_t root = inv ? powmod(kPsi[m], q - 2, q) : kPsi[m];
+  std::vector<uint32_t> T(N);
+  for (uint32_t i = 0; i < (uint32_t)N; i++) T[i] = (uint32_t)powmod(root, brv11_h(i), q);
+  std::fill(out, out + TW_WORDS, 0u);
+  for (int idx = 1; idx < 32; idx++) {
+    out[2 * idx] = T[idx];
+    out[2 * idx + 1] = shoup_companion(T[idx], q);
+  }
+  for (int lane = 0; lane < 64; lane++) {
+    const int L = lane >> 1, b = lane & 1;
+    for (int s = 0; s < TW_B_SLOTS; s++) {
+      int idx;
+      if (s == 0) idx = 32 + L;
+      else if (s < 3) idx = 64 + 2 * L + (s - 1);
+      else if (s < 7) idx = 128 + 4 * L + (s - 3);
+      else if (s < 15) idx = 256 + 8 * L + (s - 7);
+      else if (s < 31) idx = 512 + 16 * L + (s - 15);
+      else idx = 1024 + 32 * L + 16 * b + (s - 31);
+      out[TW_A_WORDS + (2 * s) * 64 + lane] = T[idx];
+      out[TW_A_WORDS + (2 * s + 1) * 64 + lane] = shoup_companion(T[idx], q);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct sp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t *d_tw = nullptr;
+  uint32_t *d_neg1 = nullptr;
+  DevTables tables{};
+};
+
+static hipStream_t pick(sp_ctx *ctx, void *stream) {
+  return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+extern "C" int sp_ctx_create(int device, sp_ctx **out) {
+  if (!out) return fail(SP_ERR_ARG, "sp_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(SP_ERR_ARG, "sp_ctx_create: device %d of %d", device, ndev);
+  HIP_TRY(hipSetDevice(device));
+  sp_ctx *c = new sp_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(SP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  std::vector<uint32_t> tw((size_t)4 * TW_WORDS);
+  for (int m = 0; m < 2; m++)
+    for (int inv = 0; inv < 2; inv++) build_tw(tw.data() + (size_t)(m * 2 + inv) * TW_WORDS, m, inv);
+  // v_neg1[r] = NTT(-x^(N - 2^r)) = x^(-2^r): value psi^(-2^r (2 brv(z) + 1)) at position z.
+  std::vector<uint32_t> neg1((size_t)LOGN * 2 * 2 * N);
+  for (int r = 0; r < LOGN; r++)
+    for (int m = 0; m < 2; m++) {
+      const uint32_t q = kQ[m];
+      const uint64_t psi_inv = powmod(kPsi[m], q - 2, q);
+      uint32_t *w = neg1.data() + ((size_t)r * 2 + m) * 2 * N;
+      for (int p = 0; p < N; p++) {
+        const uint32_t z = (uint32_t)z_of_p(p);
+        const uint64_t e = ((uint64_t)(2 * brv11_h(z) + 1) << r) % (2 * N);
+        const uint32_t val = (uint32_t)powmod(psi_inv, e, q);
+        w[p] = val;
+        w[N + p] = shoup_companion(val, q);
+      }
+    }
+  e = hipMalloc(&c->d_tw, tw.size() * 4);
+  if (e == hipSuccess) e = hipMalloc(&c->d_neg1, neg1.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(c->d_tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_neg1, neg1.data(), neg1.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    int code = fail(SP_ERR_HIP, "sp_ctx_create tables: %s", hipGetErrorString(e));
+    sp_ctx_destroy(c);
+    return code;
+  }
+  c->tables.tw = c->d_tw;
+  c->tables.neg1 = c->d_neg1;
+  c->tables.mc[0] = make_modconst(0);
+  c->tables.mc[1] = make_modconst(1);
+  *out = c;
+  return SP_OK;
+}
+
+extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
+  if (!ctx) return SP_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_tw) (void)hipFree(ctx->d_tw);
+  if (ctx->d_neg1) (void)hipFree(ctx->d_neg1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SP_OK;
+}
+
+extern "C" int sp_ctx_synchronize(sp_ctx *ctx) {
+  if (!ctx) return fail(SP_ERR_ARG, "ctx is NULL");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SP_OK;
+}
+
+// RAII device buffer for temporaries of boundary operators.
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 4); }
+  uint32_t *u32() const { return static_cast<uint32_t *>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// reference-surface operators
+// ---------------------------------------------------------------------------
+extern "C" int sp_ntt_fwd(sp_ctx *ctx, uint32_t *dev_polys, size_t n_polys, int residue, void *stream) {
+  if (!ctx || (!dev_polys && n_polys) || residue < 0 || residue > 1) return fail(SP_ERR_ARG, "sp_ntt_fwd: bad argument");
+  if (!n_polys) return SP_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_ntt_natural(false, dev_polys, n_polys, ctx->tables, residue, 0, pick(ctx, stream)));
+  return SP_OK;
+}
+
+extern "C" int sp_ntt_inv(sp_ctx *ctx, uint32_t *dev_polys, size_t n_polys, int residue, int scale, void *stream) {
+  if (!ctx || (!dev_polys && n_polys) || residue < 0 || residue > 1) return fail(SP_ERR_ARG, "sp_ntt_inv: bad argument");
+  if (!n_polys) return SP_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_ntt_natural(true, dev_polys, n_polys, ctx->tables, residue, scale ? 1 : 0, pick(ctx, stream)));
+  return SP_OK;
+}
+
+extern "C" int sp_from_ntt(sp_ctx *ctx, const uint32_t *dev_ntt, uint32_t *dev_raw, size_t n_polys, void *stream) {
+  if (!ctx || ((!dev_ntt || !dev_raw) && n_polys)) return fail(SP_ERR_ARG, "sp_from_ntt: bad argument");
+  if (!n_polys) return SP_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_from_ntt(false, dev_ntt, dev_raw, n_polys, 2 * N, 2 * N, ctx->tables, pick(ctx, stream)));
+  return SP_OK;
+}
+
+extern "C" int sp_to_ntt(sp_ctx *ctx, const uint32_t *dev_raw, uint32_t *dev_ntt, size_t n_polys, void *stream) {
+  if (!ctx || ((!dev_ntt || !dev_raw) && n_polys)) return fail(SP_ERR_ARG, "sp_to_ntt: bad argument");
+  if (!n_polys) return SP_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_to_ntt(dev_raw, dev_ntt, n_polys, ctx->tables, pick(ctx, stream)));
+  return SP_OK;
+}
+
+extern "C" int sp_automorph(sp_ctx *ctx, const uint64_t *dev_in, uint64_t *dev_out, size_t n_polys, uint32_t t,
+                            void *stream) {
+  if (!ctx || ((!dev_in || !dev_out) && n_polys)) return fail(SP_ERR_ARG, "sp_automorph: bad argument");
+  if ((t & 1u) == 0) return fail(SP_ERR_ARG, "sp_automorph: t = %u must be odd", t);
+  if (dev_in == dev_out) return fail(SP_ERR_ARG, "sp_automorph: in-place is not supported");
+  if (!n_polys) return SP_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_automorph_u64(dev_in, dev_out, n_polys, t % (2 * N), pick(ctx, stream)));
+  return SP_OK;
+}
+
+static int gadget_bits(int num_elems) { return (56 + num_elems - 1) / num_elems; }
+
+extern "C" int sp_gadget_extprod(sp_ctx *ctx, const uint32_t *dev_in, int rdim, int mx, const uint32_t *dev_keys,
+                                 uint32_t *dev_acc, size_t n_items, size_t in_stride, size_t key_stride,
+                                 size_t acc_stride, void *stream) {
+  if (!ctx || ((!dev_in || !dev_keys || !dev_acc) && n_items)) return fail(SP_ERR_ARG, "sp_gadget_extprod: NULL");
+  if (rdim < 1 || rdim > 2 || mx < 2 * rdim || mx % rdim || mx > 56)
+    return fail(SP_ERR_ARG, "sp_gadget_extprod: rdim %d, mx %d unsupported (rdim 1|2, 2 <= mx/rdim, mx <= 56)",
+                rdim, mx);
+  if (!n_items) return SP_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_extprod(dev_in, rdim, mx, gadget_bits(mx / rdim), dev_keys, dev_acc, n_items, in_stride,
+                         key_stride, acc_stride, ctx->tables, pick(ctx, stream)));
+  return SP_OK;
+}
+
+static bool db_shape_ok(int nu1, int nu2) { return nu1 >= 2 && nu1 <= 14 && nu2 >= 1 && nu2 <= 12; }
+
+extern "C" int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *dev_first_dim, uint32_t *dev_out,
+                         int nu1, int nu2, void *stream) {
+  if (!ctx || !dev_db || !dev_first_dim || !dev_out) return fail(SP_ERR_ARG, "sp_db_mul: NULL");
+  if (!db_shape_ok(nu1, nu2)) return fail(SP_ERR_ARG, "sp_db_mul: shape (%d,%d) unsupported", nu1, nu2);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const int R = 1 << nu1, C = 1 << nu2;
+  DevBuf db, sel, out;
+  HIP_TRY(db.alloc((size_t)2 * R * C * N * 4));
+  HIP_TRY(sel.alloc((size_t)R * 4 * N * 4));
+  HIP_TRY(out.alloc((size_t)C * 4 * N * 4));
+  HIP_TRY(launch_db_from_ref(dev_db, db.u32(), R, C, s));
+  HIP_TRY(launch_nat_to_p(dev_first_dim, sel.u32(), (size_t)R * 4, 1, 0, ctx->tables, s));
+  HIP_TRY(launch_db_mul(db.u32(), sel.u32(), 4 * N, out.u32(), 1, R, C, ctx->tables, s));
+  HIP_TRY(launch_p_to_nat(out.u32(), dev_out, (size_t)C * 4, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SP_OK;
+}
+
+extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
+                       const uint32_t *dev_v_folding_neg, void *stream) {
+  if (!ctx || !dev_accs || !dev_v_folding) return fail(SP_ERR_ARG, "sp_fold: NULL");
+  if (nu2 < 1 || nu2 > 12 || t_gsw < 2 || t_gsw > 28) return fail(SP_ERR_ARG, "sp_fold: nu2 %d, t_gsw %d", nu2, t_gsw);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const int mx = 2 * t_gsw;
+  const size_t level_polys = (size_t)2 * mx * 2;
+  DevBuf kp, kn;
+  HIP_TRY(kp.alloc(level_polys * nu2 * N * 4));
+  HIP_TRY(launch_nat_to_p(dev_v_folding, kp.u32(), level_polys * nu2, 2 * mx, 1, ctx->tables, s));
+  const bool fast = dev_v_folding_neg == nullptr;
+  if (!fast) {
+    HIP_TRY(kn.alloc(level_polys * nu2 * N * 4));
+    HIP_TRY(launch_nat_to_p(dev_v_folding_neg, kn.u32(), level_polys * nu2, 2 * mx, 1, ctx->tables, s));
+  }
+  int num_per = 1 << nu2;
+  for (int cur = 0; cur < nu2; cur++) {
+    num_per >>= 1;
+    const size_t lvl = (size_t)(nu2 - 1 - cur) * level_polys * N;
+    HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, fast ? nullptr : kn.u32() + lvl, 0,
+                              t_gsw, gadget_bits(t_gsw), ctx->tables, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return SP_OK;
+}
+
+static int ceil_log2(uint64_t x) {
+  int v = 0;
+  uint64_t p = 1;
+  while (p < x) { p <<= 1; v++; }
+  return v;
+}
+
+extern "C" int sp_expansion_depth(int nu1, int nu2, int t_gsw, int *g, int *stop_round) {
+  if (nu1 < 1 || nu2 < 1 || t_gsw < 1) return fail(SP_ERR_ARG, "sp_expansion_depth: bad shape");
+  const int gg = ceil_log2((uint64_t)t_gsw * nu2 + (1ull << nu1));
+  const int st = ceil_log2((uint64_t)t_gsw * nu2);
+  if (gg > LOGN) return fail(SP_ERR_ARG, "expansion depth %d exceeds log2(N)", gg);
+  if (g) *g = gg;
+  if (stop_round) *stop_round = st;
+  return SP_OK;
+}
+
+// Runs `rounds` expansion rounds on internal-layout v (n_queries sets).
+static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_queries, int nu1, int nu2, int t_gsw,
+                         int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t key_query_stride_l,
+                         size_t key_query_stride_r, int rounds, hipStream_t s) {
+  int g = 0, stop = 0;
+  int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
+  if (rc) return rc;
+  if (rounds < 0 || rounds > g) rounds = g;
+  if (key_query_stride_l != key_query_stride_r && n_queries > 1)
+    return fail(SP_ERR_ARG, "run_expansion: key strides differ");
+  const size_t key_level = (size_t)2 * t_exp * 2 * N;
+  for (int r = 0; r < rounds; r++) {
+    const int num_in = 1 << r;
+    HIP_TRY(launch_expand_neg1(v, q_stride, n_queries, num_in, ctx->tables.neg1 + (size_t)r * 2 * 2 * N,
+                               ctx->tables, s));
+    const uint32_t *lr = left_p + (size_t)r * key_level;
+    const uint32_t *rr = right_p + (size_t)std::min(r, stop) * key_level;
+    HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr,
+                                 key_query_stride_l, t_exp, gadget_bits(t_exp), ctx->tables, s));
+  }
+  return SP_OK;
+}
+
+extern "C" int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_gsw, int t_exp,
+                         const uint32_t *dev_left, const uint32_t *dev_right, int rounds, void *stream) {
+  if (!ctx || !dev_v || !dev_left || !dev_right) return fail(SP_ERR_ARG, "sp_expand: NULL");
+  if (t_exp < 2 || t_exp > 56 || (t_exp & 1)) return fail(SP_ERR_ARG, "sp_expand: t_exp %d unsupported", t_exp);
+  int g = 0, stop = 0;
+  int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
+  if (rc) return rc;
+  if (stop < 1) return fail(SP_ERR_ARG, "sp_expand: stop_round must be >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const size_t n_ct = (size_t)1 << g;
+  const size_t key_level_polys = (size_t)2 * t_exp * 2;
+  DevBuf v, kl, kr;
+  HIP_TRY(v.alloc(n_ct * 4 * N * 4));
+  HIP_TRY(kl.alloc((size_t)g * key_level_polys * N * 4));
+  HIP_TRY(kr.alloc((size_t)(stop + 1) * key_level_polys * N * 4));
+  HIP_TRY(launch_nat_to_p(dev_v, v.u32(), n_ct * 4, 1, 0, ctx->tables, s));
+  HIP_TRY(launch_nat_to_p(dev_left, kl.u32(), (size_t)g * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
+  HIP_TRY(launch_nat_to_p(dev_right, kr.u32(), (size_t)(stop + 1) * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
+  rc = run_expansion(ctx, v.u32(), 0, 1, nu1, nu2, t_gsw, t_exp, kl.u32(), kr.u32(), 0, 0, rounds, s);
+  if (rc) return rc;
+  HIP_TRY(launch_p_to_nat(v.u32(), dev_v, n_ct * 4, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// resident database
+// ---------------------------------------------------------------------------
+struct sp_db {
+  sp_ctx *ctx = nullptr;
+  int nu1 = 0, nu2 = 0, n2 = 0;
+  uint32_t *d_db = nullptr;  // [n2][2][R][C][P]
+  size_t bytes = 0;
+};
+
+extern "C" int sp_db_create(sp_ctx *ctx, int nu1, int nu2, int n2, sp_db **out) {
+  if (!ctx || !out) return fail(SP_ERR_ARG, "sp_db_create: NULL");
+  *out = nullptr;
+  if (!db_shape_ok(nu1, nu2) || n2 < 1 || n2 > 16) return fail(SP_ERR_ARG, "sp_db_create: shape (%d,%d,%d)", nu1, nu2, n2);
+  HIP_TRY(hipSetDevice(ctx->device));
+  sp_db *d = new sp_db();
+  d->ctx = ctx;
+  d->nu1 = nu1; d->nu2 = nu2; d->n2 = n2;
+  d->bytes = (size_t)n2 * 2 * ((size_t)1 << (nu1 + nu2)) * N * 4;
+  hipError_t e = hipMalloc(&d->d_db, d->bytes);
+  if (e != hipSuccess) {
+    delete d;
+    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_db_create: hipMalloc(%zu): %s", (size_t)0,
+                hipGetErrorString(e));
+  }
+  *out = d;
+  return SP_OK;
+}
+
+extern "C" int sp_db_destroy(sp_db *db) {
+  if (!db) return SP_OK;
+  (void)hipSetDevice(db->ctx->device);
+  (void)hipStreamSynchronize(db->ctx->stream);
+  if (db->d_db) (void)hipFree(db->d_db);
+  delete db;
+  return SP_OK;
+}
+
+extern "C" size_t sp_db_device_bytes(const sp_db *db) { return db ? db->bytes : 0; }
+
+extern "C" int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes) {
+  if (!db || !host_bytes) return fail(SP_ERR_ARG, "sp_db_load_bytes: NULL");
+  sp_ctx *ctx = db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t nrec = (size_t)1 << (db->nu1 + db->nu2);
+  const size_t rec_bytes = (size_t)db->n2 * N;
+  const size_t chunk = std::max<size_t>(1, ((size_t)64 << 20) / rec_bytes);
+  DevBuf stage;
+  HIP_TRY(stage.alloc(std::min(chunk, nrec) * rec_bytes));
+  for (size_t r0 = 0; r0 < nrec; r0 += chunk) {
+    const size_t cnt = std::min(chunk, nrec - r0);
+    HIP_TRY(hipMemcpyAsync(stage.p, host_bytes + r0 * rec_bytes, cnt * rec_bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_db_encode(static_cast<const uint8_t *>(stage.p), r0, cnt, db->d_db, db->n2, 1 << db->nu1,
+                             1 << db->nu2, ctx->tables, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return SP_OK;
+}
+
+extern "C" int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt) {
+  if (!db || !host_ntt) return fail(SP_ERR_ARG, "sp_db_load_ntt: NULL");
+  sp_ctx *ctx = db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int R = 1 << db->nu1, C = 1 << db->nu2;
+  const size_t trial_words = (size_t)2 * R * C * N;
+  DevBuf stage;
+  HIP_TRY(stage.alloc(trial_words * 4));
+  for (int t = 0; t < db->n2; t++) {
+    HIP_TRY(hipMemcpyAsync(stage.p, host_ntt + t * trial_words, trial_words * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_db_from_ref(stage.u32(), db->d_db + t * trial_words, R, C, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return SP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-query answer pipeline
+// ---------------------------------------------------------------------------
+struct sp_query {
+  sp_db *db = nullptr;
+  int t_gsw = 0, t_exp = 0, g = 0, stop = 0;
+  bool fast_fold = true;
+  uint32_t *query_ct = nullptr;  // the query Ciphertext, layout P (expansion input, kept pristine)
+  uint32_t *v = nullptr;       // [2^g] Ciphertext, layout P
+  uint32_t *left = nullptr;    // [g][m][t_exp][row][P], 2^32-scaled
+  uint32_t *right = nullptr;   // [stop+1][...]
+  uint32_t *fkeys = nullptr;   // [nu2][m][2 t_gsw][row][P], 2^32-scaled
+  uint32_t *fkeys_neg = nullptr;
+  uint32_t *acc = nullptr;     // [n2][C] Ciphertext: NTT (P) after db_mul, raw after from_ntt
+  uint32_t *stage = nullptr;   // host-transfer staging
+  size_t stage_words = 0;
+};
+
+static void query_free(sp_query *q) {
+  if (!q) return;
+  uint32_t *bufs[] = {q->query_ct, q->v, q->left, q->right, q->fkeys, q->fkeys_neg, q->acc, q->stage};
+  for (uint32_t *b : bufs)
+    if (b) (void)hipFree(b);
+  delete q;
+}
+
+extern "C" int sp_query_create(sp_db *db, int t_gsw, int t_exp, sp_query **out) {
+  if (!db || !out) return fail(SP_ERR_ARG, "sp_query_create: NULL");
+  *out = nullptr;
+  if (t_gsw < 2 || t_gsw > 28 || t_exp < 2 || t_exp > 56 || (t_exp & 1))
+    return fail(SP_ERR_ARG, "sp_query_create: t_gsw %d / t_exp %d unsupported", t_gsw, t_exp);
+  int g = 0, stop = 0;
+  int rc = sp_expansion_depth(db->nu1, db->nu2, t_gsw, &g, &stop);
+  if (rc) return rc;
+  if (stop < 1) return fail(SP_ERR_ARG, "sp_query_create: stop_round must be >= 1");
+  if ((1 << g) < 2 * (1 << db->nu1)) return fail(SP_ERR_ARG, "sp_query_create: expansion too shallow");
+  sp_ctx *ctx = db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  sp_query *q = new sp_query();
+  q->db = db; q->t_gsw = t_gsw; q->t_exp = t_exp; q->g = g; q->stop = stop;
+  const int C = 1 << db->nu2;
+  const size_t kl = (size_t)2 * t_exp * 2 * N;
+  const size_t kf = (size_t)2 * (2 * t_gsw) * 2 * N;
+  q->stage_words = std::max<size_t>({(size_t)g * kl, (size_t)db->nu2 * kf, (size_t)db->n2 * C * 4 * N,
+                                     ((size_t)1 << g) * 4 * N});
+  hipError_t e = hipSuccess;
+  auto A = [&](uint32_t **p, size_t words) {
+    if (e == hipSuccess) e = hipMalloc(p, words * 4);
+  };
+  A(&q->query_ct, (size_t)4 * N);
+  A(&q->v, ((size_t)1 << g) * 4 * N);
+  A(&q->left, (size_t)g * kl);
+  A(&q->right, (size_t)(stop + 1) * kl);
+  A(&q->fkeys, (size_t)db->nu2 * kf);
+  A(&q->acc, (size_t)db->n2 * C * 4 * N);
+  A(&q->stage, q->stage_words);
+  if (e != hipSuccess) {
+    query_free(q);
+    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_query_create: %s", hipGetErrorString(e));
+  }
+  *out = q;
+  return SP_OK;
+}
+
+extern "C" int sp_query_destroy(sp_query *q) {
+  if (!q) return SP_OK;
+  (void)hipSetDevice(q->db->ctx->device);
+  (void)hipStreamSynchronize(q->db->ctx->stream);
+  query_free(q);
+  return SP_OK;
+}
+
+// Host -> device, natural layout -> internal P (2^32-scaled when mont).
+static int upload_p(sp_query *q, const uint32_t *host, uint32_t *dst, size_t n_polys, int m_div, int mont) {
+  sp_ctx *ctx = q->db->ctx;
+  if (n_polys * N > q->stage_words) return fail(SP_ERR_ARG, "upload_p: staging too small");
+  HIP_TRY(hipMemcpyAsync(q->stage, host, n_polys * N * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(launch_nat_to_p(q->stage, dst, n_polys, m_div, mont, ctx->tables, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SP_OK;
+}
+
+extern "C" int sp_query_load(sp_query *q, const uint32_t *host_query, const uint32_t *host_left,
+                             const uint32_t *host_right, const uint32_t *host_v_folding,
+                             const uint32_t *host_v_folding_neg) {
+  if (!q || !host_query || !host_left || !host_right || !host_v_folding) return fail(SP_ERR_ARG, "sp_query_load: NULL");
+  sp_ctx *ctx = q->db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t kl_polys = (size_t)2 * q->t_exp * 2;
+  const size_t kf_polys = (size_t)2 * (2 * q->t_gsw) * 2;
+  int rc;
+  if ((rc = upload_p(q, host_query, q->query_ct, 4, 1, 0))) return rc;
+  if ((rc = upload_p(q, host_left, q->left, (size_t)q->g * kl_polys, 2 * q->t_exp, 1))) return rc;
+  if ((rc = upload_p(q, host_right, q->right, (size_t)(q->stop + 1) * kl_polys, 2 * q->t_exp, 1))) return rc;
+  if ((rc = upload_p(q, host_v_folding, q->fkeys, (size_t)q->db->nu2 * kf_polys, 4 * q->t_gsw, 1))) return rc;
+  q->fast_fold = host_v_folding_neg == nullptr;
+  if (!q->fast_fold) {
+    if (!q->fkeys_neg) HIP_TRY(hipMalloc(&q->fkeys_neg, (size_t)q->db->nu2 * kf_polys * N * 4));
+    if ((rc = upload_p(q, host_v_folding_neg, q->fkeys_neg, (size_t)q->db->nu2 * kf_polys, 4 * q->t_gsw, 1)))
+      return rc;
+  }
+  return SP_OK;
+}
+
+// Stage bits: 1 = expansion, 2 = database multiply, 4 = from_ntt + fold.
+extern "C" int sp_query_run(sp_query *q, int stages, void *stream) {
+  if (!q) return fail(SP_ERR_ARG, "sp_query_run: NULL");
+  sp_db *db = q->db;
+  sp_ctx *ctx = db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const int R = 1 << db->nu1, C = 1 << db->nu2;
+  int rc;
+  if (stages & 1) {
+    // Expansion works in place on v; v[0] starts as the stored query.
+    HIP_TRY(hipMemcpyAsync(q->v, q->query_ct, (size_t)4 * N * 4, hipMemcpyDeviceToDevice, s));
+    rc = run_expansion(ctx, q->v, 0, 1, db->nu1, db->nu2, q->t_gsw, q->t_exp, q->left, q->right, 0, 0, -1, s);
+    if (rc) return rc;
+  }
+  if (stages & 2) HIP_TRY(launch_db_mul(db->d_db, q->v, 2 * 4 * N, q->acc, db->n2, R, C, ctx->tables, s));
+  if (stages & 4) {
+    HIP_TRY(launch_from_ntt(true, q->acc, q->acc, (size_t)db->n2 * C * 2, 2 * N, 2 * N, ctx->tables, s));
+    const int mx = 2 * q->t_gsw;
+    const size_t level = (size_t)2 * mx * 2 * N;
+    int num_per = C;
+    for (int cur = 0; cur < db->nu2; cur++) {
+      num_per >>= 1;
+      const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
+      HIP_TRY(launch_fold_level(q->fast_fold, q->acc, (size_t)C * 4 * N, db->n2, 1, num_per, q->fkeys + off,
+                                q->fast_fold ? nullptr : q->fkeys_neg + off, 0, q->t_gsw, gadget_bits(q->t_gsw),
+                                ctx->tables, s));
+    }
+  }
+  (void)R;
+  return SP_OK;
+}
+
+// Debug / parity readouts.  what: 0 = result (n2 raw Ciphertexts),
+// 1 = expanded v (2^g NTT Ciphertexts, natural order),
+// 2 = database-multiply accumulators (n2*C NTT Ciphertexts, natural order;
+//     valid after stage 2 and before stage 4),
+// 3 = all n2*C raw accumulators.
+extern "C" int sp_query_read(sp_query *q, int what, uint32_t *host_out) {
+  if (!q || !host_out) return fail(SP_ERR_ARG, "sp_query_read: NULL");
+  sp_db *db = q->db;
+  sp_ctx *ctx = db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int C = 1 << db->nu2;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (what == 0) {
+    for (int t = 0; t < db->n2; t++)
+      HIP_TRY(hipMemcpy(host_out + (size_t)t * 4 * N, q->acc + (size_t)t * C * 4 * N, 4 * N * 4, hipMemcpyDeviceToHost));
+  } else if (what == 1) {
+    const size_t n = ((size_t)1 << q->g) * 4;
+    HIP_TRY(launch_p_to_nat(q->v, q->stage, n, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host_out, q->stage, n * N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  } else if (what == 2) {
+    const size_t n = (size_t)db->n2 * C * 4;
+    HIP_TRY(launch_p_to_nat(q->acc, q->stage, n, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host_out, q->stage, n * N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  } else if (what == 3) {
+    HIP_TRY(hipMemcpyAsync(host_out, q->acc, (size_t)db->n2 * C * 4 * N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    return fail(SP_ERR_ARG, "sp_query_read: what = %d", what);
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SP_OK;
+}
+
+extern "C" int sp_answer(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_query, const uint32_t *host_left,
+                         const uint32_t *host_right, const uint32_t *host_v_folding, uint32_t *host_out) {
+  sp_query *q = nullptr;
+  int rc = sp_query_create(db, t_gsw, t_exp, &q);
+  if (rc) return rc;
+  rc = sp_query_load(q, host_query, host_left, host_right, host_v_folding, nullptr);
+  if (!rc) rc = sp_query_run(q, 7, nullptr);
+  if (!rc) rc = sp_query_read(q, 0, host_out);
+  sp_query_destroy(q);
+  return rc;
+}

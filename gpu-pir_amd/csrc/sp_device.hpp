@@ -1,0 +1,266 @@
+// sp_device.hpp — device-side building blocks for the Spiral PIR answer path
+// on CDNA4 (gfx950): modular arithmetic on the two 28-bit CRT primes, the
+// wave-level 2048-point negacyclic NTT/INTT, and the library's internal
+// NTT-domain layout.
+//
+// Mathematics follows EricArcherman/GPU-PIR (gpu-pir/ntt_28bit.cuh,
+// arith.cuh, gadget.cuh); the schedule is this library's own:
+//
+//  * One wavefront (64 lanes) owns one 2048-point transform; each lane holds
+//    32 coefficients in VGPRs.  No workgroup barrier is needed inside a
+//    transform: the single index transpose goes through a wave-private LDS
+//    tile (2112 words: 2 pad words per 64 keep both access patterns free of
+//    bank conflicts) and the one cross-lane butterfly stage uses a DPP
+//    neighbour swap.
+//  * Butterflies are Harvey's lazy Cooley-Tukey / Gentleman-Sande forms with
+//    Shoup multiplication (w, w' = floor(w * 2^32 / q)): 3 integer multiplies
+//    per butterfly.  Forward values live in [0, 4q), inverse in [0, 2q).
+//
+// Index bookkeeping (N = 2048, index bits b10..b0):
+//   phase A   lane l, register i  <->  coefficient 64*i + l
+//   phase B   lane l = 2L + b, register i <-> index 64*L + 2*i + b
+//   NTT side  lane l, register r  <->  NTT position 32*l + r
+// Internal NTT-domain layout "P" stores position z = 32*l + 4*c + e at word
+// 256*c + 4*l + e, so that chunk c of a lane's 32 positions is one coalesced
+// 16-byte load across the wave.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sp {
+
+constexpr int N = 2048;
+constexpr int LOGN = 11;
+constexpr uint32_t Q0 = 268369921u;  // gpu-pir/params.cuh:25
+constexpr uint32_t Q1 = 249561089u;  // gpu-pir/params.cuh:28
+constexpr uint64_t QQ = (uint64_t)Q0 * (uint64_t)Q1;
+constexpr int LDS_TILE = 2112;  // words of wave-private transpose scratch
+
+// Twiddle table per (residue, direction), u32 words:
+//   [0, 64)          phase-A twiddles: (w, w') for index 1..31, uniform per wave
+//   [64, 64+47*128)  phase-B slots: slot s -> w at 64 + (2s)*64 + lane,
+//                    w' at 64 + (2s+1)*64 + lane
+constexpr int TW_A_WORDS = 64;
+constexpr int TW_B_SLOTS = 47;
+constexpr int TW_WORDS = TW_A_WORDS + TW_B_SLOTS * 2 * 64;  // 6080
+
+// Per-residue constants (device copy lives in the context's constant block).
+struct ModConst {
+  uint32_t q;
+  uint32_t qinv_neg;    // -q^-1 mod 2^32  (Montgomery REDC)
+  uint32_t r2;          // 2^64 mod q
+  uint32_t crt_c;       // N^-1 * (q_other^-1 mod q) mod q
+  uint32_t crt_cp;      // Shoup companion of crt_c
+  uint32_t ninv;        // N^-1 mod q
+  uint32_t ninv_p;      // Shoup companion
+  uint32_t two32;       // 2^32 mod q
+  uint32_t two32_p;     // Shoup companion of two32
+  uint32_t mu1;         // floor(2^32 / q)  (Shoup companion of 1)
+};
+
+__device__ __forceinline__ uint32_t shoup_mul(uint32_t y, uint32_t w, uint32_t wp, uint32_t q) {
+  // y * w mod q in [0, 2q) for any 32-bit y and w < q (Shoup / Harvey).
+  const uint32_t hi = __umulhi(y, wp);
+  return y * w - hi * q;
+}
+
+__device__ __forceinline__ uint32_t reduce_2q(uint32_t x, uint32_t two_q) {
+  // [0, 4q) -> [0, 2q): min(x, x - 2q) as unsigned.
+  return min(x, x - two_q);
+}
+
+__device__ __forceinline__ uint32_t canon(uint32_t x, uint32_t q) {
+  // [0, 2q) -> [0, q)
+  return min(x, x - q);
+}
+
+__device__ __forceinline__ uint32_t canon4(uint32_t x, uint32_t q) {
+  // [0, 4q) -> [0, q)
+  x = min(x, x - 2 * q);
+  return min(x, x - q);
+}
+
+__device__ __forceinline__ void ct_bf(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
+  // Harvey forward butterfly: inputs [0, 4q), outputs [0, 4q).
+  const uint32_t tq = 2 * q;
+  const uint32_t xr = reduce_2q(x, tq);
+  const uint32_t t = shoup_mul(y, w, wp, q);
+  y = xr - t + tq;
+  x = xr + t;
+}
+
+__device__ __forceinline__ void gs_bf(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
+  // Harvey inverse butterfly: inputs [0, 2q), outputs [0, 2q).
+  const uint32_t tq = 2 * q;
+  const uint32_t u = reduce_2q(x + y, tq);
+  const uint32_t v = x - y + tq;
+  y = shoup_mul(v, w, wp, q);
+  x = u;
+}
+
+// Montgomery reduction of a 64-bit accumulator: returns acc * 2^-32 mod q,
+// canonical.  Valid for acc < 2^62 (result < 2^30 + q < 5q before the final
+// corrections).
+__device__ __forceinline__ uint32_t redc64(uint64_t acc, uint32_t q, uint32_t qinv_neg) {
+  const uint32_t m = (uint32_t)acc * qinv_neg;
+  const uint64_t t = (acc + (uint64_t)m * q) >> 32;
+  uint32_t r = (uint32_t)t;
+  r = min(r, r - 4 * q);
+  r = min(r, r - 2 * q);
+  return min(r, r - q);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  // Orders this wave's LDS traffic (LDS instructions of one wave execute in
+  // issue order; the fences stop the compiler from moving accesses across).
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t swap_lane_pair(uint32_t v) {
+  // Value of lane (l ^ 1): DPP quad_perm [1,0,3,2].
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t tw_a(const uint32_t *__restrict__ tw, int idx, int k) {
+  return tw[idx * 2 + k];
+}
+__device__ __forceinline__ uint32_t tw_b(const uint32_t *__restrict__ tw, int slot, int k, int lane) {
+  return tw[TW_A_WORDS + (slot * 2 + k) * 64 + lane];
+}
+
+// Forward negacyclic NTT of one polynomial held by one wave.
+// In:  r[i] = coefficient 64*i + lane, values in [0, 4q).
+// Out: r[j] = NTT position 32*lane + j (natural Spiral order), in [0, 4q).
+__device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
+                                             uint32_t *__restrict__ lds, uint32_t q, int lane) {
+  // Phase A: strides 1024 .. 64 (index bits 10..6 live in the register index).
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const int half = 16 >> s;
+#pragma unroll
+    for (int blk = 0; blk < (1 << s); blk++) {
+      const uint32_t w = tw_a(tw, (1 << s) + blk, 0), wp = tw_a(tw, (1 << s) + blk, 1);
+#pragma unroll
+      for (int i = 0; i < half; i++) ct_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+    }
+  }
+  // Transpose to phase B through the wave's LDS tile.
+#pragma unroll
+  for (int i = 0; i < 32; i++) lds[66 * i + lane] = r[i];
+  wave_lds_sync();
+  const int L = lane >> 1, b = lane & 1;
+#pragma unroll
+  for (int i = 0; i < 32; i++) r[i] = lds[66 * L + 2 * i + b];
+  wave_lds_sync();
+  // Phase B: strides 32 .. 2 in registers.
+  int slot = 0;
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const int half = 16 >> s;
+#pragma unroll
+    for (int blk = 0; blk < (1 << s); blk++) {
+      const uint32_t w = tw_b(tw, slot + blk, 0, lane), wp = tw_b(tw, slot + blk, 1, lane);
+#pragma unroll
+      for (int i = 0; i < half; i++) ct_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+    }
+    slot += (1 << s);
+  }
+  // Stride 1 crosses the lane pair (2L, 2L+1): each lane keeps 16 of the 32
+  // butterflies after a half exchange and ends holding 32 consecutive outputs.
+  uint32_t o[32];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t send = b ? r[k] : r[16 + k];
+    const uint32_t recv = swap_lane_pair(send);
+    uint32_t x = b ? recv : r[k];
+    uint32_t y = b ? r[16 + k] : recv;
+    ct_bf(x, y, tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
+    o[2 * k] = x;
+    o[2 * k + 1] = y;
+  }
+#pragma unroll
+  for (int j = 0; j < 32; j++) r[j] = o[j];
+}
+
+// Inverse negacyclic NTT (without the factor N^-1) of one polynomial.
+// In:  r[j] = NTT position 32*lane + j, values in [0, 2q).
+// Out: r[i] = N * coefficient (64*i + lane) mod q, values in [0, 2q).
+__device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
+                                             uint32_t *__restrict__ lds, uint32_t q, int lane) {
+  const int L = lane >> 1, b = lane & 1;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    gs_bf(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
+  uint32_t n[32];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t send = b ? r[2 * k] : r[2 * k + 1];
+    const uint32_t recv = swap_lane_pair(send);
+    n[k] = b ? recv : r[2 * k];
+    n[16 + k] = b ? r[2 * k + 1] : recv;
+  }
+  // Strides 2 .. 32 in registers.
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const int half = 1 << s;
+    const int nblk = 16 >> s;
+#pragma unroll
+    for (int blk = 0; blk < nblk; blk++) {
+      const uint32_t w = tw_b(tw, nblk - 1 + blk, 0, lane), wp = tw_b(tw, nblk - 1 + blk, 1, lane);
+#pragma unroll
+      for (int i = 0; i < half; i++) gs_bf(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 32; i++) lds[66 * L + 2 * i + b] = n[i];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < 32; i++) r[i] = lds[66 * i + lane];
+  wave_lds_sync();
+  // Phase A: strides 64 .. 1024.
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const int half = 1 << s;
+    const int nblk = 16 >> s;
+#pragma unroll
+    for (int blk = 0; blk < nblk; blk++) {
+      const uint32_t w = tw_a(tw, nblk + blk, 0), wp = tw_a(tw, nblk + blk, 1);
+#pragma unroll
+      for (int i = 0; i < half; i++) gs_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+    }
+  }
+}
+
+// ---- internal NTT-domain layout P --------------------------------------
+__device__ __forceinline__ void load_p(uint32_t (&r)[32], const uint32_t *__restrict__ base, int lane) {
+  const uint4 *b4 = reinterpret_cast<const uint4 *>(base);
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const uint4 v = b4[c * 64 + lane];
+    r[4 * c] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
+  }
+}
+__device__ __forceinline__ void store_p(uint32_t *__restrict__ base, const uint32_t (&r)[32], int lane) {
+  uint4 *b4 = reinterpret_cast<uint4 *>(base);
+#pragma unroll
+  for (int c = 0; c < 8; c++) b4[c * 64 + lane] = make_uint4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+}
+__host__ __device__ __forceinline__ int p_of_z(int z) {
+  // natural NTT position -> internal word
+  const int l = z >> 5, c = (z >> 2) & 7, e = z & 3;
+  return 256 * c + 4 * l + e;
+}
+__host__ __device__ __forceinline__ int z_of_p(int p) {
+  const int c = p >> 8, l = (p >> 2) & 63, e = p & 3;
+  return 32 * l + 4 * c + e;
+}
+
+// phase-A coefficient load/store (natural coefficient order, coalesced).
+__device__ __forceinline__ void load_coeffs(uint32_t (&r)[32], const uint32_t *__restrict__ base, int lane) {
+#pragma unroll
+  for (int i = 0; i < 32; i++) r[i] = base[64 * i + lane];
+}
+
+}  // namespace sp

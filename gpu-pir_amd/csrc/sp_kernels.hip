@@ -1,0 +1,750 @@
+// sp_kernels.hip — CDNA4 kernels of the Spiral PIR answer path.
+//
+// All kernels compute exact modular integer arithmetic; every output is a
+// canonical residue (or a canonical raw value in [0, Q]) so results compare
+// bit for bit with the reference semantics.  See sp_device.hpp for the NTT
+// schedule and the internal NTT-domain layout P.
+#include "sp_kernels.hpp"
+
+namespace sp {
+
+// ---------------------------------------------------------------------------
+// Boundary transforms (natural order in and out)
+// ---------------------------------------------------------------------------
+template <bool INV>
+__global__ __launch_bounds__(256) void k_ntt_natural(uint32_t *__restrict__ polys, size_t n,
+                                                     const uint32_t *__restrict__ tw, ModConst mc,
+                                                     int scale) {
+  __shared__ uint32_t lds_all[4 * LDS_TILE];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  uint32_t *lds = lds_all + wave * LDS_TILE;
+  const uint32_t q = mc.q;
+  for (size_t pi = (size_t)blockIdx.x * 4 + wave; pi < n; pi += (size_t)gridDim.x * 4) {
+    uint32_t *p = polys + pi * N;
+    uint32_t r[32];
+    if (!INV) {
+      load_coeffs(r, p, lane);
+      ntt_fwd_wave(r, tw, lds, q, lane);
+      uint4 *o = reinterpret_cast<uint4 *>(p + 32 * lane);
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        o[c] = make_uint4(canon4(r[4 * c], q), canon4(r[4 * c + 1], q), canon4(r[4 * c + 2], q),
+                          canon4(r[4 * c + 3], q));
+    } else {
+      const uint4 *in = reinterpret_cast<const uint4 *>(p + 32 * lane);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint4 v = in[c];
+        r[4 * c] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
+      }
+      ntt_inv_wave(r, tw, lds, q, lane);
+#pragma unroll
+      for (int i = 0; i < 32; i++) {
+        uint32_t x = r[i];
+        if (scale) x = shoup_mul(x, mc.ninv, mc.ninv_p, q);
+        p[64 * i + lane] = canon(x, q);
+      }
+    }
+  }
+}
+
+// NTT Poly (natural positions, or layout P) -> raw Poly.  One poly per
+// 128-thread block; wave m inverts residue m, the CRT pass is split.
+template <bool LP>
+__global__ __launch_bounds__(128) void k_from_ntt(const uint32_t *__restrict__ ntt,
+                                                  uint32_t *__restrict__ raw, size_t n,
+                                                  size_t in_stride, size_t out_stride,
+                                                  const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                  ModConst mc1) {
+  __shared__ uint32_t lds_all[2 * LDS_TILE];
+  __shared__ uint32_t xm[2][N];
+  const int m = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  uint32_t *lds = lds_all + m * LDS_TILE;
+  const ModConst mc = m ? mc1 : mc0;
+  const uint32_t *tw = tw_base + (m * 2 + 1) * TW_WORDS;
+  for (size_t pi = blockIdx.x; pi < n; pi += gridDim.x) {
+    const uint32_t *src = ntt + pi * in_stride + m * N;
+    uint32_t r[32];
+    if (LP) {
+      load_p(r, src, lane);
+    } else {
+      const uint4 *in = reinterpret_cast<const uint4 *>(src + 32 * lane);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint4 v = in[c];
+        r[4 * c] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
+      }
+    }
+    ntt_inv_wave(r, tw, lds, mc.q, lane);
+#pragma unroll
+    for (int i = 0; i < 32; i++) xm[m][64 * i + lane] = canon(shoup_mul(r[i], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+    __syncthreads();
+    uint32_t *dst = raw + pi * out_stride;
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+      const int z = 64 * (16 * m + ii) + lane;
+      uint64_t x = (uint64_t)xm[0][z] * Q1 + (uint64_t)xm[1][z] * Q0;
+      x = (x >= QQ) ? x - QQ : x;
+      dst[z] = (uint32_t)(x >> 32);
+      dst[N + z] = (uint32_t)x;
+    }
+    __syncthreads();
+  }
+}
+
+// x (raw, given as hi/lo) mod q, canonical.  hi < 2^32, lo < 2^32.
+__device__ __forceinline__ uint32_t raw_mod(uint32_t hi, uint32_t lo, const ModConst &mc) {
+  const uint32_t t1 = shoup_mul(hi, mc.two32, mc.two32_p, mc.q);  // hi * 2^32 mod q, [0, 2q)
+  const uint32_t t2 = lo - __umulhi(lo, mc.mu1) * mc.q;           // lo mod q, [0, 2q)
+  return canon4(t1 + t2, mc.q);
+}
+
+// raw Poly -> NTT Poly (natural positions).
+__global__ __launch_bounds__(128) void k_to_ntt(const uint32_t *__restrict__ raw, uint32_t *__restrict__ ntt,
+                                                size_t n, const uint32_t *__restrict__ tw_base,
+                                                ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t lds_all[2 * LDS_TILE];
+  const int m = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  uint32_t *lds = lds_all + m * LDS_TILE;
+  const ModConst mc = m ? mc1 : mc0;
+  const uint32_t *tw = tw_base + (m * 2 + 0) * TW_WORDS;
+  for (size_t pi = blockIdx.x; pi < n; pi += gridDim.x) {
+    const uint32_t *src = raw + pi * 2 * N;
+    uint32_t r[32];
+#pragma unroll
+    for (int i = 0; i < 32; i++) r[i] = raw_mod(src[64 * i + lane], src[N + 64 * i + lane], mc);
+    __syncthreads();  // both waves read both planes before either overwrites one
+    ntt_fwd_wave(r, tw, lds, mc.q, lane);
+    uint4 *o = reinterpret_cast<uint4 *>(ntt + pi * 2 * N + m * N + 32 * lane);
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      o[c] = make_uint4(canon4(r[4 * c], mc.q), canon4(r[4 * c + 1], mc.q), canon4(r[4 * c + 2], mc.q),
+                        canon4(r[4 * c + 3], mc.q));
+    __syncthreads();
+  }
+}
+
+// automorph kernel (gpu-pir/automorph.cuh:54-62 semantics) on u64 coefficients.
+__global__ __launch_bounds__(256) void k_automorph_u64(const uint64_t *__restrict__ in,
+                                                       uint64_t *__restrict__ out, size_t n, uint32_t t) {
+  const size_t total = n * N;
+  for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
+    const size_t poly = g / N;
+    const uint64_t i = g % N;
+    const uint64_t it = i * t;
+    const uint64_t x = in[g];
+    const uint64_t idx = it & (N - 1);
+    out[poly * N + idx] = ((it >> LOGN) & 1) ? QQ - x : x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gadget external product (boundary form: natural keys in standard form)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t digit_of(uint32_t hi, uint32_t lo, int shift, uint32_t mask) {
+  const uint64_t x = ((uint64_t)hi << 32) | lo;
+  return shift >= 64 ? 0u : (uint32_t)((x >> shift) & mask);
+}
+
+__global__ __launch_bounds__(256) void k_extprod(const uint32_t *__restrict__ in, int rdim, int mx, int bits,
+                                                 const uint32_t *__restrict__ keys, uint32_t *__restrict__ acc,
+                                                 size_t n, size_t in_stride, size_t key_stride, size_t acc_stride,
+                                                 const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t lds_all[4 * LDS_TILE];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int m = wave >> 1, h = wave & 1;
+  uint32_t *lds = lds_all + wave * LDS_TILE;
+  const ModConst mc = m ? mc1 : mc0;
+  const uint32_t q = mc.q;
+  const uint32_t *tw = tw_base + (m * 2 + 0) * TW_WORDS;
+  const uint32_t mask = (bits >= 32) ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+  for (size_t it = blockIdx.x; it < n; it += gridDim.x) {
+    uint64_t a0[32], a1[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) { a0[j] = 0; a1[j] = 0; }
+    for (int col = h; col < mx; col += 2) {
+      const int j = col % rdim, k = col / rdim;
+      const uint32_t *hi = in + it * in_stride + (size_t)j * 2 * N;
+      const uint32_t *lo = hi + N;
+      uint32_t r[32];
+#pragma unroll
+      for (int i = 0; i < 32; i++) r[i] = digit_of(hi[64 * i + lane], lo[64 * i + lane], k * bits, mask);
+      ntt_fwd_wave(r, tw, lds, q, lane);
+      const uint32_t *k0 = keys + it * key_stride + ((size_t)(m * mx + col) * 2 + 0) * N + 32 * lane;
+      const uint32_t *k1 = k0 + N;
+#pragma unroll
+      for (int jj = 0; jj < 32; jj++) {
+        const uint32_t d = reduce_2q(r[jj], 2 * q);
+        a0[jj] += (uint64_t)d * k0[jj];
+        a1[jj] += (uint64_t)d * k1[jj];
+      }
+    }
+    uint32_t own[32];
+    __syncthreads();  // every wave is done with its transpose tile
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+      const uint32_t v0 = shoup_mul(redc64(a0[jj], q, mc.qinv_neg), mc.two32, mc.two32_p, q);
+      const uint32_t v1 = shoup_mul(redc64(a1[jj], q, mc.qinv_neg), mc.two32, mc.two32_p, q);
+      own[jj] = canon(h ? v1 : v0, q);
+      lds_all[wave * N + 32 * lane + jj] = canon(h ? v0 : v1, q);
+    }
+    __syncthreads();
+    uint32_t *dst = acc + it * acc_stride + (size_t)(h * 2 + m) * N + 32 * lane;
+    const uint32_t *other = lds_all + (wave ^ 1) * N + 32 * lane;
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+      const uint32_t s = canon(own[jj] + other[jj], q);
+      dst[jj] = canon(s + dst[jj], q);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion natural <-> P (optionally scaling residues by 2^32)
+// ---------------------------------------------------------------------------
+// Polys are indexed k = 0..n-1; poly k is residue (k / m_div) & 1.
+__global__ __launch_bounds__(256) void k_nat_to_p(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                  size_t n, int m_div, int mont, ModConst mc0, ModConst mc1) {
+  const size_t total = n * N;
+  for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
+    const size_t k = g / N;
+    const int p = (int)(g % N);
+    uint32_t x = in[k * N + z_of_p(p)];
+    if (mont) {
+      const ModConst &mc = ((k / m_div) & 1) ? mc1 : mc0;
+      x = canon(shoup_mul(x, mc.two32, mc.two32_p, mc.q), mc.q);
+    }
+    out[g] = x;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_p_to_nat(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                  size_t n) {
+  const size_t total = n * N;
+  for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
+    const size_t k = g / N;
+    const int z = (int)(g % N);
+    out[g] = in[k * N + p_of_z(z)];
+  }
+}
+
+// Reference Database layout [c][j][m][z] -> internal [m][j][c][p] (one trial).
+__global__ __launch_bounds__(256) void k_db_from_ref(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                     int R, int C) {
+  const size_t total = (size_t)2 * R * C * N;
+  for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
+    const int p = (int)(g % N);
+    size_t rest = g / N;
+    const int c = (int)(rest % C); rest /= C;
+    const int j = (int)(rest % R); rest /= R;
+    const int m = (int)rest;
+    out[g] = in[(((size_t)c * R + j) * 2 + m) * N + z_of_p(p)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Database ingest: plaintext bytes -> NTT residues, internal layout
+//   db[t][m][j][c][p], record index = j * C + c.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ bytes, size_t rec0, size_t nrec,
+                                                   uint32_t *__restrict__ db, int n2, int R, int C,
+                                                   const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                   ModConst mc1) {
+  __shared__ uint32_t lds_all[4 * LDS_TILE];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  uint32_t *lds = lds_all + wave * LDS_TILE;
+  const size_t total = nrec * n2 * 2;
+  for (size_t w = (size_t)blockIdx.x * 4 + wave; w < total; w += (size_t)gridDim.x * 4) {
+    const int m = (int)(w & 1);
+    const size_t pt = w >> 1;  // (record, trial) within the chunk
+    const int t = (int)(pt % n2);
+    const size_t rl = pt / n2;
+    const size_t rec = rec0 + rl;
+    const int j = (int)(rec / C), c = (int)(rec % C);
+    const uint8_t *src = bytes + pt * N;
+    const ModConst mc = m ? mc1 : mc0;
+    uint32_t r[32];
+#pragma unroll
+    for (int i = 0; i < 32; i++) r[i] = src[64 * i + lane];
+    ntt_fwd_wave(r, tw_base + (m * 2) * TW_WORDS, lds, mc.q, lane);
+#pragma unroll
+    for (int i = 0; i < 32; i++) r[i] = canon4(r[i], mc.q);
+    store_p(db + ((((size_t)t * 2 + m) * R + j) * C + c) * N, r, lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// First-dimension DB multiply (single query), internal layouts:
+//   db  [T][m][R][C][P]      sel j at sel + j*sel_stride: [row][m][P]
+//   out [T][C][row][m][P]
+// Block = 4 waves splitting the R rows; a lane owns 2 positions and CG
+// columns, accumulating sel*db lazily in u64 (folded every 128 rows).
+// Blocks are numbered so that the 8 XCDs each own a quarter of the
+// (residue, position-block) selector groups, keeping selector re-reads in
+// one XCD's L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t fold32(uint64_t x, uint32_t two32) {
+  return (x >> 32) * (uint64_t)two32 + (x & 0xFFFFFFFFull);
+}
+__device__ __forceinline__ uint32_t mod_u64(uint64_t x, const ModConst &mc) {
+  x = fold32(x, mc.two32);
+  x = fold32(x, mc.two32);
+  x = fold32(x, mc.two32);
+  const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+  uint32_t t = canon(lo - __umulhi(lo, mc.mu1) * mc.q, mc.q);
+  t += hi ? mc.two32 : 0u;
+  return canon(t, mc.q);
+}
+
+template <int CG>
+__global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db, const uint32_t *__restrict__ sel,
+                                                size_t sel_stride, uint32_t *__restrict__ out, int T, int R, int C,
+                                                ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t red[4][CG][2][128];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ncg = C / CG;
+  const int bid = blockIdx.x;
+  const int xcd = bid & 7, loc = bid >> 3;
+  const int sg = xcd * 4 + (loc & 3);
+  const int rest = loc >> 2;
+  const int tt = rest % T, cg = rest / T;
+  if (cg >= ncg) return;
+  const int m = sg >> 4, pb = sg & 15;
+  const ModConst mc = m ? mc1 : mc0;
+  const int p0 = pb * 128 + lane * 2;
+  const int rows_per_wave = R / 4;
+  const int j0 = wave * rows_per_wave;
+
+  uint64_t acc[CG][2][2];
+#pragma unroll
+  for (int c = 0; c < CG; c++)
+#pragma unroll
+    for (int pp = 0; pp < 2; pp++) { acc[c][pp][0] = 0; acc[c][pp][1] = 0; }
+
+  const uint32_t *dbase = db + ((((size_t)tt * 2 + m) * R) * C + (size_t)cg * CG) * N + p0;
+  for (int jb = 0; jb < rows_per_wave; jb += 128) {
+    const int jend = min(rows_per_wave, jb + 128);
+#pragma unroll 2
+    for (int jj = jb; jj < jend; jj++) {
+      const int j = j0 + jj;
+      const uint32_t *sp = sel + (size_t)j * sel_stride + (size_t)m * N + p0;
+      const uint2 s0 = *reinterpret_cast<const uint2 *>(sp);
+      const uint2 s1 = *reinterpret_cast<const uint2 *>(sp + 2 * N);
+      const uint32_t *dp = dbase + (size_t)j * C * N;
+#pragma unroll
+      for (int c = 0; c < CG; c++) {
+        const uint2 d = *reinterpret_cast<const uint2 *>(dp + (size_t)c * N);
+        acc[c][0][0] += (uint64_t)s0.x * d.x;
+        acc[c][1][0] += (uint64_t)s0.y * d.y;
+        acc[c][0][1] += (uint64_t)s1.x * d.x;
+        acc[c][1][1] += (uint64_t)s1.y * d.y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CG; c++)
+#pragma unroll
+      for (int pp = 0; pp < 2; pp++) {
+        acc[c][pp][0] = fold32(acc[c][pp][0], mc.two32);
+        acc[c][pp][1] = fold32(acc[c][pp][1], mc.two32);
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < CG; c++)
+#pragma unroll
+    for (int row = 0; row < 2; row++) {
+      red[wave][c][row][lane * 2] = mod_u64(acc[c][0][row], mc);
+      red[wave][c][row][lane * 2 + 1] = mod_u64(acc[c][1][row], mc);
+    }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < CG; k++) {
+    const int idx = k * 256 + threadIdx.x;
+    const int c = idx >> 8, row = (idx >> 7) & 1, pos = idx & 127;
+    uint32_t s = red[0][c][row][pos] + red[1][c][row][pos];
+    s = canon(s, mc.q);
+    s = canon(s + red[2][c][row][pos], mc.q);
+    s = canon(s + red[3][c][row][pos], mc.q);
+    out[((((size_t)tt * C + cg * CG + c) * 2 + row) * 2 + m) * N + pb * 128 + pos] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fold: one level, one block per (pair i, trial, query).
+//   accs: raw Ciphertexts, set (query, trial) at accs + set*set_stride
+//   keys: level key matrix, layout [m][col][row][P], 2^32-scaled
+// FAST evaluates  Y + C * NTT(g^-1(X) - g^-1(Y))  (valid when
+// C_neg = G - C); otherwise  C * NTT(g^-1 X) + C_neg * NTT(g^-1 Y).
+// ---------------------------------------------------------------------------
+template <bool FAST>
+__global__ __launch_bounds__(256) void k_fold_level(uint32_t *__restrict__ accs, size_t set_stride, int n_trials,
+                                                    int num_per, const uint32_t *__restrict__ cpos,
+                                                    const uint32_t *__restrict__ cneg, size_t key_query_stride,
+                                                    int t_gsw, int bits, const uint32_t *__restrict__ tw_base,
+                                                    ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t lds_all[4 * LDS_TILE];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int m = wave >> 1, h = wave & 1;
+  uint32_t *lds = lds_all + wave * LDS_TILE;
+  const ModConst mc = m ? mc1 : mc0;
+  const uint32_t q = mc.q;
+  const int mx = 2 * t_gsw;
+  const uint32_t mask = (1u << bits) - 1u;
+  const int i = blockIdx.x;
+  const int set = blockIdx.z * n_trials + blockIdx.y;
+  uint32_t *base = accs + (size_t)set * set_stride;
+  const uint32_t *X = base + (size_t)(num_per + i) * 4 * N;
+  uint32_t *Y = base + (size_t)i * 4 * N;
+  const uint32_t *kp = cpos + (size_t)blockIdx.z * key_query_stride + (size_t)m * mx * 2 * N;
+  const uint32_t *kn = FAST ? nullptr : cneg + (size_t)blockIdx.z * key_query_stride + (size_t)m * mx * 2 * N;
+  const uint32_t *tw = tw_base + (m * 2 + 0) * TW_WORDS;
+
+  uint64_t a0[32], a1[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) { a0[j] = 0; a1[j] = 0; }
+
+  if (FAST) {
+    const uint32_t *xh = X + (size_t)h * 2 * N, *xl = xh + N;
+    const uint32_t *yh = Y + (size_t)h * 2 * N, *yl = yh + N;
+    for (int k = 0; k < t_gsw; k++) {
+      uint32_t r[32];
+#pragma unroll
+      for (int ii = 0; ii < 32; ii++) {
+        const int z = 64 * ii + lane;
+        r[ii] = digit_of(xh[z], xl[z], k * bits, mask) + q - digit_of(yh[z], yl[z], k * bits, mask);
+      }
+      ntt_fwd_wave(r, tw, lds, q, lane);
+      const uint32_t *k0 = kp + ((size_t)(h + 2 * k) * 2) * N;
+      uint32_t kv0[32], kv1[32];
+      load_p(kv0, k0, lane);
+      load_p(kv1, k0 + N, lane);
+#pragma unroll
+      for (int jj = 0; jj < 32; jj++) {
+        const uint32_t d = reduce_2q(r[jj], 2 * q);
+        a0[jj] += (uint64_t)d * kv0[jj];
+        a1[jj] += (uint64_t)d * kv1[jj];
+      }
+    }
+  } else {
+    const uint32_t *in = h ? Y : X;
+    const uint32_t *kk = h ? kn : kp;
+    for (int col = 0; col < mx; col++) {
+      const int j = col & 1, k = col >> 1;
+      const uint32_t *ih = in + (size_t)j * 2 * N, *il = ih + N;
+      uint32_t r[32];
+#pragma unroll
+      for (int ii = 0; ii < 32; ii++) {
+        const int z = 64 * ii + lane;
+        r[ii] = digit_of(ih[z], il[z], k * bits, mask);
+      }
+      ntt_fwd_wave(r, tw, lds, q, lane);
+      uint32_t kv0[32], kv1[32];
+      load_p(kv0, kk + ((size_t)col * 2) * N, lane);
+      load_p(kv1, kk + ((size_t)col * 2 + 1) * N, lane);
+#pragma unroll
+      for (int jj = 0; jj < 32; jj++) {
+        const uint32_t d = reduce_2q(r[jj], 2 * q);
+        a0[jj] += (uint64_t)d * kv0[jj];
+        a1[jj] += (uint64_t)d * kv1[jj];
+      }
+    }
+  }
+  // Row sums across the two waves of this residue.
+  uint32_t s[32];
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < 32; jj++) {
+    const uint32_t v0 = redc64(a0[jj], q, mc.qinv_neg);
+    const uint32_t v1 = redc64(a1[jj], q, mc.qinv_neg);
+    s[jj] = h ? v1 : v0;
+    lds_all[wave * N + 32 * lane + jj] = h ? v0 : v1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < 32; jj++) s[jj] = canon(s[jj] + lds_all[(wave ^ 1) * N + 32 * lane + jj], q);
+  __syncthreads();
+  ntt_inv_wave(s, tw_base + (m * 2 + 1) * TW_WORDS, lds, q, lane);
+#pragma unroll
+  for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mc.crt_c, mc.crt_cp, q), q);
+  __syncthreads();
+  // CRT: wave (m, h) finishes coefficients 64*ii + lane, ii in [16m, 16m+16).
+  uint32_t *xm = lds_all + wave * 1024;
+#pragma unroll
+  for (int ii = 0; ii < 16; ii++) xm[ii * 64 + lane] = s[16 * (1 - m) + ii];
+  __syncthreads();
+  const uint32_t *xo = lds_all + (wave ^ 2) * 1024;
+  uint32_t *yh = Y + (size_t)h * 2 * N, *yl = yh + N;
+#pragma unroll
+  for (int ii = 0; ii < 16; ii++) {
+    const int z = 64 * (16 * m + ii) + lane;
+    const uint32_t own = s[16 * m + ii], oth = xo[ii * 64 + lane];
+    const uint32_t x0 = m ? oth : own, x1 = m ? own : oth;
+    uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
+    x = (x >= QQ) ? x - QQ : x;
+    if (FAST) {
+      x += ((uint64_t)yh[z] << 32) | yl[z];
+      x = (x >= QQ) ? x - QQ : x;
+    }
+    yh[z] = (uint32_t)(x >> 32);
+    yl[z] = (uint32_t)x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Expansion
+// ---------------------------------------------------------------------------
+// v[num_in + i] = v_neg1[r] (.) v[i] for i < num_in; neg1: [m][2 (w, w')][P].
+__global__ __launch_bounds__(256) void k_expand_neg1(uint32_t *__restrict__ v, size_t q_stride, int num_in,
+                                                     const uint32_t *__restrict__ neg1, ModConst mc0,
+                                                     ModConst mc1) {
+  uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
+  const size_t total = (size_t)num_in * (4 * N / 4);
+  for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
+    const size_t i = g / (N);  // N uint4 per ciphertext
+    const int u = (int)(g % N);
+    const int word = 4 * u;
+    const int m = (word / N) & 1;
+    const int p = word % N;
+    const ModConst &mc = m ? mc1 : mc0;
+    const uint4 x = reinterpret_cast<const uint4 *>(vq + i * 4 * N)[u];
+    const uint4 w = reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + p)[0];
+    const uint4 wp = reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + N + p)[0];
+    uint4 y;
+    y.x = canon(shoup_mul(x.x, w.x, wp.x, mc.q), mc.q);
+    y.y = canon(shoup_mul(x.y, w.y, wp.y, mc.q), mc.q);
+    y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
+    y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
+    reinterpret_cast<uint4 *>(vq + ((size_t)num_in + i) * 4 * N)[u] = y;
+  }
+}
+
+__device__ __forceinline__ uint32_t brv11(uint32_t x) { return __brev(x) >> 21; }
+
+// One action_expand step (gpu-pir/expansion.cuh:87-166) for element i of
+// round r, in place on v[i] (layout P).  Block = 4 waves.
+__global__ __launch_bounds__(256) void k_expand_action(uint32_t *__restrict__ v, size_t q_stride, int r,
+                                                       int stop, int max_bits, const uint32_t *__restrict__ left,
+                                                       const uint32_t *__restrict__ right, size_t key_query_stride,
+                                                       int t_exp, int bits, const uint32_t *__restrict__ tw_base,
+                                                       ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t scratch[4 * LDS_TILE];
+  __shared__ uint64_t ta[N];
+  __shared__ uint32_t as_e[2][N];
+  const int i = blockIdx.x;
+  if (stop > 0 && r > stop && (i & 1)) return;
+  if (stop > 0 && r == stop && (i & 1) && (i / 2) >= max_bits) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  const bool use_left = (r != 0) && ((i & 1) == 0);
+  const uint32_t *W = (use_left ? left : right) + (size_t)blockIdx.y * key_query_stride;
+  uint32_t *xm = scratch + 2 * LDS_TILE;  // tiles of waves 2 and 3, unused in phase 1
+
+  // Phase 1: waves 0/1 take v_i.a to raw residues; waves 2/3 stage v_i.as_e.
+  if (wave < 2) {
+    const int m = wave;
+    const ModConst mc = m ? mc1 : mc0;
+    uint32_t rr[32];
+    load_p(rr, vi + (size_t)m * N, lane);
+    ntt_inv_wave(rr, tw_base + (m * 2 + 1) * TW_WORDS, scratch + wave * LDS_TILE, mc.q, lane);
+#pragma unroll
+    for (int ii = 0; ii < 32; ii++)
+      xm[m * N + 64 * ii + lane] = canon(shoup_mul(rr[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  } else {
+    const int m = wave - 2;
+    const uint4 *src = reinterpret_cast<const uint4 *>(vi + (size_t)(2 + m) * N);
+    uint4 *dst = reinterpret_cast<uint4 *>(as_e[m]);
+#pragma unroll
+    for (int c = 0; c < 8; c++) dst[c * 64 + lane] = src[c * 64 + lane];
+  }
+  __syncthreads();
+  // Phase 2: CRT and the raw automorphism tau_t of a (zero negates to Q).
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t z = threadIdx.x + 256 * k;
+    uint64_t x = (uint64_t)xm[z] * Q1 + (uint64_t)xm[N + z] * Q0;
+    x = (x >= QQ) ? x - QQ : x;
+    const uint32_t zt = z * t;
+    ta[zt & (N - 1)] = ((zt >> LOGN) & 1) ? QQ - x : x;
+  }
+  __syncthreads();
+  // Phase 3: gadget digits of tau(a), NTT, multiply-accumulate with W.
+  const int m = wave >> 1, h = wave & 1;
+  const ModConst mc = m ? mc1 : mc0;
+  const uint32_t q = mc.q;
+  const uint32_t mask = (1u << bits) - 1u;
+  uint64_t a0[32], a1[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) { a0[j] = 0; a1[j] = 0; }
+  const int per = t_exp / 2;
+  for (int kk = 0; kk < per + (h ? (t_exp & 1) : 0); kk++) {
+    const int k = h * per + kk;
+    uint32_t rr[32];
+    const int shift = k * bits;
+#pragma unroll
+    for (int ii = 0; ii < 32; ii++) {
+      const uint64_t x = ta[64 * ii + lane];
+      rr[ii] = shift >= 64 ? 0u : (uint32_t)((x >> shift) & mask);
+    }
+    ntt_fwd_wave(rr, tw_base + (m * 2) * TW_WORDS, scratch + wave * LDS_TILE, q, lane);
+    uint32_t kv0[32], kv1[32];
+    const uint32_t *wk = W + ((size_t)(m * t_exp + k) * 2) * N;
+    load_p(kv0, wk, lane);
+    load_p(kv1, wk + N, lane);
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+      const uint32_t d = reduce_2q(rr[jj], 2 * q);
+      a0[jj] += (uint64_t)d * kv0[jj];
+      a1[jj] += (uint64_t)d * kv1[jj];
+    }
+  }
+  uint32_t s[32];
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < 32; jj++) {
+    const uint32_t v0 = redc64(a0[jj], q, mc.qinv_neg);
+    const uint32_t v1 = redc64(a1[jj], q, mc.qinv_neg);
+    s[jj] = h ? v1 : v0;
+    scratch[wave * N + 32 * lane + jj] = h ? v0 : v1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < 32; jj++) s[jj] = canon(s[jj] + scratch[(wave ^ 1) * N + 32 * lane + jj], q);
+  // Phase 5: v_i(row h, residue m) += W g^-1(tau a)  [+ NTT(tau as_e) on row 1].
+  uint32_t vv[32];
+  uint32_t *dst = vi + (size_t)(h * 2 + m) * N;
+  load_p(vv, dst, lane);
+#pragma unroll
+  for (int jj = 0; jj < 32; jj++) {
+    uint32_t x = vv[jj] + s[jj];
+    if (h) {
+      // NTT-domain automorphism: position z of tau(f) is position z' of f,
+      // 2*brv(z')+1 = t*(2*brv(z)+1) mod 2N.
+      const uint32_t z = 32 * lane + jj;
+      const uint32_t e = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
+      const uint32_t zs = brv11((e - 1) >> 1);
+      x += as_e[m][p_of_z((int)zs)];
+    }
+    x = min(x, x - 2 * q);
+    vv[jj] = canon(x, q);
+  }
+  store_p(dst, vv, lane);
+}
+
+}  // namespace sp
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+namespace sp {
+
+static inline unsigned grid_for(size_t items, size_t per_block, unsigned cap = 65535u * 4) {
+  size_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+hipError_t launch_ntt_natural(bool inv, uint32_t *polys, size_t n, const DevTables &t, int m, int scale,
+                              hipStream_t s) {
+  const unsigned g = grid_for(n, 4);
+  if (inv)
+    k_ntt_natural<true><<<g, 256, 0, s>>>(polys, n, tw_of(t, m, 1), t.mc[m], scale);
+  else
+    k_ntt_natural<false><<<g, 256, 0, s>>>(polys, n, tw_of(t, m, 0), t.mc[m], scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_from_ntt(bool layout_p, const uint32_t *ntt, uint32_t *raw, size_t n, size_t in_stride,
+                           size_t out_stride, const DevTables &t, hipStream_t s) {
+  const unsigned g = grid_for(n, 1);
+  if (layout_p)
+    k_from_ntt<true><<<g, 128, 0, s>>>(ntt, raw, n, in_stride, out_stride, t.tw, t.mc[0], t.mc[1]);
+  else
+    k_from_ntt<false><<<g, 128, 0, s>>>(ntt, raw, n, in_stride, out_stride, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_ntt(const uint32_t *raw, uint32_t *ntt, size_t n, const DevTables &t, hipStream_t s) {
+  k_to_ntt<<<grid_for(n, 1), 128, 0, s>>>(raw, ntt, n, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_automorph_u64(const uint64_t *in, uint64_t *out, size_t n, uint32_t tt, hipStream_t s) {
+  k_automorph_u64<<<grid_for(n * N, 256), 256, 0, s>>>(in, out, n, tt);
+  return hipGetLastError();
+}
+
+hipError_t launch_extprod(const uint32_t *in, int rdim, int mx, int bits, const uint32_t *keys, uint32_t *acc,
+                          size_t n, size_t in_stride, size_t key_stride, size_t acc_stride, const DevTables &t,
+                          hipStream_t s) {
+  k_extprod<<<grid_for(n, 1), 256, 0, s>>>(in, rdim, mx, bits, keys, acc, n, in_stride, key_stride, acc_stride,
+                                           t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_nat_to_p(const uint32_t *in, uint32_t *out, size_t n, int m_div, int mont, const DevTables &t,
+                           hipStream_t s) {
+  k_nat_to_p<<<grid_for(n * N, 256), 256, 0, s>>>(in, out, n, m_div, mont, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_p_to_nat(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s) {
+  k_p_to_nat<<<grid_for(n * N, 256), 256, 0, s>>>(in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s) {
+  k_db_from_ref<<<grid_for((size_t)2 * R * C * N, 256), 256, 0, s>>>(in, out, R, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int R, int C,
+                            const DevTables &t, hipStream_t s) {
+  k_db_encode<<<grid_for(nrec * n2 * 2, 4), 256, 0, s>>>(bytes, rec0, nrec, db, n2, R, C, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, uint32_t *out, int T, int R,
+                         int C, const DevTables &t, hipStream_t s) {
+  const int cg = C >= 8 ? 8 : C;
+  const unsigned blocks = (unsigned)(32 * T * (C / cg));
+  switch (cg) {
+    case 8: k_db_mul<8><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
+    case 4: k_db_mul<4><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
+    case 2: k_db_mul<2><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
+    default: k_db_mul<1><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_level(bool fast, uint32_t *accs, size_t set_stride, int n_trials, int n_queries,
+                             int num_per, const uint32_t *cpos, const uint32_t *cneg, size_t key_query_stride,
+                             int t_gsw, int bits, const DevTables &t, hipStream_t s) {
+  dim3 grid(num_per, n_trials, n_queries);
+  if (fast)
+    k_fold_level<true><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, cneg, key_query_stride,
+                                            t_gsw, bits, t.tw, t.mc[0], t.mc[1]);
+  else
+    k_fold_level<false><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, cneg, key_query_stride,
+                                             t_gsw, bits, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_neg1(uint32_t *v, size_t q_stride, int n_queries, int num_in, const uint32_t *neg1_r,
+                              const DevTables &t, hipStream_t s) {
+  dim3 grid(grid_for((size_t)num_in * N, 256, 65535u), n_queries);
+  k_expand_neg1<<<grid, 256, 0, s>>>(v, q_stride, num_in, neg1_r, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop,
+                                int max_bits, const uint32_t *left, const uint32_t *right,
+                                size_t key_query_stride, int t_exp, int bits, const DevTables &t, hipStream_t s) {
+  dim3 grid(num_out, n_queries);
+  k_expand_action<<<grid, 256, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, key_query_stride, t_exp, bits,
+                                       t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+}  // namespace sp

@@ -1,0 +1,42 @@
+// sp_kernels.hpp — host-side launchers for the kernels in sp_kernels.hip.
+#pragma once
+#include "sp_device.hpp"
+
+namespace sp {
+
+// Device-resident tables of a context.
+struct DevTables {
+  const uint32_t *tw;    // [m][dir][TW_WORDS]
+  const uint32_t *neg1;  // [LOGN][m][2][N]  (w, w') in layout P
+  ModConst mc[2];
+};
+
+inline const uint32_t *tw_of(const DevTables &t, int m, int inv) { return t.tw + (size_t)(m * 2 + inv) * TW_WORDS; }
+
+hipError_t launch_ntt_natural(bool inv, uint32_t *polys, size_t n, const DevTables &t, int m, int scale,
+                              hipStream_t s);
+hipError_t launch_from_ntt(bool layout_p, const uint32_t *ntt, uint32_t *raw, size_t n, size_t in_stride,
+                           size_t out_stride, const DevTables &t, hipStream_t s);
+hipError_t launch_to_ntt(const uint32_t *raw, uint32_t *ntt, size_t n, const DevTables &t, hipStream_t s);
+hipError_t launch_automorph_u64(const uint64_t *in, uint64_t *out, size_t n, uint32_t tt, hipStream_t s);
+hipError_t launch_extprod(const uint32_t *in, int rdim, int mx, int bits, const uint32_t *keys, uint32_t *acc,
+                          size_t n, size_t in_stride, size_t key_stride, size_t acc_stride, const DevTables &t,
+                          hipStream_t s);
+hipError_t launch_nat_to_p(const uint32_t *in, uint32_t *out, size_t n, int m_div, int mont, const DevTables &t,
+                           hipStream_t s);
+hipError_t launch_p_to_nat(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s);
+hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s);
+hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int R, int C,
+                            const DevTables &t, hipStream_t s);
+hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, uint32_t *out, int T, int R,
+                         int C, const DevTables &t, hipStream_t s);
+hipError_t launch_fold_level(bool fast, uint32_t *accs, size_t set_stride, int n_trials, int n_queries,
+                             int num_per, const uint32_t *cpos, const uint32_t *cneg, size_t key_query_stride,
+                             int t_gsw, int bits, const DevTables &t, hipStream_t s);
+hipError_t launch_expand_neg1(uint32_t *v, size_t q_stride, int n_queries, int num_in, const uint32_t *neg1_r,
+                              const DevTables &t, hipStream_t s);
+hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop,
+                                int max_bits, const uint32_t *left, const uint32_t *right,
+                                size_t key_query_stride, int t_exp, int bits, const DevTables &t, hipStream_t s);
+
+}  // namespace sp

@@ -1,0 +1,151 @@
+"""Server side of the answer path: context, resident database, per-query state.
+
+Mirrors what the reference's test mains assemble by hand (gen_twiddles_gpu,
+Database / FirstDim / FoldingQueryStorage structs, the db_mul and fold
+launches; gpu-pir/db_mul_test.cu:131-212, fold_test.cu:30-94) as objects
+backed by device memory owned by libspiral_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .params import Shape
+
+STAGE_EXPAND = 1
+STAGE_DB_MUL = 2
+STAGE_FOLD = 4
+STAGES_ALL = 7
+
+READ_RESPONSE = 0
+READ_EXPANDED = 1
+READ_DB_MUL = 2
+READ_RAW_ACCS = 3
+
+
+def _ptr(a) -> ctypes.c_void_p:
+    if a is None:
+        return ctypes.c_void_p(0)
+    assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.uint32, np.uint8)
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """Device context: twiddle tables, constant polynomials, a HIP stream.
+
+    Replaces gen_twiddles_gpu (gpu-pir/host.cu:126-138)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _lib.call("sp_ctx_create", device, ctypes.byref(h))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        _lib.call("sp_ctx_synchronize", self._h)
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("sp_ctx_destroy", self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Database:
+    """A database resident in HBM in NTT form (n2 sub-databases)."""
+
+    def __init__(self, ctx: Context, shape: Shape):
+        self.ctx = ctx
+        self.shape = shape
+        h = ctypes.c_void_p()
+        _lib.call("sp_db_create", ctx.handle, shape.nu1, shape.nu2, shape.n2, ctypes.byref(h))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def load_bytes(self, records: np.ndarray) -> None:
+        """records: uint8 [2^(nu1+nu2)][n2][2048]; converted to NTT form on the GPU."""
+        r = np.ascontiguousarray(records, dtype=np.uint8)
+        assert r.size == self.shape.records * self.shape.record_bytes
+        _lib.call("sp_db_load_bytes", self._h, _ptr(r))
+
+    def load_ntt(self, db_ntt: np.ndarray) -> None:
+        """Reference Database layout per sub-database: [n2][2^nu2][2^nu1][2][N] u32."""
+        d = np.ascontiguousarray(db_ntt, dtype=np.uint32)
+        assert d.size == self.shape.records * self.shape.n2 * 2 * 2048
+        _lib.call("sp_db_load_ntt", self._h, _ptr(d))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.load().sp_db_device_bytes(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("sp_db_destroy", self._h)
+            self._h = None
+
+
+class Query:
+    """Device-resident state of one query: its ciphertext, keys and workspace."""
+
+    def __init__(self, db: Database):
+        self.db = db
+        s = db.shape
+        h = ctypes.c_void_p()
+        _lib.call("sp_query_create", db.handle, s.t_gsw, s.t_exp, ctypes.byref(h))
+        self._h = h
+
+    def load(self, query: np.ndarray, left: np.ndarray, right: np.ndarray, v_folding: np.ndarray,
+             v_folding_neg: np.ndarray | None = None) -> None:
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (query, left, right, v_folding)]
+        neg = None if v_folding_neg is None else np.ascontiguousarray(v_folding_neg, dtype=np.uint32)
+        _lib.call("sp_query_load", self._h, *[_ptr(a) for a in arrs], _ptr(neg))
+
+    def run(self, stages: int = STAGES_ALL, stream=None) -> None:
+        _lib.call("sp_query_run", self._h, stages, ctypes.c_void_p(stream or 0))
+
+    def read(self, what: int = READ_RESPONSE) -> np.ndarray:
+        s = self.db.shape
+        if what == READ_RESPONSE:
+            out = np.zeros((s.n2, 2, 2, 2048), np.uint32)
+        elif what == READ_EXPANDED:
+            out = np.zeros((1 << s.g, 2, 2, 2048), np.uint32)
+        elif what == READ_DB_MUL:
+            out = np.zeros((s.n2, s.cols, 2, 2, 2048), np.uint32)
+        elif what == READ_RAW_ACCS:
+            out = np.zeros((s.n2, s.cols, 2, 2, 2048), np.uint32)
+        else:
+            raise ValueError(what)
+        _lib.call("sp_query_read", self._h, what, _ptr(out))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("sp_query_destroy", self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def answer(db: Database, query, left, right, v_folding) -> np.ndarray:
+    """One full answer (expansion, database multiply, fold) -> n2 raw ciphertexts."""
+    s = db.shape
+    out = np.zeros((s.n2, 2, 2, 2048), np.uint32)
+    arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (query, left, right, v_folding)]
+    _lib.call("sp_answer", db.handle, s.t_gsw, s.t_exp, *[_ptr(a) for a in arrs], _ptr(out))
+    return out

@@ -1,0 +1,198 @@
+/*
+ * spiral_pir.h — C ABI of libspiral_hip.so, the MI355X (gfx950) Spiral PIR
+ * server answer path: coefficient expansion -> first-dimension database
+ * multiply -> GSW fold, on the negacyclic 2048-point NTT over the CRT primes
+ * q1 = 268369921 and q2 = 249561089 (Q = q1*q2), gadget base 2^7 x 8 digits.
+ *
+ * The reference (EricArcherman/GPU-PIR) has no FFI layer: its operator
+ * surface is a set of CUDA kernels, the host helpers of gpu-pir/host.h and
+ * the struct layouts of gpu-pir/structures.cuh.  Each entry point below names
+ * the reference symbol it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status: SP_OK (0) or a negative SP_ERR_*.
+ *    Nothing calls exit(); sp_last_error() returns a message for the calling
+ *    thread's most recent failure.
+ *  - "dev" pointers are device memory (hipMalloc / torch) on the context's
+ *    device; "host" pointers are host memory.  Device-pointer functions are
+ *    asynchronous on `stream` (a hipStream_t; NULL means the context's own
+ *    stream).  Host-pointer functions synchronize before returning.
+ *  - Layouts (little-endian u32, structures.cuh:150-210):
+ *      NTT Poly        u32[2][2048]  crt[m][k] = residue mod q_m at NTT position
+ *                                    k (natural Spiral order), canonical [0,q_m)
+ *      raw Poly        u32[2][2048]  [0][i] = high 32 bits, [1][i] = low 32 bits
+ *                                    of coefficient i, a value in [0, Q]
+ *      Ciphertext      Poly[2]       row 0 = a, row 1 = a*s + e
+ *      CiphertextHalf  u32[2][2048]  [row][k], one residue
+ *      key matrix      CiphertextHalf[2 crt][cols]  = u32[crt][col][row][k]
+ *  - A context is single-threaded: the caller serializes calls on one ctx.
+ */
+#ifndef SPIRAL_PIR_H
+#define SPIRAL_PIR_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_OK 0
+#define SP_ERR_ARG (-1)     /* invalid argument (shape, null, range)      */
+#define SP_ERR_HIP (-2)     /* HIP runtime error                          */
+#define SP_ERR_NOMEM (-3)   /* device or host allocation failed           */
+#define SP_ERR_STATE (-4)   /* object used in the wrong state             */
+
+#define SP_POLY_LEN 2048
+#define SP_Q1 268369921u
+#define SP_Q2 249561089u
+
+typedef struct sp_ctx sp_ctx;
+typedef struct sp_db sp_db;
+typedef struct sp_client sp_client;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces gen_twiddles_gpu (gpu-pir/host.cu:126-138, host.h:10): the
+ * context owns the twiddle tables, automorphism permutations and constant
+ * polynomials on its device. */
+int sp_ctx_create(int device, sp_ctx **out);
+int sp_ctx_destroy(sp_ctx *ctx);
+int sp_ctx_synchronize(sp_ctx *ctx);
+const char *sp_last_error(void);
+/* Library build string (kernel variant names); never NULL. */
+const char *sp_version(void);
+
+/* ---- transforms ----------------------------------------------------------
+ * sp_ntt_fwd replaces NTT_R (gpu-pir/ntt_28bit.cuh:251): in place, n_polys
+ * contiguous u32[2048] polynomials of one residue (0 -> q1, 1 -> q2),
+ * coefficients canonical in, canonical NTT values out.
+ * sp_ntt_inv replaces INTT_R (gpu-pir/ntt_28bit.cuh:292).  With scale = 0 it
+ * keeps the reference's convention of leaving out N^-1 (the reference folds
+ * it into inv_crt); scale = 1 gives the true inverse.  Outputs canonical. */
+int sp_ntt_fwd(sp_ctx *ctx, uint32_t *dev_polys, size_t n_polys, int residue, void *stream);
+int sp_ntt_inv(sp_ctx *ctx, uint32_t *dev_polys, size_t n_polys, int residue, int scale,
+               void *stream);
+
+/* NTT Poly -> raw Poly: from_ntt_regs (gpu-pir/gadget.cuh:8-24) / to_raw
+ * (gpu-pir/classes.cuh:69-99).  n_polys NTT Polys in, n_polys raw Polys out. */
+int sp_from_ntt(sp_ctx *ctx, const uint32_t *dev_ntt, uint32_t *dev_raw, size_t n_polys,
+                void *stream);
+/* raw Poly -> NTT Poly: to_ntt (gpu-pir/classes.cuh:140-173). */
+int sp_to_ntt(sp_ctx *ctx, const uint32_t *dev_raw, uint32_t *dev_ntt, size_t n_polys,
+              void *stream);
+
+/* x -> x^t on u64 coefficients in [0, Q] (t odd): the `automorph` kernel
+ * (gpu-pir/automorph.cuh:54-62), n_polys polynomials of u64[2048]; negated
+ * entries are Q - x, so a negated zero becomes Q as in the reference. */
+int sp_automorph(sp_ctx *ctx, const uint64_t *dev_in, uint64_t *dev_out, size_t n_polys,
+                 uint32_t t, void *stream);
+
+/* ---- gadget external product --------------------------------------------
+ * inv_gadget_ntt_mult_rdim1/_rdim2 (gpu-pir/gadget.cuh:84-101):
+ *   acc (NTT Ciphertext, canonical) += M * NTT(g^-1(in))
+ * in: rdim (1 or 2) raw Polys; M: key matrix with mx columns, standard form,
+ * layout u32[crt][mx][row][k]; g^-1 takes mx/rdim digits of
+ * ceil(56/(mx/rdim)) bits.  n_items independent products, each operand
+ * advanced by its stride (in u32 words; key stride 0 shares one matrix). */
+int sp_gadget_extprod(sp_ctx *ctx, const uint32_t *dev_in, int rdim, int mx,
+                      const uint32_t *dev_keys, uint32_t *dev_acc, size_t n_items,
+                      size_t in_stride, size_t key_stride, size_t acc_stride, void *stream);
+
+/* ---- first-dimension database multiply ----------------------------------
+ * db_mul_kernel (gpu-pir/db_mul.cuh:26-70), reference layouts:
+ *   db         Database: col[2^nu2].row[2^nu1] NTT Poly
+ *   first_dim  FirstDim: selectors[2^nu1] NTT Ciphertext
+ *   out        FoldingInputs: accumulators[2^nu2] NTT Ciphertext
+ * out[c] = sum_j first_dim[j] * db[c][j] mod q, both ciphertext rows.
+ * Converts the layouts on every call; the answer path uses a resident
+ * sp_db instead. */
+int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *dev_first_dim,
+              uint32_t *dev_out, int nu1, int nu2, void *stream);
+
+/* ---- fold ------------------------------------------------------------------
+ * FOLD_CIPHERTEXTS / fold_all_ciphertexts (gpu-pir/folding.cuh:45-65):
+ * in place on accs (2^nu2 raw Ciphertexts); the result is accs[0].
+ * v_folding / v_folding_neg: CiphertextHalf[nu2][2 crt][2*t_gsw] (standard
+ * form).  v_folding_neg may be NULL: it is then taken as G - v_folding, the
+ * form a Spiral server derives, which allows the folding product to be
+ * evaluated with half the NTTs. */
+int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
+            const uint32_t *dev_v_folding_neg, void *stream);
+
+/* ---- coefficient expansion -----------------------------------------------
+ * coefficient_expansion_r over all rounds (gpu-pir/expansion.cuh:222-260,
+ * spiral-rs semantics described at expansion.cuh:16-28).
+ * dev_v: NTT Ciphertext[2^g], v[0] = query on entry; g and stop_round as in
+ * expansion.cuh:83-85.  left: CiphertextHalf[g][2][t_exp],
+ * right: CiphertextHalf[stop_round+1][2][t_exp].  rounds < 0 runs all g. */
+int sp_expansion_depth(int nu1, int nu2, int t_gsw, int *g, int *stop_round);
+int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_gsw, int t_exp,
+              const uint32_t *dev_left, const uint32_t *dev_right, int rounds, void *stream);
+
+/* ---- resident database and answer path -------------------------------------
+ * A database of 2^(nu1+nu2) records, each n2 polynomials of 2048 plaintext
+ * bytes (n2 = n*n sub-databases; record index = i1 * 2^nu2 + i2).  It is
+ * converted to NTT form on the GPU once and stays resident in HBM. */
+int sp_db_create(sp_ctx *ctx, int nu1, int nu2, int n2, sp_db **out);
+/* host_bytes: [records][n2][2048] u8, uploaded in chunks and NTT'd on device. */
+int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes);
+/* Reference layout (NTT Polys, [n2][2^nu2 cols][2^nu1 rows]), host memory. */
+int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt);
+int sp_db_destroy(sp_db *db);
+size_t sp_db_device_bytes(const sp_db *db);
+
+/* Per-query device state: the query ciphertext, its expansion and folding
+ * keys (resident in HBM) and the workspace of one answer.  Replaces the
+ * QueryStorage / FoldingQueryStorage structs (gpu-pir/structures.cuh:179-190),
+ * which the reference fills by hand in its test mains.
+ *   sp_query_load   host pointers as for sp_answer; v_folding_neg may be NULL
+ *                   (then G - v_folding, evaluated with half the NTTs)
+ *   sp_query_run    asynchronous on `stream`; stage bits 1 = expansion,
+ *                   2 = database multiply, 4 = from_ntt + fold.  Stages can
+ *                   run one at a time; stage 1 restarts from the stored query
+ *                   each time, so a run can be repeated without reloading.
+ *   sp_query_read   synchronizes, then copies out: what = 0 the n2 response
+ *                   ciphertexts (raw), 1 the expanded v (NTT, natural order),
+ *                   2 the database-multiply accumulators (NTT, natural order,
+ *                   between stages 2 and 4), 3 all n2*2^nu2 raw accumulators. */
+typedef struct sp_query sp_query;
+int sp_query_create(sp_db *db, int t_gsw, int t_exp, sp_query **out);
+int sp_query_load(sp_query *q, const uint32_t *host_query, const uint32_t *host_left,
+                  const uint32_t *host_right, const uint32_t *host_v_folding,
+                  const uint32_t *host_v_folding_neg);
+int sp_query_run(sp_query *q, int stages, void *stream);
+int sp_query_read(sp_query *q, int what, uint32_t *host_out);
+int sp_query_destroy(sp_query *q);
+
+/* One query against a resident database, stage by stage on the device:
+ *   query        NTT Ciphertext (natural order)
+ *   left/right   expansion keys as for sp_expand
+ *   v_folding    CiphertextHalf[nu2][2][2*t_gsw]; v_folding_neg = G - v_folding
+ *   out          n2 raw Ciphertexts
+ * All host pointers. */
+int sp_answer(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_query,
+              const uint32_t *host_left, const uint32_t *host_right,
+              const uint32_t *host_v_folding, uint32_t *host_out);
+
+/* ---- client ----------------------------------------------------------------
+ * The reference has no client: its inputs are spiral-rs dumps
+ * (the .dat files under gpu-pir/test_data, gpu-pir/corrected_store_vars.rs:1-55).  This host
+ * client (no GPU) produces inputs of the same structure for the parameters
+ * above: ternary secret, discrete Gaussian errors (sigma 6.4), expansion keys
+ * for sp_expand / sp_query_load, the packed query, GSW folding keys, and the
+ * decoding of a response.  Buffers are host u32 in the layouts above; sizes
+ * in u32 words come from sp_client_sizes.  A client is single-threaded. */
+int sp_client_create(int nu1, int nu2, int n2, int t_gsw, int t_exp, uint64_t seed,
+                     sp_client **out);
+int sp_client_destroy(sp_client *c);
+int sp_client_sizes(const sp_client *c, size_t *query_words, size_t *left_words,
+                    size_t *right_words, size_t *fold_words, size_t *response_words);
+int sp_client_keygen(sp_client *c, uint32_t *host_left, uint32_t *host_right);
+int sp_client_query(sp_client *c, uint64_t index, uint32_t *host_query,
+                    uint32_t *host_v_folding);
+int sp_client_decode(sp_client *c, const uint32_t *host_response, uint8_t *host_record);
+const char *sp_client_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

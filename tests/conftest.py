@@ -1,0 +1,77 @@
+"""Shared test setup.
+
+Markers: ``gpu`` tests need an MI355X (run with ``-m gpu``); everything else
+runs on the CPU.  The oracle (oracle/) is the checker; the product package
+is gpu-pir_amd/spiral_pir.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "gpu-pir_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+REF_DATA = "/root/reference/gpu-pir"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a GPU (MI355X); run with -m gpu")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as O
+
+    O.lib()
+    return O
+
+
+@pytest.fixture(scope="session")
+def ref_data():
+    """Directory of the reference's fixture files (present only where the reference is mounted)."""
+    if not os.path.isdir(os.path.join(REF_DATA, "test_data")):
+        pytest.skip("reference fixtures not mounted")
+    return REF_DATA
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.fail("gpu test selected but no GPU is visible")
+    from spiral_pir.server import Context
+
+    torch.cuda.init()
+    ctx = Context(0)
+    yield ctx
+    ctx.close()
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.default_rng(seed)
+
+
+def rand_residues(g: np.random.Generator, n: int) -> np.ndarray:
+    """n NTT Polys with canonical residues: u32 [n][2][2048]."""
+    out = np.empty((n, 2, 2048), np.uint32)
+    out[:, 0] = g.integers(0, 268369921, size=(n, 2048), dtype=np.uint32)
+    out[:, 1] = g.integers(0, 249561089, size=(n, 2048), dtype=np.uint32)
+    return out
+
+
+def rand_raw(g: np.random.Generator, n: int, include_q: bool = False) -> np.ndarray:
+    """n raw Polys with coefficients in [0, Q) (optionally a few equal to Q)."""
+    Q = 268369921 * 249561089
+    x = g.integers(0, Q, size=(n, 2048), dtype=np.uint64)
+    if include_q:
+        x[:, ::97] = Q
+    out = np.empty((n, 2, 2048), np.uint32)
+    out[:, 0] = (x >> np.uint64(32)).astype(np.uint32)
+    out[:, 1] = (x & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    return out
