@@ -128,8 +128,11 @@ struct sp_ctx {
   DevTables tables{};
 };
 
+// The stream argument of device-pointer functions is a hipStream_t as is:
+// NULL is HIP's null stream (torch's default stream), not the context's.
 static hipStream_t pick(sp_ctx *ctx, void *stream) {
-  return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+  (void)ctx;
+  return reinterpret_cast<hipStream_t>(stream);
 }
 
 extern "C" int sp_ctx_create(int device, sp_ctx **out) {
@@ -281,7 +284,7 @@ extern "C" int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *de
   HIP_TRY(out.alloc((size_t)C * 4 * N * 4));
   HIP_TRY(launch_db_from_ref(dev_db, db.u32(), R, C, s));
   HIP_TRY(launch_nat_to_p(dev_first_dim, sel.u32(), (size_t)R * 4, 1, 0, ctx->tables, s));
-  HIP_TRY(launch_db_mul(db.u32(), sel.u32(), 4 * N, out.u32(), 1, R, C, ctx->tables, s));
+  HIP_TRY(launch_db_mul(db.u32(), sel.u32(), 4 * N, 0, out.u32(), 0, 1, 1, R, C, ctx->tables, s));
   HIP_TRY(launch_p_to_nat(out.u32(), dev_out, (size_t)C * 4, s));
   HIP_TRY(hipStreamSynchronize(s));
   return SP_OK;
@@ -331,16 +334,15 @@ extern "C" int sp_expansion_depth(int nu1, int nu2, int t_gsw, int *g, int *stop
   return SP_OK;
 }
 
-// Runs `rounds` expansion rounds on internal-layout v (n_queries sets).
+// Runs `rounds` expansion rounds on internal-layout v (n_queries sets; query
+// b's ciphertexts at v + b*q_stride, its keys at left/right + b*stride).
 static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_queries, int nu1, int nu2, int t_gsw,
-                         int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t key_query_stride_l,
-                         size_t key_query_stride_r, int rounds, hipStream_t s) {
+                         int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t left_qstride,
+                         size_t right_qstride, int rounds, hipStream_t s) {
   int g = 0, stop = 0;
   int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
   if (rc) return rc;
   if (rounds < 0 || rounds > g) rounds = g;
-  if (key_query_stride_l != key_query_stride_r && n_queries > 1)
-    return fail(SP_ERR_ARG, "run_expansion: key strides differ");
   const size_t key_level = (size_t)2 * t_exp * 2 * N;
   for (int r = 0; r < rounds; r++) {
     const int num_in = 1 << r;
@@ -348,8 +350,8 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
                                ctx->tables, s));
     const uint32_t *lr = left_p + (size_t)r * key_level;
     const uint32_t *rr = right_p + (size_t)std::min(r, stop) * key_level;
-    HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr,
-                                 key_query_stride_l, t_exp, gadget_bits(t_exp), ctx->tables, s));
+    HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
+                                 right_qstride, t_exp, gadget_bits(t_exp), ctx->tables, s));
   }
   return SP_OK;
 }
@@ -456,172 +458,309 @@ extern "C" int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt) {
 }
 
 // ---------------------------------------------------------------------------
-// per-query answer pipeline
+// batched answer pipeline (a single query is a batch of capacity 1)
 // ---------------------------------------------------------------------------
-struct sp_query {
+struct sp_batch {
   sp_db *db = nullptr;
-  int t_gsw = 0, t_exp = 0, g = 0, stop = 0;
-  bool fast_fold = true;
-  uint32_t *query_ct = nullptr;  // the query Ciphertext, layout P (expansion input, kept pristine)
-  uint32_t *v = nullptr;       // [2^g] Ciphertext, layout P
-  uint32_t *left = nullptr;    // [g][m][t_exp][row][P], 2^32-scaled
-  uint32_t *right = nullptr;   // [stop+1][...]
-  uint32_t *fkeys = nullptr;   // [nu2][m][2 t_gsw][row][P], 2^32-scaled
-  uint32_t *fkeys_neg = nullptr;
-  uint32_t *acc = nullptr;     // [n2][C] Ciphertext: NTT (P) after db_mul, raw after from_ntt
-  uint32_t *stage = nullptr;   // host-transfer staging
+  int cap = 0, t_gsw = 0, t_exp = 0, g = 0, stop = 0;
+  int fold_mode = -1;             // -1 unset, 1 fast (G - C), 0 general (given C_neg)
+  size_t v_q = 0, l_q = 0, r_q = 0, f_q = 0, acc_q = 0;  // per-query strides in words
+  uint32_t *query_ct = nullptr;   // [cap][4N]  queries, layout P (expansion input)
+  uint32_t *v = nullptr;          // [cap][2^g][4N]
+  uint32_t *left = nullptr;       // [cap][g][m][t_exp][row][P], 2^32-scaled
+  uint32_t *right = nullptr;      // [cap][stop+1][...]
+  uint32_t *fkeys = nullptr;      // [cap][nu2][m][2 t_gsw][row][P], 2^32-scaled
+  uint32_t *fkeys_neg = nullptr;  // general fold mode only
+  uint32_t *acc = nullptr;        // [cap][n2][C][4N]: NTT (P) after db_mul, raw after from_ntt
+  uint32_t *stage = nullptr;      // host-transfer staging
   size_t stage_words = 0;
+  hipStream_t io = nullptr;       // uploads / readouts
+  hipStream_t cap_stream = nullptr;
+  hipEvent_t done = nullptr;      // recorded after every run
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  std::vector<hipGraph_t> graph_defs;
 };
 
-static void query_free(sp_query *q) {
-  if (!q) return;
-  uint32_t *bufs[] = {q->query_ct, q->v, q->left, q->right, q->fkeys, q->fkeys_neg, q->acc, q->stage};
-  for (uint32_t *b : bufs)
-    if (b) (void)hipFree(b);
-  delete q;
+struct sp_query {
+  sp_batch *b = nullptr;
+};
+
+static void batch_free(sp_batch *b) {
+  if (!b) return;
+  for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
+  for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
+  uint32_t *bufs[] = {b->query_ct, b->v, b->left, b->right, b->fkeys, b->fkeys_neg, b->acc, b->stage};
+  for (uint32_t *p : bufs)
+    if (p) (void)hipFree(p);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->io) (void)hipStreamDestroy(b->io);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  delete b;
 }
 
-extern "C" int sp_query_create(sp_db *db, int t_gsw, int t_exp, sp_query **out) {
-  if (!db || !out) return fail(SP_ERR_ARG, "sp_query_create: NULL");
+extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp_batch **out) {
+  if (!db || !out) return fail(SP_ERR_ARG, "sp_batch_create: NULL");
   *out = nullptr;
+  if (capacity < 1 || capacity > 4096) return fail(SP_ERR_ARG, "sp_batch_create: capacity %d", capacity);
   if (t_gsw < 2 || t_gsw > 28 || t_exp < 2 || t_exp > 56 || (t_exp & 1))
-    return fail(SP_ERR_ARG, "sp_query_create: t_gsw %d / t_exp %d unsupported", t_gsw, t_exp);
+    return fail(SP_ERR_ARG, "sp_batch_create: t_gsw %d / t_exp %d unsupported", t_gsw, t_exp);
   int g = 0, stop = 0;
   int rc = sp_expansion_depth(db->nu1, db->nu2, t_gsw, &g, &stop);
   if (rc) return rc;
-  if (stop < 1) return fail(SP_ERR_ARG, "sp_query_create: stop_round must be >= 1");
-  if ((1 << g) < 2 * (1 << db->nu1)) return fail(SP_ERR_ARG, "sp_query_create: expansion too shallow");
+  if (stop < 1) return fail(SP_ERR_ARG, "sp_batch_create: stop_round must be >= 1");
+  if ((1 << g) < 2 * (1 << db->nu1)) return fail(SP_ERR_ARG, "sp_batch_create: expansion too shallow");
   sp_ctx *ctx = db->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
-  sp_query *q = new sp_query();
-  q->db = db; q->t_gsw = t_gsw; q->t_exp = t_exp; q->g = g; q->stop = stop;
+  sp_batch *b = new sp_batch();
+  b->db = db; b->cap = capacity; b->t_gsw = t_gsw; b->t_exp = t_exp; b->g = g; b->stop = stop;
   const int C = 1 << db->nu2;
   const size_t kl = (size_t)2 * t_exp * 2 * N;
   const size_t kf = (size_t)2 * (2 * t_gsw) * 2 * N;
-  q->stage_words = std::max<size_t>({(size_t)g * kl, (size_t)db->nu2 * kf, (size_t)db->n2 * C * 4 * N,
-                                     ((size_t)1 << g) * 4 * N});
+  b->v_q = ((size_t)1 << g) * 4 * N;
+  b->l_q = (size_t)g * kl;
+  b->r_q = (size_t)(stop + 1) * kl;
+  b->f_q = (size_t)db->nu2 * kf;
+  b->acc_q = (size_t)db->n2 * C * 4 * N;
+  b->stage_words = std::max<size_t>({b->l_q, b->r_q, b->f_q, b->acc_q, b->v_q});
+  const size_t cap = (size_t)capacity;
   hipError_t e = hipSuccess;
   auto A = [&](uint32_t **p, size_t words) {
     if (e == hipSuccess) e = hipMalloc(p, words * 4);
   };
-  A(&q->query_ct, (size_t)4 * N);
-  A(&q->v, ((size_t)1 << g) * 4 * N);
-  A(&q->left, (size_t)g * kl);
-  A(&q->right, (size_t)(stop + 1) * kl);
-  A(&q->fkeys, (size_t)db->nu2 * kf);
-  A(&q->acc, (size_t)db->n2 * C * 4 * N);
-  A(&q->stage, q->stage_words);
+  A(&b->query_ct, cap * 4 * N);
+  A(&b->v, cap * b->v_q);
+  A(&b->left, cap * b->l_q);
+  A(&b->right, cap * b->r_q);
+  A(&b->fkeys, cap * b->f_q);
+  A(&b->acc, cap * b->acc_q);
+  A(&b->stage, b->stage_words);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->io, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
   if (e != hipSuccess) {
-    query_free(q);
-    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_query_create: %s", hipGetErrorString(e));
+    batch_free(b);
+    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_batch_create: %s", hipGetErrorString(e));
   }
-  *out = q;
+  *out = b;
+  return SP_OK;
+}
+
+extern "C" int sp_batch_destroy(sp_batch *b) {
+  if (!b) return SP_OK;
+  (void)hipSetDevice(b->db->ctx->device);
+  (void)hipDeviceSynchronize();
+  batch_free(b);
+  return SP_OK;
+}
+
+// Host -> device (natural layout) -> internal P (2^32-scaled when mont), on the batch's io stream.
+static int upload_p(sp_batch *b, const uint32_t *host, uint32_t *dst, size_t n_polys, int m_div, int mont) {
+  sp_ctx *ctx = b->db->ctx;
+  if (n_polys * N > b->stage_words) return fail(SP_ERR_ARG, "upload_p: staging too small");
+  HIP_TRY(hipMemcpyAsync(b->stage, host, n_polys * N * 4, hipMemcpyHostToDevice, b->io));
+  HIP_TRY(launch_nat_to_p(b->stage, dst, n_polys, m_div, mont, ctx->tables, b->io));
+  HIP_TRY(hipStreamSynchronize(b->io));
+  return SP_OK;
+}
+
+static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const uint32_t *host_left,
+                      const uint32_t *host_right, const uint32_t *host_v_folding, const uint32_t *host_v_folding_neg) {
+  if (!b || !host_query || !host_left || !host_right || !host_v_folding) return fail(SP_ERR_ARG, "load: NULL");
+  if (slot < 0 || slot >= b->cap) return fail(SP_ERR_ARG, "load: slot %d of %d", slot, b->cap);
+  const int mode = host_v_folding_neg ? 0 : 1;
+  if (b->cap > 1 && mode == 0) return fail(SP_ERR_ARG, "load: batches derive v_folding_neg = G - v_folding");
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));  // the previous run must be done with the slot
+  const size_t kl_polys = (size_t)2 * b->t_exp * 2;
+  const size_t kf_polys = (size_t)2 * (2 * b->t_gsw) * 2;
+  const size_t sl = (size_t)slot;
+  int rc;
+  if ((rc = upload_p(b, host_query, b->query_ct + sl * 4 * N, 4, 1, 0))) return rc;
+  if ((rc = upload_p(b, host_left, b->left + sl * b->l_q, (size_t)b->g * kl_polys, 2 * b->t_exp, 1))) return rc;
+  if ((rc = upload_p(b, host_right, b->right + sl * b->r_q, (size_t)(b->stop + 1) * kl_polys, 2 * b->t_exp, 1)))
+    return rc;
+  if ((rc = upload_p(b, host_v_folding, b->fkeys + sl * b->f_q, (size_t)b->db->nu2 * kf_polys, 4 * b->t_gsw, 1)))
+    return rc;
+  if (mode == 0) {
+    if (!b->fkeys_neg) HIP_TRY(hipMalloc(&b->fkeys_neg, (size_t)b->cap * b->f_q * 4));
+    if ((rc = upload_p(b, host_v_folding_neg, b->fkeys_neg + sl * b->f_q, (size_t)b->db->nu2 * kf_polys,
+                       4 * b->t_gsw, 1)))
+      return rc;
+  }
+  b->fold_mode = mode;
+  return SP_OK;
+}
+
+extern "C" int sp_batch_load(sp_batch *b, int slot, const uint32_t *host_query, const uint32_t *host_left,
+                             const uint32_t *host_right, const uint32_t *host_v_folding) {
+  return batch_load(b, slot, host_query, host_left, host_right, host_v_folding, nullptr);
+}
+
+// Enqueues the stages for slots [0, n) on stream s (no synchronization).
+static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
+  sp_db *db = b->db;
+  sp_ctx *ctx = db->ctx;
+  const int R = 1 << db->nu1, C = 1 << db->nu2;
+  if (stages & 1) {
+    HIP_TRY(launch_copy_queries(b->query_ct, 4 * N, b->v, b->v_q, n, s));
+    int rc = run_expansion(ctx, b->v, b->v_q, n, db->nu1, db->nu2, b->t_gsw, b->t_exp, b->left, b->right, b->l_q,
+                           b->r_q, -1, s);
+    if (rc) return rc;
+  }
+  if (stages & 2)
+    HIP_TRY(launch_db_mul(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, ctx->tables, s));
+  if (stages & 4) {
+    HIP_TRY(launch_from_ntt(true, b->acc, b->acc, (size_t)n * db->n2 * C * 2, 2 * N, 2 * N, ctx->tables, s));
+    const bool fast = b->fold_mode != 0;
+    const int mx = 2 * b->t_gsw;
+    const size_t level = (size_t)2 * mx * 2 * N;
+    int num_per = C;
+    for (int cur = 0; cur < db->nu2; cur++) {
+      num_per >>= 1;
+      const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
+      HIP_TRY(launch_fold_level(fast, b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off,
+                                fast ? nullptr : b->fkeys_neg + off, b->f_q, b->t_gsw, gadget_bits(b->t_gsw),
+                                ctx->tables, s));
+    }
+  }
+  return SP_OK;
+}
+
+extern "C" int sp_batch_run(sp_batch *b, int n, int stages, void *stream) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_run: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_run: n = %d, capacity %d", n, b->cap);
+  if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_run: nothing loaded");
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  int rc = batch_enqueue(b, n, stages, s);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(b->done, s));
+  return SP_OK;
+}
+
+// Replays a captured graph of all stages for n queries (captured on first use).
+extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_graph_launch: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_graph_launch: n = %d, capacity %d", n, b->cap);
+  if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_graph_launch: nothing loaded");
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  hipGraphExec_t exec = nullptr;
+  for (auto &ge : b->graphs)
+    if (ge.first == n) exec = ge.second;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b, n, 7, b->cap_stream);
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(SP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(SP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    b->graphs.push_back({n, exec});
+    b->graph_defs.push_back(graph);
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIP_TRY(hipGraphLaunch(exec, s));
+  HIP_TRY(hipEventRecord(b->done, s));
+  return SP_OK;
+}
+
+// what: 0 = response of `slot` (n2 raw Ciphertexts), 1 = its expanded v (NTT,
+// natural), 2 = its database-multiply accumulators (NTT, natural; between
+// stages 2 and 4), 3 = its n2*C raw accumulators.
+static int batch_read(sp_batch *b, int slot, int what, uint32_t *host_out) {
+  if (!b || !host_out) return fail(SP_ERR_ARG, "read: NULL");
+  if (slot < 0 || slot >= b->cap) return fail(SP_ERR_ARG, "read: slot %d of %d", slot, b->cap);
+  sp_db *db = b->db;
+  HIP_TRY(hipSetDevice(db->ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));
+  const int C = 1 << db->nu2;
+  const uint32_t *acc = b->acc + (size_t)slot * b->acc_q;
+  if (what == 0) {
+    for (int t = 0; t < db->n2; t++)
+      HIP_TRY(hipMemcpyAsync(host_out + (size_t)t * 4 * N, acc + (size_t)t * C * 4 * N, 4 * N * 4,
+                             hipMemcpyDeviceToHost, b->io));
+  } else if (what == 1) {
+    const size_t n = ((size_t)1 << b->g) * 4;
+    HIP_TRY(launch_p_to_nat(b->v + (size_t)slot * b->v_q, b->stage, n, b->io));
+    HIP_TRY(hipMemcpyAsync(host_out, b->stage, n * N * 4, hipMemcpyDeviceToHost, b->io));
+  } else if (what == 2) {
+    const size_t n = (size_t)db->n2 * C * 4;
+    HIP_TRY(launch_p_to_nat(acc, b->stage, n, b->io));
+    HIP_TRY(hipMemcpyAsync(host_out, b->stage, n * N * 4, hipMemcpyDeviceToHost, b->io));
+  } else if (what == 3) {
+    HIP_TRY(hipMemcpyAsync(host_out, acc, b->acc_q * 4, hipMemcpyDeviceToHost, b->io));
+  } else {
+    return fail(SP_ERR_ARG, "read: what = %d", what);
+  }
+  HIP_TRY(hipStreamSynchronize(b->io));
+  return SP_OK;
+}
+
+extern "C" int sp_batch_read(sp_batch *b, int slot, int what, uint32_t *host_out) {
+  return batch_read(b, slot, what, host_out);
+}
+
+// Response of slots [0, n) into host_out [n][n2][4N] in one call.
+extern "C" int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out) {
+  if (!b || !host_out) return fail(SP_ERR_ARG, "sp_batch_read_responses: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_read_responses: n = %d", n);
+  sp_db *db = b->db;
+  HIP_TRY(hipSetDevice(db->ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));
+  const int C = 1 << db->nu2;
+  // Response t of slot q is the first ciphertext of (q, t): strided 2-D copy.
+  HIP_TRY(hipMemcpy2DAsync(host_out, (size_t)4 * N * 4, b->acc, (size_t)C * 4 * N * 4, (size_t)4 * N * 4,
+                           (size_t)n * db->n2, hipMemcpyDeviceToHost, b->io));
+  HIP_TRY(hipStreamSynchronize(b->io));
+  return SP_OK;
+}
+
+extern "C" int sp_batch_synchronize(sp_batch *b) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_synchronize: NULL");
+  HIP_TRY(hipEventSynchronize(b->done));
+  return SP_OK;
+}
+
+// ---- single query = batch of capacity 1 -------------------------------------
+extern "C" int sp_query_create(sp_db *db, int t_gsw, int t_exp, sp_query **out) {
+  if (!out) return fail(SP_ERR_ARG, "sp_query_create: NULL");
+  *out = nullptr;
+  sp_batch *b = nullptr;
+  int rc = sp_batch_create(db, 1, t_gsw, t_exp, &b);
+  if (rc) return rc;
+  *out = new sp_query{b};
   return SP_OK;
 }
 
 extern "C" int sp_query_destroy(sp_query *q) {
   if (!q) return SP_OK;
-  (void)hipSetDevice(q->db->ctx->device);
-  (void)hipStreamSynchronize(q->db->ctx->stream);
-  query_free(q);
-  return SP_OK;
-}
-
-// Host -> device, natural layout -> internal P (2^32-scaled when mont).
-static int upload_p(sp_query *q, const uint32_t *host, uint32_t *dst, size_t n_polys, int m_div, int mont) {
-  sp_ctx *ctx = q->db->ctx;
-  if (n_polys * N > q->stage_words) return fail(SP_ERR_ARG, "upload_p: staging too small");
-  HIP_TRY(hipMemcpyAsync(q->stage, host, n_polys * N * 4, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(launch_nat_to_p(q->stage, dst, n_polys, m_div, mont, ctx->tables, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  sp_batch_destroy(q->b);
+  delete q;
   return SP_OK;
 }
 
 extern "C" int sp_query_load(sp_query *q, const uint32_t *host_query, const uint32_t *host_left,
                              const uint32_t *host_right, const uint32_t *host_v_folding,
                              const uint32_t *host_v_folding_neg) {
-  if (!q || !host_query || !host_left || !host_right || !host_v_folding) return fail(SP_ERR_ARG, "sp_query_load: NULL");
-  sp_ctx *ctx = q->db->ctx;
-  HIP_TRY(hipSetDevice(ctx->device));
-  const size_t kl_polys = (size_t)2 * q->t_exp * 2;
-  const size_t kf_polys = (size_t)2 * (2 * q->t_gsw) * 2;
-  int rc;
-  if ((rc = upload_p(q, host_query, q->query_ct, 4, 1, 0))) return rc;
-  if ((rc = upload_p(q, host_left, q->left, (size_t)q->g * kl_polys, 2 * q->t_exp, 1))) return rc;
-  if ((rc = upload_p(q, host_right, q->right, (size_t)(q->stop + 1) * kl_polys, 2 * q->t_exp, 1))) return rc;
-  if ((rc = upload_p(q, host_v_folding, q->fkeys, (size_t)q->db->nu2 * kf_polys, 4 * q->t_gsw, 1))) return rc;
-  q->fast_fold = host_v_folding_neg == nullptr;
-  if (!q->fast_fold) {
-    if (!q->fkeys_neg) HIP_TRY(hipMalloc(&q->fkeys_neg, (size_t)q->db->nu2 * kf_polys * N * 4));
-    if ((rc = upload_p(q, host_v_folding_neg, q->fkeys_neg, (size_t)q->db->nu2 * kf_polys, 4 * q->t_gsw, 1)))
-      return rc;
-  }
-  return SP_OK;
+  if (!q) return fail(SP_ERR_ARG, "sp_query_load: NULL");
+  return batch_load(q->b, 0, host_query, host_left, host_right, host_v_folding, host_v_folding_neg);
 }
 
-// Stage bits: 1 = expansion, 2 = database multiply, 4 = from_ntt + fold.
 extern "C" int sp_query_run(sp_query *q, int stages, void *stream) {
   if (!q) return fail(SP_ERR_ARG, "sp_query_run: NULL");
-  sp_db *db = q->db;
-  sp_ctx *ctx = db->ctx;
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipStream_t s = pick(ctx, stream);
-  const int R = 1 << db->nu1, C = 1 << db->nu2;
-  int rc;
-  if (stages & 1) {
-    // Expansion works in place on v; v[0] starts as the stored query.
-    HIP_TRY(hipMemcpyAsync(q->v, q->query_ct, (size_t)4 * N * 4, hipMemcpyDeviceToDevice, s));
-    rc = run_expansion(ctx, q->v, 0, 1, db->nu1, db->nu2, q->t_gsw, q->t_exp, q->left, q->right, 0, 0, -1, s);
-    if (rc) return rc;
-  }
-  if (stages & 2) HIP_TRY(launch_db_mul(db->d_db, q->v, 2 * 4 * N, q->acc, db->n2, R, C, ctx->tables, s));
-  if (stages & 4) {
-    HIP_TRY(launch_from_ntt(true, q->acc, q->acc, (size_t)db->n2 * C * 2, 2 * N, 2 * N, ctx->tables, s));
-    const int mx = 2 * q->t_gsw;
-    const size_t level = (size_t)2 * mx * 2 * N;
-    int num_per = C;
-    for (int cur = 0; cur < db->nu2; cur++) {
-      num_per >>= 1;
-      const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
-      HIP_TRY(launch_fold_level(q->fast_fold, q->acc, (size_t)C * 4 * N, db->n2, 1, num_per, q->fkeys + off,
-                                q->fast_fold ? nullptr : q->fkeys_neg + off, 0, q->t_gsw, gadget_bits(q->t_gsw),
-                                ctx->tables, s));
-    }
-  }
-  (void)R;
-  return SP_OK;
+  return sp_batch_run(q->b, 1, stages, stream);
 }
 
-// Debug / parity readouts.  what: 0 = result (n2 raw Ciphertexts),
-// 1 = expanded v (2^g NTT Ciphertexts, natural order),
-// 2 = database-multiply accumulators (n2*C NTT Ciphertexts, natural order;
-//     valid after stage 2 and before stage 4),
-// 3 = all n2*C raw accumulators.
 extern "C" int sp_query_read(sp_query *q, int what, uint32_t *host_out) {
-  if (!q || !host_out) return fail(SP_ERR_ARG, "sp_query_read: NULL");
-  sp_db *db = q->db;
-  sp_ctx *ctx = db->ctx;
-  HIP_TRY(hipSetDevice(ctx->device));
-  const int C = 1 << db->nu2;
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (what == 0) {
-    for (int t = 0; t < db->n2; t++)
-      HIP_TRY(hipMemcpy(host_out + (size_t)t * 4 * N, q->acc + (size_t)t * C * 4 * N, 4 * N * 4, hipMemcpyDeviceToHost));
-  } else if (what == 1) {
-    const size_t n = ((size_t)1 << q->g) * 4;
-    HIP_TRY(launch_p_to_nat(q->v, q->stage, n, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(host_out, q->stage, n * N * 4, hipMemcpyDeviceToHost, ctx->stream));
-  } else if (what == 2) {
-    const size_t n = (size_t)db->n2 * C * 4;
-    HIP_TRY(launch_p_to_nat(q->acc, q->stage, n, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(host_out, q->stage, n * N * 4, hipMemcpyDeviceToHost, ctx->stream));
-  } else if (what == 3) {
-    HIP_TRY(hipMemcpyAsync(host_out, q->acc, (size_t)db->n2 * C * 4 * N * 4, hipMemcpyDeviceToHost, ctx->stream));
-  } else {
-    return fail(SP_ERR_ARG, "sp_query_read: what = %d", what);
-  }
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return SP_OK;
+  if (!q) return fail(SP_ERR_ARG, "sp_query_read: NULL");
+  return batch_read(q->b, 0, what, host_out);
 }
 
 extern "C" int sp_answer(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_query, const uint32_t *host_left,
@@ -630,7 +769,7 @@ extern "C" int sp_answer(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_q
   int rc = sp_query_create(db, t_gsw, t_exp, &q);
   if (rc) return rc;
   rc = sp_query_load(q, host_query, host_left, host_right, host_v_folding, nullptr);
-  if (!rc) rc = sp_query_run(q, 7, nullptr);
+  if (!rc) rc = sp_query_run(q, 7, q->b->io);
   if (!rc) rc = sp_query_read(q, 0, host_out);
   sp_query_destroy(q);
   return rc;

@@ -284,32 +284,31 @@ __global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ b
 // (residue, position-block) selector groups, keeping selector re-reads in
 // one XCD's L2.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t fold32(uint64_t x, uint32_t two32) {
-  return (x >> 32) * (uint64_t)two32 + (x & 0xFFFFFFFFull);
-}
+// Exact x mod q for any 64-bit x (canonical): (hi * 2^32 + lo) mod q.
 __device__ __forceinline__ uint32_t mod_u64(uint64_t x, const ModConst &mc) {
-  x = fold32(x, mc.two32);
-  x = fold32(x, mc.two32);
-  x = fold32(x, mc.two32);
-  const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
-  uint32_t t = canon(lo - __umulhi(lo, mc.mu1) * mc.q, mc.q);
-  t += hi ? mc.two32 : 0u;
-  return canon(t, mc.q);
+  return raw_mod((uint32_t)(x >> 32), (uint32_t)x, mc);
 }
 
 template <int CG>
-__global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db, const uint32_t *__restrict__ sel,
-                                                size_t sel_stride, uint32_t *__restrict__ out, int T, int R, int C,
-                                                ModConst mc0, ModConst mc1) {
+__global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db, const uint32_t *__restrict__ sel_all,
+                                                size_t sel_stride, size_t sel_qstride, uint32_t *__restrict__ out_all,
+                                                size_t out_qstride, int nq, int T, int R, int C, ModConst mc0,
+                                                ModConst mc1) {
   __shared__ uint32_t red[4][CG][2][128];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ncg = C / CG;
   const int bid = blockIdx.x;
+  // Consecutive blocks of one XCD take the same database tile for different
+  // queries, so a batch reads each tile from HBM about once.
   const int xcd = bid & 7, loc = bid >> 3;
-  const int sg = xcd * 4 + (loc & 3);
-  const int rest = loc >> 2;
+  const int qi = loc % nq;
+  const int rest0 = loc / nq;
+  const int sg = xcd * 4 + (rest0 & 3);
+  const int rest = rest0 >> 2;
   const int tt = rest % T, cg = rest / T;
   if (cg >= ncg) return;
+  const uint32_t *__restrict__ sel = sel_all + (size_t)qi * sel_qstride;
+  uint32_t *__restrict__ out = out_all + (size_t)qi * out_qstride;
   const int m = sg >> 4, pb = sg & 15;
   const ModConst mc = m ? mc1 : mc0;
   const int p0 = pb * 128 + lane * 2;
@@ -322,6 +321,8 @@ __global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db,
 #pragma unroll
     for (int pp = 0; pp < 2; pp++) { acc[c][pp][0] = 0; acc[c][pp][1] = 0; }
 
+  // Products are < q^2 < 2^56; after each 128-row block the accumulators are
+  // reduced mod q, so a block starts below q and ends below q + 128 q^2 < 2^63.
   const uint32_t *dbase = db + ((((size_t)tt * 2 + m) * R) * C + (size_t)cg * CG) * N + p0;
   for (int jb = 0; jb < rows_per_wave; jb += 128) {
     const int jend = min(rows_per_wave, jb + 128);
@@ -341,13 +342,15 @@ __global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db,
         acc[c][1][1] += (uint64_t)s1.y * d.y;
       }
     }
+    if (jend < rows_per_wave) {
 #pragma unroll
-    for (int c = 0; c < CG; c++)
+      for (int c = 0; c < CG; c++)
 #pragma unroll
-      for (int pp = 0; pp < 2; pp++) {
-        acc[c][pp][0] = fold32(acc[c][pp][0], mc.two32);
-        acc[c][pp][1] = fold32(acc[c][pp][1], mc.two32);
-      }
+        for (int pp = 0; pp < 2; pp++) {
+          acc[c][pp][0] = mod_u64(acc[c][pp][0], mc);
+          acc[c][pp][1] = mod_u64(acc[c][pp][1], mc);
+        }
+    }
   }
 #pragma unroll
   for (int c = 0; c < CG; c++)
@@ -518,15 +521,24 @@ __global__ __launch_bounds__(256) void k_expand_neg1(uint32_t *__restrict__ v, s
   }
 }
 
+// v[b][0] = query[b] for b < nq (restarts the in-place expansion).
+__global__ __launch_bounds__(256) void k_copy_queries(const uint32_t *__restrict__ query, size_t query_qstride,
+                                                      uint32_t *__restrict__ v, size_t v_qstride) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(query + (size_t)blockIdx.y * query_qstride);
+  uint4 *dst = reinterpret_cast<uint4 *>(v + (size_t)blockIdx.y * v_qstride);
+  for (int u = blockIdx.x * 256 + threadIdx.x; u < N; u += gridDim.x * 256) dst[u] = src[u];
+}
+
 __device__ __forceinline__ uint32_t brv11(uint32_t x) { return __brev(x) >> 21; }
 
 // One action_expand step (gpu-pir/expansion.cuh:87-166) for element i of
 // round r, in place on v[i] (layout P).  Block = 4 waves.
 __global__ __launch_bounds__(256) void k_expand_action(uint32_t *__restrict__ v, size_t q_stride, int r,
                                                        int stop, int max_bits, const uint32_t *__restrict__ left,
-                                                       const uint32_t *__restrict__ right, size_t key_query_stride,
-                                                       int t_exp, int bits, const uint32_t *__restrict__ tw_base,
-                                                       ModConst mc0, ModConst mc1) {
+                                                       const uint32_t *__restrict__ right, size_t left_qstride,
+                                                       size_t right_qstride, int t_exp, int bits,
+                                                       const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                       ModConst mc1) {
   __shared__ uint32_t scratch[4 * LDS_TILE];
   __shared__ uint64_t ta[N];
   __shared__ uint32_t as_e[2][N];
@@ -537,7 +549,7 @@ __global__ __launch_bounds__(256) void k_expand_action(uint32_t *__restrict__ v,
   uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
   const uint32_t t = (uint32_t)(N >> r) + 1u;
   const bool use_left = (r != 0) && ((i & 1) == 0);
-  const uint32_t *W = (use_left ? left : right) + (size_t)blockIdx.y * key_query_stride;
+  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
   uint32_t *xm = scratch + 2 * LDS_TILE;  // tiles of waves 2 and 3, unused in phase 1
 
   // Phase 1: waves 0/1 take v_i.a to raw residues; waves 2/3 stage v_i.as_e.
@@ -705,16 +717,21 @@ hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint
   return hipGetLastError();
 }
 
-hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, uint32_t *out, int T, int R,
-                         int C, const DevTables &t, hipStream_t s) {
+hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
+                         uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,
+                         hipStream_t s) {
   const int cg = C >= 8 ? 8 : C;
-  const unsigned blocks = (unsigned)(32 * T * (C / cg));
+  const unsigned blocks = (unsigned)(32 * T * (C / cg) * nq);
+#define SP_DBMUL(CGV)                                                                                        \
+  k_db_mul<CGV><<<blocks, 256, 0, s>>>(db, sel, sel_stride, sel_qstride, out, out_qstride, nq, T, R, C, t.mc[0], \
+                                      t.mc[1])
   switch (cg) {
-    case 8: k_db_mul<8><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
-    case 4: k_db_mul<4><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
-    case 2: k_db_mul<2><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
-    default: k_db_mul<1><<<blocks, 256, 0, s>>>(db, sel, sel_stride, out, T, R, C, t.mc[0], t.mc[1]); break;
+    case 8: SP_DBMUL(8); break;
+    case 4: SP_DBMUL(4); break;
+    case 2: SP_DBMUL(2); break;
+    default: SP_DBMUL(1); break;
   }
+#undef SP_DBMUL
   return hipGetLastError();
 }
 
@@ -739,12 +756,20 @@ hipError_t launch_expand_neg1(uint32_t *v, size_t q_stride, int n_queries, int n
 }
 
 hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop,
-                                int max_bits, const uint32_t *left, const uint32_t *right,
-                                size_t key_query_stride, int t_exp, int bits, const DevTables &t, hipStream_t s) {
+                                int max_bits, const uint32_t *left, const uint32_t *right, size_t left_qstride,
+                                size_t right_qstride, int t_exp, int bits, const DevTables &t, hipStream_t s) {
   dim3 grid(num_out, n_queries);
-  k_expand_action<<<grid, 256, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, key_query_stride, t_exp, bits,
-                                       t.tw, t.mc[0], t.mc[1]);
+  k_expand_action<<<grid, 256, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride,
+                                       t_exp, bits, t.tw, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 
+}  // namespace sp
+
+namespace sp {
+hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t v_qstride, int nq,
+                               hipStream_t s) {
+  k_copy_queries<<<dim3(2, nq), 256, 0, s>>>(query, query_qstride, v, v_qstride);
+  return hipGetLastError();
+}
 }  // namespace sp

@@ -28,15 +28,19 @@ hipError_t launch_p_to_nat(const uint32_t *in, uint32_t *out, size_t n, hipStrea
 hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s);
 hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int R, int C,
                             const DevTables &t, hipStream_t s);
-hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, uint32_t *out, int T, int R,
-                         int C, const DevTables &t, hipStream_t s);
+// Batched over nq queries: selectors of query b at sel + b*sel_qstride, outputs at out + b*out_qstride.
+hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
+                         uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,
+                         hipStream_t s);
 hipError_t launch_fold_level(bool fast, uint32_t *accs, size_t set_stride, int n_trials, int n_queries,
                              int num_per, const uint32_t *cpos, const uint32_t *cneg, size_t key_query_stride,
                              int t_gsw, int bits, const DevTables &t, hipStream_t s);
 hipError_t launch_expand_neg1(uint32_t *v, size_t q_stride, int n_queries, int num_in, const uint32_t *neg1_r,
                               const DevTables &t, hipStream_t s);
 hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop,
-                                int max_bits, const uint32_t *left, const uint32_t *right,
-                                size_t key_query_stride, int t_exp, int bits, const DevTables &t, hipStream_t s);
+                                int max_bits, const uint32_t *left, const uint32_t *right, size_t left_qstride,
+                                size_t right_qstride, int t_exp, int bits, const DevTables &t, hipStream_t s);
+hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t v_qstride, int nq,
+                               hipStream_t s);
 
 }  // namespace sp

@@ -59,6 +59,14 @@ SIGNATURES = {
     "sp_query_read": [P, I, P],
     "sp_query_destroy": [P],
     "sp_answer": [P, I, I, P, P, P, P, P],
+    "sp_batch_create": [P, I, I, I, PP],
+    "sp_batch_load": [P, I, P, P, P, P],
+    "sp_batch_run": [P, I, I, P],
+    "sp_batch_graph_launch": [P, I, P],
+    "sp_batch_read": [P, I, I, P],
+    "sp_batch_read_responses": [P, I, P],
+    "sp_batch_synchronize": [P],
+    "sp_batch_destroy": [P],
     "sp_client_create": [I, I, I, I, I, U64, PP],
     "sp_client_destroy": [P],
     "sp_client_sizes": [P, PSZ, PSZ, PSZ, PSZ, PSZ],

@@ -142,6 +142,55 @@ class Query:
         return self._h
 
 
+class Batch:
+    """`capacity` query slots against one database, answered together: one launch per
+    stage for the whole batch, optionally replayed from a captured hipGraph."""
+
+    def __init__(self, db: Database, capacity: int):
+        self.db = db
+        self.capacity = capacity
+        s = db.shape
+        h = ctypes.c_void_p()
+        _lib.call("sp_batch_create", db.handle, capacity, s.t_gsw, s.t_exp, ctypes.byref(h))
+        self._h = h
+
+    def load(self, slot: int, query, left, right, v_folding) -> None:
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (query, left, right, v_folding)]
+        _lib.call("sp_batch_load", self._h, slot, *[_ptr(a) for a in arrs])
+
+    def run(self, n: int, stages: int = STAGES_ALL, stream=None) -> None:
+        _lib.call("sp_batch_run", self._h, n, stages, ctypes.c_void_p(stream or 0))
+
+    def graph_launch(self, n: int, stream=None) -> None:
+        _lib.call("sp_batch_graph_launch", self._h, n, ctypes.c_void_p(stream or 0))
+
+    def synchronize(self) -> None:
+        _lib.call("sp_batch_synchronize", self._h)
+
+    def read(self, slot: int, what: int = READ_RESPONSE) -> np.ndarray:
+        s = self.db.shape
+        shapes = {READ_RESPONSE: (s.n2, 2, 2, 2048), READ_EXPANDED: (1 << s.g, 2, 2, 2048),
+                  READ_DB_MUL: (s.n2, s.cols, 2, 2, 2048), READ_RAW_ACCS: (s.n2, s.cols, 2, 2, 2048)}
+        out = np.zeros(shapes[what], np.uint32)
+        _lib.call("sp_batch_read", self._h, slot, what, _ptr(out))
+        return out
+
+    def responses(self, n: int) -> np.ndarray:
+        s = self.db.shape
+        out = np.zeros((n, s.n2, 2, 2, 2048), np.uint32)
+        _lib.call("sp_batch_read_responses", self._h, n, _ptr(out))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("sp_batch_destroy", self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+
 def answer(db: Database, query, left, right, v_folding) -> np.ndarray:
     """One full answer (expansion, database multiply, fold) -> n2 raw ciphertexts."""
     s = db.shape

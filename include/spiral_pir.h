@@ -15,8 +15,9 @@
  *    thread's most recent failure.
  *  - "dev" pointers are device memory (hipMalloc / torch) on the context's
  *    device; "host" pointers are host memory.  Device-pointer functions are
- *    asynchronous on `stream` (a hipStream_t; NULL means the context's own
- *    stream).  Host-pointer functions synchronize before returning.
+ *    asynchronous on `stream`, a hipStream_t taken as is (NULL is HIP's null
+ *    stream, which is also torch's default stream).  Host-pointer functions
+ *    work on the context's own stream and synchronize before returning.
  *  - Layouts (little-endian u32, structures.cuh:150-210):
  *      NTT Poly        u32[2][2048]  crt[m][k] = residue mod q_m at NTT position
  *                                    k (natural Spiral order), canonical [0,q_m)
@@ -162,6 +163,28 @@ int sp_query_load(sp_query *q, const uint32_t *host_query, const uint32_t *host_
 int sp_query_run(sp_query *q, int stages, void *stream);
 int sp_query_read(sp_query *q, int what, uint32_t *host_out);
 int sp_query_destroy(sp_query *q);
+
+/* Batches: `capacity` query slots against one resident database, run
+ * together (one launch per stage for the whole batch).  The server derives
+ * v_folding_neg = G - v_folding for every slot.
+ *   sp_batch_load            upload one query's ciphertext and keys into a slot
+ *                            (waits for the batch's previous run)
+ *   sp_batch_run             enqueue stages for slots [0, n) on `stream`
+ *   sp_batch_graph_launch    same as sp_batch_run(n, all stages) through a
+ *                            hipGraph captured on first use for this n
+ *   sp_batch_read            per-slot readouts as sp_query_read
+ *   sp_batch_read_responses  responses of slots [0, n): [n][n2] raw Ciphertexts
+ *   sp_batch_synchronize     wait for the batch's last run */
+typedef struct sp_batch sp_batch;
+int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp_batch **out);
+int sp_batch_load(sp_batch *b, int slot, const uint32_t *host_query, const uint32_t *host_left,
+                  const uint32_t *host_right, const uint32_t *host_v_folding);
+int sp_batch_run(sp_batch *b, int n, int stages, void *stream);
+int sp_batch_graph_launch(sp_batch *b, int n, void *stream);
+int sp_batch_read(sp_batch *b, int slot, int what, uint32_t *host_out);
+int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out);
+int sp_batch_synchronize(sp_batch *b);
+int sp_batch_destroy(sp_batch *b);
 
 /* One query against a resident database, stage by stage on the device:
  *   query        NTT Ciphertext (natural order)

@@ -430,7 +430,8 @@ void or_answer(const uint32_t *db, int n2, int nu1, int nu2, int t_gsw, int t_ex
     for (size_t j = 0; j < rows; j++)
         memcpy(sel + j * ct_words, v + 2 * j * ct_words, ct_words * sizeof(uint32_t));
     for (int t = 0; t < n2; t++) {
-        or_db_mul(db + (size_t)t * cols * rows * 2 * OR_N, sel, acc, nu1, nu2);
+        /* same result as or_db_mul (tests check it), spread over the host cores */
+        or_db_mul_omp(db + (size_t)t * cols * rows * 2 * OR_N, sel, acc, nu1, nu2, 0);
         for (size_t c = 0; c < cols; c++) {
             or_from_ntt(acc + c * ct_words, raw + c * ct_words);
             or_from_ntt(acc + c * ct_words + 2 * OR_N, raw + c * ct_words + 2 * OR_N);
@@ -444,16 +445,19 @@ void or_answer(const uint32_t *db, int n2, int nu1, int nu2, int t_gsw, int t_ex
 void or_db_encode(const uint8_t *bytes, uint32_t *db, int n2, int nu1, int nu2) {
     init();
     const size_t rows = (size_t)1 << nu1, cols = (size_t)1 << nu2;
-    for (size_t i1 = 0; i1 < rows; i1++)
-        for (size_t i2 = 0; i2 < cols; i2++) {
-            const size_t idx = i1 * cols + i2;
-            for (int t = 0; t < n2; t++) {
-                const uint8_t *p = bytes + (idx * n2 + t) * OR_N;
-                uint32_t *o = db + (((size_t)t * cols + i2) * rows + i1) * 2 * OR_N;
-                for (int m = 0; m < 2; m++) {
-                    for (int z = 0; z < OR_N; z++) o[m * OR_N + z] = p[z];
-                    or_ntt_fwd(o + m * OR_N, m);
-                }
+    const long nrec = (long)(rows * cols);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (long idx_l = 0; idx_l < nrec; idx_l++) {
+        const size_t idx = (size_t)idx_l, i1 = idx / cols, i2 = idx % cols;
+        for (int t = 0; t < n2; t++) {
+            const uint8_t *p = bytes + (idx * n2 + t) * OR_N;
+            uint32_t *o = db + (((size_t)t * cols + i2) * rows + i1) * 2 * OR_N;
+            for (int m = 0; m < 2; m++) {
+                for (int z = 0; z < OR_N; z++) o[m * OR_N + z] = p[z];
+                or_ntt_fwd(o + m * OR_N, m);
             }
         }
+    }
 }

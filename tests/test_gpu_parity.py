@@ -249,7 +249,8 @@ def _answer_case(gpu_ctx, oracle, nu1, nu2, seed_db, seed_client, index):
     db_ntt = oracle.db_encode(recs, s.n2, nu1, nu2)
     sel = np.ascontiguousarray(v_exp[0:2 * s.rows:2])
     for t in range(s.n2):
-        np.testing.assert_array_equal(acc_gpu[t], oracle.db_mul(db_ntt[t], sel, nu1, nu2), err_msg=f"trial {t}")
+        np.testing.assert_array_equal(acc_gpu[t], oracle.db_mul(db_ntt[t], sel, nu1, nu2, nthreads=0),
+                                      err_msg=f"trial {t}")
     vneg = oracle.folding_neg(vf, nu2, s.t_gsw)
     exp = oracle.answer(db_ntt, s.n2, nu1, nu2, s.t_gsw, s.t_exp, qct, left, right, vf, vneg)
     np.testing.assert_array_equal(resp, exp)
@@ -265,3 +266,34 @@ def test_answer_small_stage_by_stage(gpu_ctx, oracle):
 
 def test_answer_other_shape(gpu_ctx, oracle):
     _answer_case(gpu_ctx, oracle, 4, 3, 1004, 2004, 101)
+
+
+def test_answer_config1_shape(gpu_ctx, oracle):
+    """(nu1, nu2) = (9, 6), n = 2: the single-query benchmark shape, every stage bit-exact."""
+    _answer_case(gpu_ctx, oracle, 9, 6, 1001, 2001, 12345)
+
+
+def test_query_rerun_is_identical(gpu_ctx):
+    """Running the same resident query twice gives the same response (stage 1 restarts
+    from the stored query)."""
+    from spiral_pir import Shape
+    from spiral_pir.client import Client
+    from spiral_pir.server import Database, Query
+
+    s = Shape(5, 2)
+    recs = rng(31).integers(0, 256, size=(s.records, s.n2, N), dtype=np.uint8)
+    db = Database(gpu_ctx, s)
+    db.load_bytes(recs)
+    c = Client(s, 32)
+    left, right = c.keygen()
+    qct, vf = c.query(77)
+    q = Query(db)
+    q.load(qct, left, right, vf)
+    q.run()
+    r1 = q.read()
+    q.run()
+    r2 = q.read()
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_array_equal(c.decode(r1), recs[77])
+    q.close()
+    db.close()
