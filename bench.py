@@ -77,6 +77,37 @@ def cpu_baseline(shape, q, left, right, vf, sample_cols: int) -> dict:
     }
 
 
+def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, stream) -> dict:
+    """B queries (8 clients) answered per hipGraph launch; queries/s over `steps` launches."""
+    n_clients = min(8, B)
+    clients = [Client(shape, 2100 + k) for k in range(n_clients)]
+    keys = [c.keygen() for c in clients]
+    g = np.random.default_rng(2200)
+    idx = [int(x) for x in g.integers(0, shape.records, size=B)]
+    batch = Batch(db, B)
+    for slot in range(B):
+        c = slot % n_clients
+        q, vf = clients[c].query(idx[slot])
+        batch.load(slot, q, keys[c][0], keys[c][1], vf)
+    sptr = stream.cuda_stream
+    batch.graph_launch(B, sptr)  # captures the graph
+    batch.graph_launch(B, sptr)
+    torch.cuda.synchronize()
+    resp = batch.responses(B)
+    ok = all(np.array_equal(clients[s % n_clients].decode(resp[s]), records[idx[s]]) for s in range(B))
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        batch.graph_launch(B, sptr)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    batch.close()
+    return {"B": B, "ms_per_batch": ms, "queries_per_s": B * 1000.0 / ms, "steps": steps,
+            "launch": "hipGraph replay of the whole stage chain", "decoded_all_correct": ok}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +118,8 @@ def main() -> int:
     ap.add_argument("--index", type=int, default=12345)
     ap.add_argument("--cpu-sample-cols", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=64, help="batched leg: queries per graph launch (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -97,7 +130,7 @@ def main() -> int:
     import torch
     from spiral_pir import Shape, version
     from spiral_pir.client import Client
-    from spiral_pir.server import (Context, Database, Query, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD,
+    from spiral_pir.server import (Batch, Context, Database, Query, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD,
                                    STAGES_ALL)
 
     torch.cuda.init()
@@ -159,6 +192,11 @@ def main() -> int:
         st["fold"].append(e[2].elapsed_time(e[3]))
     stage_ms = {k: float(np.median(v)) for k, v in st.items()}
 
+    batched = None
+    if args.batch > 0:
+        batched = run_batched(torch, Batch, Client, db, shape, records, args.batch, args.batch_steps, stream)
+        correct = correct and batched["decoded_all_correct"]
+
     alg_bytes = db_mul_algorithmic_bytes(shape)
     achieved = alg_bytes / (dbmul_ms * 1e-3) / 1e9
     db_scan_gbps = shape.db_device_bytes / (dbmul_ms * 1e-3) / 1e9
@@ -195,11 +233,14 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
         },
+        "batched": None if batched is None else {k: v for k, v in batched.items() if k != "decoded_all_correct"},
         "ingest_s": t_ingest,
         "library": version(),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(shape, q, left, right, vf, args.cpu_sample_cols)
+    if batched is not None:
+        line["batched"]["speedup_vs_single_query"] = batched["queries_per_s"] / line["value"]
     print(json.dumps(line), flush=True)
     query.close()
     db.close()

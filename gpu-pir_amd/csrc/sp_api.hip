@@ -122,6 +122,7 @@ static void build_tw(uint32_t *out, int m, int inv) {
 // ---------------------------------------------------------------------------
 struct sp_ctx {
   int device = 0;
+  int variant[3] = {0, 0, 0};  // per stage (SP_STAGE_*): 0 = automatic choice
   hipStream_t stream = nullptr;
   uint32_t *d_tw = nullptr;
   uint32_t *d_neg1 = nullptr;
@@ -189,6 +190,13 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
   if (ctx->d_neg1) (void)hipFree(ctx->d_neg1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return SP_OK;
+}
+
+extern "C" int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant) {
+  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > 2)
+    return fail(SP_ERR_ARG, "sp_ctx_set_variant: stage %d variant %d", stage, variant);
+  ctx->variant[stage] = variant;
   return SP_OK;
 }
 
@@ -310,8 +318,11 @@ extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, cons
   for (int cur = 0; cur < nu2; cur++) {
     num_per >>= 1;
     const size_t lvl = (size_t)(nu2 - 1 - cur) * level_polys * N;
-    HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, fast ? nullptr : kn.u32() + lvl, 0,
-                              t_gsw, gadget_bits(t_gsw), ctx->tables, s));
+    if (fast && t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
+      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, 0, ctx->tables, s));
+    else
+      HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, fast ? nullptr : kn.u32() + lvl, 0,
+                                t_gsw, gadget_bits(t_gsw), ctx->tables, s));
   }
   HIP_TRY(hipStreamSynchronize(s));
   return SP_OK;
@@ -350,8 +361,13 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
                                ctx->tables, s));
     const uint32_t *lr = left_p + (size_t)r * key_level;
     const uint32_t *rr = right_p + (size_t)std::min(r, stop) * key_level;
-    HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
-                                 right_qstride, t_exp, gadget_bits(t_exp), ctx->tables, s));
+    const bool rounds8 = t_exp == 8 && ctx->variant[SP_STAGE_EXPAND] != 1;
+    if (rounds8)
+      HIP_TRY(launch_expand8(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
+                             right_qstride, ctx->tables, s));
+    else
+      HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
+                                   right_qstride, t_exp, gadget_bits(t_exp), ctx->tables, s));
   }
   return SP_OK;
 }
@@ -618,9 +634,12 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
     for (int cur = 0; cur < db->nu2; cur++) {
       num_per >>= 1;
       const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
-      HIP_TRY(launch_fold_level(fast, b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off,
-                                fast ? nullptr : b->fkeys_neg + off, b->f_q, b->t_gsw, gadget_bits(b->t_gsw),
-                                ctx->tables, s));
+      if (fast && b->t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
+        HIP_TRY(launch_fold8(b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off, b->f_q, ctx->tables, s));
+      else
+        HIP_TRY(launch_fold_level(fast, b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off,
+                                  fast ? nullptr : b->fkeys_neg + off, b->f_q, b->t_gsw, gadget_bits(b->t_gsw),
+                                  ctx->tables, s));
     }
   }
   return SP_OK;

@@ -773,3 +773,300 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
   return hipGetLastError();
 }
 }  // namespace sp
+
+// ===========================================================================
+// Round-structured external-product kernels (gadget base 2^7 x 8 digits).
+//
+// All gadget digits of a block's inputs are extracted once into LDS as
+// packed bytes.  Each round, every wave transforms one digit polynomial into
+// its own LDS tile (layout P); then every thread multiply-accumulates the
+// round's tiles at its own PPT positions against the key matrix.  The 64-bit
+// accumulators live only in the multiply-accumulate phase, so the NTT keeps
+// its registers and the block runs NW = 8 transforms at a time.
+// Digit bytes: coefficient z = 64 i + l of digit polynomial k sits in byte
+// (i & 3) of word (i >> 2) * 64 + l, so lane l of a wave reads its phase-A
+// registers with 8 conflict-free 4-byte LDS reads.
+// ===========================================================================
+namespace sp {
+
+constexpr int DIG_WORDS = N / 4;  // 512 words per byte-packed digit polynomial
+
+__device__ __forceinline__ void load_digit_bytes(uint32_t (&r)[32], const uint32_t *__restrict__ dig, int lane,
+                                                 bool is_signed, uint32_t offset) {
+#pragma unroll
+  for (int i4 = 0; i4 < 8; i4++) {
+    const uint32_t w = dig[i4 * 64 + lane];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const uint32_t byte = (w >> (8 * b)) & 0xFFu;
+      r[4 * i4 + b] = is_signed ? (uint32_t)((int32_t)(int8_t)byte + (int32_t)offset) : byte;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_tile_p(uint32_t *__restrict__ tile, const uint32_t (&r)[32], int lane,
+                                             uint32_t two_q) {
+  uint4 *t4 = reinterpret_cast<uint4 *>(tile);
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    t4[c * 64 + lane] = make_uint4(reduce_2q(r[4 * c], two_q), reduce_2q(r[4 * c + 1], two_q),
+                                   reduce_2q(r[4 * c + 2], two_q), reduce_2q(r[4 * c + 3], two_q));
+}
+
+// Fold level, server form C_neg = G - C:  acc[i] <- Y + from_ntt(C * NTT(g^-1 X - g^-1 Y)),
+// X = acc[num_per + i], Y = acc[i].  32 tasks = 2 residues x 2 polys x 8 digits.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_fold8(uint32_t *__restrict__ accs, size_t set_stride, int n_trials,
+                                                   int num_per, const uint32_t *__restrict__ cpos,
+                                                   size_t key_qstride, const uint32_t *__restrict__ tw_base,
+                                                   ModConst mc0, ModConst mc1) {
+  constexpr int NT = NW * 64;
+  constexpr int TASKS = 32;
+  constexpr int ROUNDS = TASKS / NW;
+  constexpr int PPT = N / NT;
+  static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+  __shared__ uint32_t tiles[NW][LDS_TILE];
+  __shared__ uint32_t dig[2][8][DIG_WORDS];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int i = blockIdx.x;
+  const int set = blockIdx.z * n_trials + blockIdx.y;
+  uint32_t *base = accs + (size_t)set * set_stride;
+  const uint32_t *X = base + (size_t)(num_per + i) * 4 * N;
+  uint32_t *Y = base + (size_t)i * 4 * N;
+  const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
+
+  // Digit differences g^-1(X) - g^-1(Y), signed bytes.
+  for (int it = tid; it < 2 * DIG_WORDS; it += NT) {
+    const int j = it / DIG_WORDS, w = it % DIG_WORDS, l = w & 63, i4 = w >> 6;
+    const uint32_t *xh = X + (size_t)j * 2 * N, *xl = xh + N;
+    const uint32_t *yh = Y + (size_t)j * 2 * N, *yl = yh + N;
+    uint32_t packed[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) packed[k] = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int z = 64 * (4 * i4 + b) + l;
+      const uint64_t x = ((uint64_t)xh[z] << 32) | xl[z];
+      const uint64_t y = ((uint64_t)yh[z] << 32) | yl[z];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int d = (int)((x >> (7 * k)) & 127u) - (int)((y >> (7 * k)) & 127u);
+        packed[k] |= ((uint32_t)d & 0xFFu) << (8 * b);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) dig[j][k][w] = packed[k];
+  }
+  __syncthreads();
+
+  uint64_t acc[2][2][PPT];
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int row = 0; row < 2; row++)
+#pragma unroll
+      for (int e = 0; e < PPT; e++) acc[m][row][e] = 0;
+
+  for (int rnd = 0; rnd < ROUNDS; rnd++) {
+    {
+      const int task = rnd * NW + wave;
+      const int m = task & 1, j = (task >> 1) & 1, k = task >> 2;
+      const uint32_t q = m ? mc1.q : mc0.q;
+      uint32_t r[32];
+      load_digit_bytes(r, dig[j][k], lane, true, q);
+      ntt_fwd_wave(r, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, lane);
+      store_tile_p(tiles[wave], r, lane, 2 * q);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w2 = 0; w2 < NW; w2++) {
+      const int task = rnd * NW + w2;
+      const int m = w2 & 1, j = (task >> 1) & 1, k = task >> 2;
+      const uint32_t *k0 = kp + ((size_t)(m * 16 + j + 2 * k) * 2) * N + PPT * tid;
+      const uint32_t *k1 = k0 + N;
+      const uint32_t *tl = &tiles[w2][PPT * tid];
+#pragma unroll
+      for (int e = 0; e < PPT; e++) {
+        const uint32_t d = tl[e];
+        acc[m][0][e] += (uint64_t)d * k0[e];
+        acc[m][1][e] += (uint64_t)d * k1[e];
+      }
+    }
+    __syncthreads();
+  }
+  // Sums of polynomial (row, m) into tile 2m + row, then INTT + CRT + Y.
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const ModConst &mc = m ? mc1 : mc0;
+#pragma unroll
+    for (int row = 0; row < 2; row++)
+#pragma unroll
+      for (int e = 0; e < PPT; e++) tiles[2 * m + row][PPT * tid + e] = redc64(acc[m][row][e], mc.q, mc.qinv_neg);
+  }
+  __syncthreads();
+  uint32_t s[32];
+  if (wave < 4) {
+    const int m = wave >> 1;
+    const ModConst mc = m ? mc1 : mc0;
+    load_p(s, tiles[wave], lane);
+    wave_lds_sync();
+    ntt_inv_wave(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+#pragma unroll
+    for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  }
+  __syncthreads();
+  if (wave < 4) {
+    const int m = wave >> 1;
+    uint32_t *xm = tiles[wave];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) xm[ii * 64 + lane] = s[16 * (1 - m) + ii];
+  }
+  __syncthreads();
+  if (wave < 4) {
+    const int m = wave >> 1, row = wave & 1;
+    const uint32_t *xo = tiles[wave ^ 2];
+    uint32_t *yh = Y + (size_t)row * 2 * N, *yl = yh + N;
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+      const int z = 64 * (16 * m + ii) + lane;
+      const uint32_t own = s[16 * m + ii], oth = xo[ii * 64 + lane];
+      const uint32_t x0 = m ? oth : own, x1 = m ? own : oth;
+      uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
+      x = (x >= QQ) ? x - QQ : x;
+      x += ((uint64_t)yh[z] << 32) | yl[z];
+      x = (x >= QQ) ? x - QQ : x;
+      yh[z] = (uint32_t)(x >> 32);
+      yl[z] = (uint32_t)x;
+    }
+  }
+}
+
+// Expansion step (gpu-pir/expansion.cuh:87-166) for element i of round r,
+// gadget 2^7 x 8: 16 tasks = 2 residues x 8 digits of tau_t(a).
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
+                                                     int max_bits, const uint32_t *__restrict__ left,
+                                                     const uint32_t *__restrict__ right, size_t left_qstride,
+                                                     size_t right_qstride, const uint32_t *__restrict__ tw_base,
+                                                     ModConst mc0, ModConst mc1) {
+  constexpr int NT = NW * 64;
+  constexpr int TASKS = 16;
+  constexpr int ROUNDS = TASKS / NW;
+  constexpr int PPT = N / NT;
+  static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+  __shared__ uint32_t tiles[NW][LDS_TILE];
+  __shared__ uint32_t dig[8][DIG_WORDS];
+  __shared__ uint32_t as_e[2][N];
+  const int i = blockIdx.x;
+  if (stop > 0 && r > stop && (i & 1)) return;
+  if (stop > 0 && r == stop && (i & 1) && (i / 2) >= max_bits) return;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  const bool use_left = (r != 0) && ((i & 1) == 0);
+  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
+  uint32_t *xm = tiles[2];  // tiles 2 and 3 (2 * LDS_TILE >= 2 N words) are free in phase 1
+
+  // Phase 1: waves 0/1 take v_i.a to CRT halves; the others stage v_i.as_e.
+  if (wave < 2) {
+    const int m = wave;
+    const ModConst mc = m ? mc1 : mc0;
+    uint32_t rr[32];
+    load_p(rr, vi + (size_t)m * N, lane);
+    ntt_inv_wave(rr, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+#pragma unroll
+    for (int ii = 0; ii < 32; ii++) xm[m * N + 64 * ii + lane] = canon(shoup_mul(rr[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  }
+  for (int u = tid - 128; u >= 0 && u < N / 2; u += NT - 128)
+    reinterpret_cast<uint4 *>(&as_e[0][0])[u] = reinterpret_cast<const uint4 *>(vi + 2 * N)[u];
+  __syncthreads();
+  // Phase 2: CRT, raw automorphism (a negated zero becomes Q), digit bytes.
+  uint8_t *db = reinterpret_cast<uint8_t *>(&dig[0][0]);
+  for (int z = tid; z < N; z += NT) {
+    uint64_t x = (uint64_t)xm[z] * Q1 + (uint64_t)xm[N + z] * Q0;
+    x = (x >= QQ) ? x - QQ : x;
+    const uint32_t zt = (uint32_t)z * t;
+    const uint32_t dst = zt & (N - 1);
+    const uint64_t y = ((zt >> LOGN) & 1) ? QQ - x : x;
+    const uint32_t word = (dst >> 8) * 64 + (dst & 63), byte = (dst >> 6) & 3;
+#pragma unroll
+    for (int k = 0; k < 8; k++) db[((size_t)k * DIG_WORDS + word) * 4 + byte] = (uint8_t)((y >> (7 * k)) & 127u);
+  }
+  __syncthreads();
+
+  uint64_t acc[2][2][PPT];
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int row = 0; row < 2; row++)
+#pragma unroll
+      for (int e = 0; e < PPT; e++) acc[m][row][e] = 0;
+
+  for (int rnd = 0; rnd < ROUNDS; rnd++) {
+    {
+      const int task = rnd * NW + wave;
+      const int m = task & 1, k = task >> 1;
+      const uint32_t q = m ? mc1.q : mc0.q;
+      uint32_t rr[32];
+      load_digit_bytes(rr, dig[k], lane, false, 0);
+      ntt_fwd_wave(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, lane);
+      store_tile_p(tiles[wave], rr, lane, 2 * q);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w2 = 0; w2 < NW; w2++) {
+      const int task = rnd * NW + w2;
+      const int m = w2 & 1, k = task >> 1;
+      const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + PPT * tid;
+      const uint32_t *k1 = k0 + N;
+      const uint32_t *tl = &tiles[w2][PPT * tid];
+#pragma unroll
+      for (int e = 0; e < PPT; e++) {
+        const uint32_t d = tl[e];
+        acc[m][0][e] += (uint64_t)d * k0[e];
+        acc[m][1][e] += (uint64_t)d * k1[e];
+      }
+    }
+    __syncthreads();
+  }
+  // v_i += W g^-1(tau a) + (0, NTT(tau as_e)), thread-owned positions.
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const ModConst &mc = m ? mc1 : mc0;
+#pragma unroll
+    for (int row = 0; row < 2; row++) {
+      uint32_t *dst = vi + (size_t)(row * 2 + m) * N + PPT * tid;
+#pragma unroll
+      for (int e = 0; e < PPT; e++) {
+        uint32_t x = dst[e] + redc64(acc[m][row][e], mc.q, mc.qinv_neg);
+        if (row) {
+          const uint32_t z = (uint32_t)z_of_p(PPT * tid + e);
+          const uint32_t ev = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
+          x += as_e[m][p_of_z((int)brv11((ev - 1) >> 1))];
+        }
+        x = min(x, x - 2 * mc.q);
+        dst[e] = canon(x, mc.q);
+      }
+    }
+  }
+}
+
+hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
+                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, hipStream_t s) {
+  dim3 grid(num_per, n_trials, n_queries);
+  k_fold8<8><<<grid, 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
+                          const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
+                          const DevTables &t, hipStream_t s) {
+  dim3 grid(num_out, n_queries);
+  k_expand8<8><<<grid, 512, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride, t.tw,
+                                    t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+}  // namespace sp

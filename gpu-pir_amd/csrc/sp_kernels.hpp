@@ -43,4 +43,11 @@ hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int
 hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t v_qstride, int nq,
                                hipStream_t s);
 
+// Gadget 2^7 x 8 (t_gsw = t_exp = 8) round-structured kernels, 8 waves per block.
+hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
+                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, hipStream_t s);
+hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
+                          const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
+                          const DevTables &t, hipStream_t s);
+
 }  // namespace sp

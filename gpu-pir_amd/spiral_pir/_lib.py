@@ -39,6 +39,7 @@ SIGNATURES = {
     "sp_ctx_create": [I, PP],
     "sp_ctx_destroy": [P],
     "sp_ctx_synchronize": [P],
+    "sp_ctx_set_variant": [P, I, I],
     "sp_ntt_fwd": [P, P, SZ, I, P],
     "sp_ntt_inv": [P, P, SZ, I, I, P],
     "sp_from_ntt": [P, P, P, SZ, P],

@@ -19,6 +19,11 @@ STAGE_DB_MUL = 2
 STAGE_FOLD = 4
 STAGES_ALL = 7
 
+# indices for Context.set_variant
+VARIANT_EXPAND = 0
+VARIANT_FOLD = 1
+VARIANT_DB_MUL = 2
+
 READ_RESPONSE = 0
 READ_EXPANDED = 1
 READ_DB_MUL = 2
@@ -49,6 +54,10 @@ class Context:
 
     def synchronize(self) -> None:
         _lib.call("sp_ctx_synchronize", self._h)
+
+    def set_variant(self, stage: int, variant: int) -> None:
+        """Kernel choice per stage (STAGE_* index 0/1/2 = expand/fold/db_mul); 0 = automatic."""
+        _lib.call("sp_ctx_set_variant", self._h, stage, variant)
 
     def close(self) -> None:
         if self._h:

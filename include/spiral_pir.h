@@ -58,6 +58,14 @@ typedef struct sp_client sp_client;
 int sp_ctx_create(int device, sp_ctx **out);
 int sp_ctx_destroy(sp_ctx *ctx);
 int sp_ctx_synchronize(sp_ctx *ctx);
+/* Kernel variant per stage, for tests and the autotuner; 0 = automatic.
+ *   SP_STAGE_EXPAND, SP_STAGE_FOLD: 1 = one block of 4 waves per item (any
+ *     gadget), 2 = round-structured 8-wave kernel (gadget 2^7 x 8 only)
+ *   SP_STAGE_DB_MUL: 1 = VALU u64 multiply-accumulate, 2 = int8-limb MFMA */
+#define SP_STAGE_EXPAND 0
+#define SP_STAGE_FOLD 1
+#define SP_STAGE_DB_MUL 2
+int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant);
 const char *sp_last_error(void);
 /* Library build string (kernel variant names); never NULL. */
 const char *sp_version(void);

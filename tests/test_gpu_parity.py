@@ -185,9 +185,19 @@ def _rand_keys(g, levels, mx):
     return k
 
 
+@pytest.fixture
+def variant(gpu_ctx, request):
+    """(stage, variant) forced for one test, automatic choice restored afterwards."""
+    stage, var = request.param
+    gpu_ctx.set_variant(stage, var)
+    yield var
+    gpu_ctx.set_variant(stage, 0)
+
+
+@pytest.mark.parametrize("variant", [(1, 1), (1, 2)], indirect=True, ids=["fold-4wave", "fold-rounds8"])
 @pytest.mark.parametrize("nu2", [1, 2, 4])
 @pytest.mark.parametrize("general", [False, True])
-def test_fold(gpu_ctx, oracle, nu2, general):
+def test_fold(gpu_ctx, oracle, nu2, general, variant):
     from spiral_pir import ops
 
     g = rng(700 + nu2 + 10 * general)
@@ -202,8 +212,9 @@ def test_fold(gpu_ctx, oracle, nu2, general):
     np.testing.assert_array_equal(got[0], exp[0])
 
 
+@pytest.mark.parametrize("variant", [(0, 1), (0, 2)], indirect=True, ids=["expand-4wave", "expand-rounds8"])
 @pytest.mark.parametrize("nu1,nu2", [(6, 2), (3, 1)])
-def test_expansion_random_keys(gpu_ctx, oracle, nu1, nu2):
+def test_expansion_random_keys(gpu_ctx, oracle, nu1, nu2, variant):
     from spiral_pir import ops
 
     g = rng(800 + nu1)
@@ -262,6 +273,17 @@ def _answer_case(gpu_ctx, oracle, nu1, nu2, seed_db, seed_client, index):
 def test_answer_small_stage_by_stage(gpu_ctx, oracle):
     """Config-0 shape (nu1, nu2) = (6, 2), n = 2: every stage bit-exact, record decoded."""
     _answer_case(gpu_ctx, oracle, 6, 2, 1000, 2000, 37)
+
+
+def test_answer_small_legacy_kernels(gpu_ctx, oracle):
+    """The same answer through the 4-wave expansion and fold kernels."""
+    gpu_ctx.set_variant(0, 1)
+    gpu_ctx.set_variant(1, 1)
+    try:
+        _answer_case(gpu_ctx, oracle, 6, 2, 1000, 2000, 37)
+    finally:
+        gpu_ctx.set_variant(0, 0)
+        gpu_ctx.set_variant(1, 0)
 
 
 def test_answer_other_shape(gpu_ctx, oracle):
