@@ -77,7 +77,7 @@ def cpu_baseline(shape, q, left, right, vf, sample_cols: int) -> dict:
     }
 
 
-def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, stream) -> dict:
+def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, stream, eager: bool = False) -> dict:
     """B queries (8 clients) answered per hipGraph launch; queries/s over `steps` launches."""
     n_clients = min(8, B)
     clients = [Client(shape, 2100 + k) for k in range(n_clients)]
@@ -90,22 +90,32 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
         q, vf = clients[c].query(idx[slot])
         batch.load(slot, q, keys[c][0], keys[c][1], vf)
     sptr = stream.cuda_stream
-    batch.graph_launch(B, sptr)  # captures the graph
-    batch.graph_launch(B, sptr)
+    batch.run(B, 1, sptr)  # expansion once, so the autotuner can time the database multiply
+    tuned = batch.tune(B)
+    def launch():
+        if eager:
+            batch.run(B, 7, sptr)
+        else:
+            batch.graph_launch(B, sptr)
+
+    launch()  # first graph launch captures the graph
+    launch()
     torch.cuda.synchronize()
     resp = batch.responses(B)
-    ok = all(np.array_equal(clients[s % n_clients].decode(resp[s]), records[idx[s]]) for s in range(B))
+    bad = [s for s in range(B) if not np.array_equal(clients[s % n_clients].decode(resp[s]), records[idx[s]])]
+    ok = not bad
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
     for _ in range(steps):
-        batch.graph_launch(B, sptr)
+        launch()
     stop.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
     batch.close()
     return {"B": B, "ms_per_batch": ms, "queries_per_s": B * 1000.0 / ms, "steps": steps,
-            "launch": "hipGraph replay of the whole stage chain", "decoded_all_correct": ok}
+            "launch": "eager (one launch per kernel)" if eager else "hipGraph replay of the whole stage chain",
+            "db_mul_autotune": tuned, "decoded_all_correct": ok, "slots_decoded_wrong": bad}
 
 
 def main() -> int:
@@ -120,6 +130,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="batched leg: queries per graph launch (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-eager", action="store_true", help="batched leg without hipGraph replay")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB 3 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -134,6 +147,14 @@ def main() -> int:
                                    STAGES_ALL)
 
     torch.cuda.init()
+    if args.copy_mib > 0:
+        # A copy kernel of known size: reads and writes copy_mib MiB per dispatch.
+        src = torch.empty(args.copy_mib << 18, dtype=torch.int32, device="cuda").fill_(7)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)  # an elementwise kernel, never a DMA copy
+        torch.cuda.synchronize()
+        del src, dst
     shape = Shape(args.nu1, args.nu2)
     index = args.index % shape.records
     ctx = Context(0)
@@ -155,6 +176,7 @@ def main() -> int:
     torch.cuda.synchronize()
     decoded = client.decode(query.read())
     correct = bool(np.array_equal(decoded, records[index]))
+    single_correct = correct
 
     # Timed region: K whole answers, events around the database multiply.
     K = args.steps
@@ -194,7 +216,8 @@ def main() -> int:
 
     batched = None
     if args.batch > 0:
-        batched = run_batched(torch, Batch, Client, db, shape, records, args.batch, args.batch_steps, stream)
+        batched = run_batched(torch, Batch, Client, db, shape, records, args.batch, args.batch_steps, stream,
+                              eager=args.batch_eager)
         correct = correct and batched["decoded_all_correct"]
 
     alg_bytes = db_mul_algorithmic_bytes(shape)
@@ -224,6 +247,7 @@ def main() -> int:
         "db_mul_ms": dbmul_ms,
         "stage_ms": stage_ms,
         "decoded_record_correct": correct,
+        "single_query_decoded_correct": single_correct,
         "roofline": {
             "kernel": "k_db_mul (first-dimension database multiply)",
             "bound": "hbm",

@@ -126,6 +126,7 @@ struct sp_ctx {
   hipStream_t stream = nullptr;
   uint32_t *d_tw = nullptr;
   uint32_t *d_neg1 = nullptr;
+  uint32_t *d_pw = nullptr;  // [m][8]: 2^(7w) mod q_m, limb weights of the MFMA multiply
   DevTables tables{};
 };
 
@@ -165,8 +166,13 @@ extern "C" int sp_ctx_create(int device, sp_ctx **out) {
         w[N + p] = shoup_companion(val, q);
       }
     }
+  uint32_t pw[16] = {0};
+  for (int m = 0; m < 2; m++)
+    for (int w = 0; w < 7; w++) pw[m * 8 + w] = (uint32_t)powmod(2, 7 * w, kQ[m]);
   e = hipMalloc(&c->d_tw, tw.size() * 4);
   if (e == hipSuccess) e = hipMalloc(&c->d_neg1, neg1.size() * 4);
+  if (e == hipSuccess) e = hipMalloc(&c->d_pw, sizeof pw);
+  if (e == hipSuccess) e = hipMemcpy(c->d_pw, pw, sizeof pw, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_neg1, neg1.data(), neg1.size() * 4, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -188,6 +194,7 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_tw) (void)hipFree(ctx->d_tw);
   if (ctx->d_neg1) (void)hipFree(ctx->d_neg1);
+  if (ctx->d_pw) (void)hipFree(ctx->d_pw);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SP_OK;
@@ -406,7 +413,24 @@ struct sp_db {
   int nu1 = 0, nu2 = 0, n2 = 0;
   uint32_t *d_db = nullptr;  // [n2][2][R][C][P]
   size_t bytes = 0;
+  uint8_t *d_limbs = nullptr;  // int8-limb copy for the MFMA multiply, built on first use
+  bool limbs_valid = false;
 };
+
+static bool mfma_shape_ok(const sp_db *db) { return db->nu2 >= 5 && db->nu1 >= 5; }
+
+// Builds (once per load) the int8-limb copy of the database on stream s.
+static int db_limbs(sp_db *db, hipStream_t s) {
+  if (db->limbs_valid) return SP_OK;
+  if (!db->d_limbs) {
+    hipError_t e = hipMalloc(&db->d_limbs, db->bytes);
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "database limbs: %s", hipGetErrorString(e));
+  }
+  HIP_TRY(launch_db_to_limbs(db->d_db, db->n2, 1 << db->nu1, 1 << db->nu2, db->d_limbs, s));
+  db->limbs_valid = true;
+  return SP_OK;
+}
 
 extern "C" int sp_db_create(sp_ctx *ctx, int nu1, int nu2, int n2, sp_db **out) {
   if (!ctx || !out) return fail(SP_ERR_ARG, "sp_db_create: NULL");
@@ -432,6 +456,7 @@ extern "C" int sp_db_destroy(sp_db *db) {
   (void)hipSetDevice(db->ctx->device);
   (void)hipStreamSynchronize(db->ctx->stream);
   if (db->d_db) (void)hipFree(db->d_db);
+  if (db->d_limbs) (void)hipFree(db->d_limbs);
   delete db;
   return SP_OK;
 }
@@ -440,6 +465,7 @@ extern "C" size_t sp_db_device_bytes(const sp_db *db) { return db ? db->bytes : 
 
 extern "C" int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes) {
   if (!db || !host_bytes) return fail(SP_ERR_ARG, "sp_db_load_bytes: NULL");
+  db->limbs_valid = false;
   sp_ctx *ctx = db->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
   const size_t nrec = (size_t)1 << (db->nu1 + db->nu2);
@@ -459,6 +485,7 @@ extern "C" int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes) {
 
 extern "C" int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt) {
   if (!db || !host_ntt) return fail(SP_ERR_ARG, "sp_db_load_ntt: NULL");
+  db->limbs_valid = false;
   sp_ctx *ctx = db->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
   const int R = 1 << db->nu1, C = 1 << db->nu2;
@@ -495,7 +522,13 @@ struct sp_batch {
   hipEvent_t done = nullptr;      // recorded after every run
   std::vector<std::pair<int, hipGraphExec_t>> graphs;
   std::vector<hipGraph_t> graph_defs;
+  uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
+  uint32_t *mfma_out = nullptr;   // p-major products workspace
+  std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
+  std::vector<std::pair<int, float>> dbmul_ms[3];  // measurements per variant, for reporting
 };
+
+static int mfma_rows(int n) { return ((2 * n + 31) / 32) * 32; }
 
 struct sp_query {
   sp_batch *b = nullptr;
@@ -505,9 +538,10 @@ static void batch_free(sp_batch *b) {
   if (!b) return;
   for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
   for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
-  uint32_t *bufs[] = {b->query_ct, b->v, b->left, b->right, b->fkeys, b->fkeys_neg, b->acc, b->stage};
+  uint32_t *bufs[] = {b->query_ct, b->v, b->left, b->right, b->fkeys, b->fkeys_neg, b->acc, b->stage, b->mfma_out};
   for (uint32_t *p : bufs)
     if (p) (void)hipFree(p);
+  if (b->mfma_a) (void)hipFree(b->mfma_a);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->io) (void)hipStreamDestroy(b->io);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -612,6 +646,31 @@ extern "C" int sp_batch_load(sp_batch *b, int slot, const uint32_t *host_query, 
   return batch_load(b, slot, host_query, host_left, host_right, host_v_folding, nullptr);
 }
 
+static int mfma_workspace(sp_batch *b) {
+  if (b->mfma_a) return SP_OK;
+  const sp_db *db = b->db;
+  const size_t I = (size_t)mfma_rows(b->cap);
+  const size_t a_bytes = (size_t)2 * N * ((size_t)1 << db->nu1) * 4 * I;
+  const size_t o_words = (size_t)2 * N * db->n2 * I * ((size_t)1 << db->nu2);
+  hipError_t e = hipMalloc(&b->mfma_a, a_bytes);
+  if (e == hipSuccess) e = hipMalloc(&b->mfma_out, o_words * 4);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "MFMA workspace: %s", hipGetErrorString(e));
+  return SP_OK;
+}
+
+// Database-multiply variant for n queries: forced by the context, else the
+// autotuner's cached choice, else VALU (the autotuner runs in sp_batch_tune).
+static int dbmul_variant(sp_batch *b, int n) {
+  const int forced = b->db->ctx->variant[SP_STAGE_DB_MUL];
+  const bool mfma_ok = mfma_shape_ok(b->db);
+  if (forced == 2 && mfma_ok) return 2;
+  if (forced == 1 || !mfma_ok) return 1;
+  for (auto &c : b->dbmul_choice)
+    if (c.first == n) return c.second;
+  return 1;
+}
+
 // Enqueues the stages for slots [0, n) on stream s (no synchronization).
 static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
   sp_db *db = b->db;
@@ -623,8 +682,19 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
                            b->r_q, -1, s);
     if (rc) return rc;
   }
-  if (stages & 2)
-    HIP_TRY(launch_db_mul(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, ctx->tables, s));
+  if (stages & 2) {
+    const int var = dbmul_variant(b, n);
+    if (var == 2) {
+      int rc = db_limbs(db, s);
+      if (rc) return rc;
+      rc = mfma_workspace(b);
+      if (rc) return rc;
+      HIP_TRY(launch_dbmul_mfma(b->v, b->v_q, n, db->d_limbs, db->n2, R, C, mfma_rows(n), b->mfma_a, b->mfma_out,
+                                b->acc, b->acc_q, ctx->d_pw, ctx->tables, s));
+    } else {
+      HIP_TRY(launch_db_mul(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, ctx->tables, s));
+    }
+  }
   if (stages & 4) {
     HIP_TRY(launch_from_ntt(true, b->acc, b->acc, (size_t)n * db->n2 * C * 2, 2 * N, 2 * N, ctx->tables, s));
     const bool fast = b->fold_mode != 0;
@@ -667,6 +737,13 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   for (auto &ge : b->graphs)
     if (ge.first == n) exec = ge.second;
   if (!exec) {
+    // One-time preparations stay out of the graph.
+    if (dbmul_variant(b, n) == 2) {
+      int rc = db_limbs(b->db, b->cap_stream);
+      if (!rc) rc = mfma_workspace(b);
+      if (rc) return rc;
+      HIP_TRY(hipStreamSynchronize(b->cap_stream));
+    }
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
     int rc = batch_enqueue(b, n, 7, b->cap_stream);
@@ -687,6 +764,55 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   HIP_TRY(hipGraphLaunch(exec, s));
   HIP_TRY(hipEventRecord(b->done, s));
+  return SP_OK;
+}
+
+// Times the database multiply of n queries with each variant (VALU u64 and,
+// where the shape allows, int8-limb MFMA) on the batch's io stream, keeps
+// the faster one for n.  Needs the expansion (stage 1) to have run.
+extern "C" int sp_batch_tune(sp_batch *b, int n, float *ms_valu, float *ms_mfma, int *chosen) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_tune: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_tune: n = %d", n);
+  if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_tune: nothing loaded");
+  sp_ctx *ctx = b->db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));
+  const int saved = ctx->variant[SP_STAGE_DB_MUL];
+  float best = 0;
+  int pick = 1;
+  float ms[3] = {-1, -1, -1};
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = SP_OK;
+  for (int var = 1; var <= 2 && !rc; var++) {
+    if (var == 2 && !mfma_shape_ok(b->db)) break;
+    ctx->variant[SP_STAGE_DB_MUL] = var;
+    rc = batch_enqueue(b, n, 2, b->io);  // warm-up (builds limbs / workspaces)
+    const int reps = 3;
+    if (!rc && hipEventRecord(e0, b->io) != hipSuccess) rc = fail(SP_ERR_HIP, "event");
+    for (int k = 0; k < reps && !rc; k++) rc = batch_enqueue(b, n, 2, b->io);
+    if (!rc && hipEventRecord(e1, b->io) != hipSuccess) rc = fail(SP_ERR_HIP, "event");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(SP_ERR_HIP, "event sync");
+    float t = 0;
+    if (!rc) {
+      (void)hipEventElapsedTime(&t, e0, e1);
+      ms[var] = t / reps;
+      b->dbmul_ms[var].push_back({n, ms[var]});
+      if (var == 1 || ms[var] < best) { best = ms[var]; pick = var; }
+    }
+  }
+  ctx->variant[SP_STAGE_DB_MUL] = saved;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  bool found = false;
+  for (auto &c : b->dbmul_choice)
+    if (c.first == n) { c.second = pick; found = true; }
+  if (!found) b->dbmul_choice.push_back({n, pick});
+  if (ms_valu) *ms_valu = ms[1];
+  if (ms_mfma) *ms_mfma = ms[2];
+  if (chosen) *chosen = pick;
   return SP_OK;
 }
 
@@ -734,10 +860,18 @@ extern "C" int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out) {
   HIP_TRY(hipSetDevice(db->ctx->device));
   HIP_TRY(hipEventSynchronize(b->done));
   const int C = 1 << db->nu2;
-  // Response t of slot q is the first ciphertext of (q, t): strided 2-D copy.
-  HIP_TRY(hipMemcpy2DAsync(host_out, (size_t)4 * N * 4, b->acc, (size_t)C * 4 * N * 4, (size_t)4 * N * 4,
-                           (size_t)n * db->n2, hipMemcpyDeviceToHost, b->io));
-  HIP_TRY(hipStreamSynchronize(b->io));
+  // Response t of slot q is the first ciphertext of (q, t), C ciphertexts apart.
+  // They are gathered into the staging buffer by a kernel and copied out
+  // contiguously: strided 2-D device-to-pageable copies have returned stale
+  // bytes under kernel tracing, plain contiguous copies have not.
+  const size_t rows = (size_t)n * db->n2;
+  const size_t chunk = std::min<size_t>(b->stage_words / (4 * N), 65535);
+  for (size_t r0 = 0; r0 < rows; r0 += chunk) {
+    const size_t nr = std::min(chunk, rows - r0);
+    HIP_TRY(launch_gather_rows(b->acc + r0 * C * 4 * N, (size_t)C * 4 * N, (int)nr, b->stage, b->io));
+    HIP_TRY(hipMemcpyAsync(host_out + r0 * 4 * N, b->stage, nr * 4 * N * 4, hipMemcpyDeviceToHost, b->io));
+    HIP_TRY(hipStreamSynchronize(b->io));
+  }
   return SP_OK;
 }
 

@@ -772,6 +772,20 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
   k_copy_queries<<<dim3(2, nq), 256, 0, s>>>(query, query_qstride, v, v_qstride);
   return hipGetLastError();
 }
+
+// One ciphertext (4N words) per blockIdx.y, gathered into consecutive rows.
+__global__ __launch_bounds__(256) void k_gather_rows(const uint32_t *__restrict__ src, size_t src_stride,
+                                                     uint32_t *__restrict__ dst) {
+  const uint4 *from = reinterpret_cast<const uint4 *>(src + (size_t)blockIdx.y * src_stride);
+  uint4 *to = reinterpret_cast<uint4 *>(dst + (size_t)blockIdx.y * 4 * N);
+  for (int u = blockIdx.x * 256 + threadIdx.x; u < N; u += gridDim.x * 256) to[u] = from[u];
+}
+
+hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s) {
+  if (rows < 1) return hipSuccess;
+  k_gather_rows<<<dim3(2, rows), 256, 0, s>>>(src, src_stride, dst);
+  return hipGetLastError();
+}
 }  // namespace sp
 
 // ===========================================================================
@@ -1066,6 +1080,212 @@ hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, in
   dim3 grid(num_out, n_queries);
   k_expand8<8><<<grid, 512, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride, t.tw,
                                     t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+}  // namespace sp
+
+// ===========================================================================
+// Batched database multiply on int8 MFMA (gfx950 v_mfma_i32_32x32x32_i8).
+//
+// Per (trial t, residue m, position p) the batch computes the modular
+// product O[i][c] = sum_j A[i][j] * D[j][c] mod q with i = 2*query + row.
+// Residues (< 2^28) split into four 7-bit limbs, so with A = sum_a 2^7a A_a
+// and D = sum_b 2^7b D_b,  O = sum_w 2^7w P_w,  P_w = sum_{a+b=w} A_a D_b:
+// 16 exact int8 products accumulated in 7 int32 weights (each below
+// 4 * 1024 * 127^2 < 2^31 for up to 1024 rows), combined mod q at the end.
+// Operand layouts (bytes, K = j contiguous in blocks of 32):
+//   A limbs  [m][p][jb][limb][I][32]      I = padded 2*batch rows
+//   D limbs  [t][m][p][jb][limb][C][32]
+// Output (u32, p-major)  [m][p][t][I][C], transposed afterwards into the
+// per-query accumulator layout [b][t][c][row][m][P].
+// A lane of the 32x32x32 instruction holds 16 K-bytes of one row of A (and
+// of one column of B); both operands are read with the same byte->K mapping,
+// so the K order inside the instruction does not matter.
+// ===========================================================================
+namespace sp {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Selector limbs from the expanded ciphertexts of a batch.  Block: 4 rows i
+// x 32 j x 64 positions through LDS; writes 128-byte runs.
+__global__ __launch_bounds__(256) void k_sel_to_limbs(const uint32_t *__restrict__ v, size_t v_q, int nq, int R,
+                                                      int I, uint8_t *__restrict__ A) {
+  __shared__ uint32_t tile[4][32][65];
+  const int pb = blockIdx.x;           // position block of 64
+  const int jb = blockIdx.y;           // j block of 32
+  const int m = blockIdx.z & 1;
+  const int i0 = (blockIdx.z >> 1) * 4;
+  const int JB = R / 32;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < 4 * 32 * 64; k += 256) {
+    const int p = k & 63, j = (k >> 6) & 31, ii = k >> 11;
+    const int i = i0 + ii;
+    uint32_t x = 0;
+    if ((i >> 1) < nq)
+      x = v[(size_t)(i >> 1) * v_q + (size_t)(2 * (jb * 32 + j)) * 4 * N + (size_t)((i & 1) * 2 + m) * N + pb * 64 + p];
+    tile[ii][j][p] = x;
+  }
+  __syncthreads();
+  // 64 positions x 4 limbs x 4 rows x 32 bytes: thread -> (p, limb), 128 bytes.
+  {
+    const int p = tid & 63, limb = tid >> 6;
+    uint32_t words[32];
+#pragma unroll
+    for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+      for (int w = 0; w < 8; w++) {
+        uint32_t packed = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) packed |= ((tile[ii][w * 4 + b][p] >> (7 * limb)) & 127u) << (8 * b);
+        words[ii * 8 + w] = packed;
+      }
+    uint4 *dst = reinterpret_cast<uint4 *>(A + (((((size_t)m * N + pb * 64 + p) * JB + jb) * 4 + limb) * I + i0) * 32);
+#pragma unroll
+    for (int u = 0; u < 8; u++) dst[u] = make_uint4(words[4 * u], words[4 * u + 1], words[4 * u + 2], words[4 * u + 3]);
+  }
+}
+
+// Database limbs from the resident u32 database [t][m][j][c][P] (one-time).
+__global__ __launch_bounds__(256) void k_db_to_limbs(const uint32_t *__restrict__ db, int R, int C,
+                                                     uint8_t *__restrict__ D) {
+  __shared__ uint32_t tile[4][32][65];
+  const int pb = blockIdx.x, jb = blockIdx.y;
+  const int cblocks = C / 4;
+  const int c0 = (blockIdx.z % cblocks) * 4;
+  const int tm = blockIdx.z / cblocks;  // t * 2 + m
+  const int JB = R / 32;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < 4 * 32 * 64; k += 256) {
+    const int p = k & 63, j = (k >> 6) & 31, cc = k >> 11;
+    tile[cc][j][p] = db[(((size_t)tm * R + jb * 32 + j) * C + c0 + cc) * N + pb * 64 + p];
+  }
+  __syncthreads();
+  const int p = tid & 63, limb = tid >> 6;
+  uint32_t words[32];
+#pragma unroll
+  for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      uint32_t packed = 0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) packed |= ((tile[cc][w * 4 + b][p] >> (7 * limb)) & 127u) << (8 * b);
+      words[cc * 8 + w] = packed;
+    }
+  uint4 *dst = reinterpret_cast<uint4 *>(D + (((((size_t)tm * N + pb * 64 + p) * JB + jb) * 4 + limb) * C + c0) * 32);
+#pragma unroll
+  for (int u = 0; u < 8; u++) dst[u] = make_uint4(words[4 * u], words[4 * u + 1], words[4 * u + 2], words[4 * u + 3]);
+}
+
+// One block = 8 waves = a 128 (i) x 64 (c) output tile of one (t, m, p).
+// Blocks of one XCD take consecutive trials of the same (m, p) so the
+// selector limbs are read from HBM about once.
+__global__ __launch_bounds__(512) void k_dbmul_mfma(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
+                                                    uint32_t *__restrict__ outp, int T, int R, int C, int I,
+                                                    int n_ig, int n_cg, ModConst mc0, ModConst mc1,
+                                                    const uint32_t *__restrict__ pw) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // block id -> (t fastest, then c group, i group, position, residue)
+  int bid = blockIdx.x;
+  const int t = bid % T; bid /= T;
+  const int cg = bid % n_cg; bid /= n_cg;
+  const int ig = bid % n_ig; bid /= n_ig;
+  const int p = bid % N;
+  const int m = bid / N;
+  const int row0 = ig * 128 + (wave >> 1) * 32;
+  const int col0 = cg * 64 + (wave & 1) * 32;
+  if (row0 >= I || col0 >= C) return;
+  const int JB = R / 32;
+  const int h = lane >> 5, l32 = lane & 31;
+  const uint8_t *a_base = A + ((((size_t)m * N + p) * JB) * 4 * I + row0 + l32) * 32 + 16 * h;
+  const uint8_t *d_base = D + (((((size_t)t * 2 + m) * N + p) * JB) * 4 * C + col0 + l32) * 32 + 16 * h;
+  const size_t a_limb = (size_t)I * 32, a_jb = 4 * a_limb;
+  const size_t d_limb = (size_t)C * 32, d_jb = 4 * d_limb;
+  v16i acc[7];
+#pragma unroll
+  for (int w = 0; w < 7; w++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[w][v] = 0;
+  v4i fa[4], fd[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    fa[a] = *reinterpret_cast<const v4i *>(a_base + a * a_limb);
+    fd[a] = *reinterpret_cast<const v4i *>(d_base + a * d_limb);
+  }
+  for (int jb = 0; jb < JB; jb++) {
+    v4i na[4], nd[4];
+    const int nj = jb + 1 < JB ? jb + 1 : jb;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      na[a] = *reinterpret_cast<const v4i *>(a_base + nj * a_jb + a * a_limb);
+      nd[a] = *reinterpret_cast<const v4i *>(d_base + nj * d_jb + a * d_limb);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fd[b], acc[a + b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 4; a++) { fa[a] = na[a]; fd[a] = nd[a]; }
+  }
+  const ModConst &mc = m ? mc1 : mc0;
+  const uint32_t *pwm = pw + m * 8;
+  const int c = col0 + l32;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
+    uint64_t s = 0;
+#pragma unroll
+    for (int w = 0; w < 7; w++) s += (uint64_t)(uint32_t)acc[w][v] * pwm[w];
+    outp[((((size_t)m * N + p) * T + t) * I + i) * C + c] = mod_u64(s, mc);
+  }
+}
+
+// p-major products [m][p][t][I][C] -> per-query accumulators [b][t][c][row][m][P].
+__global__ __launch_bounds__(256) void k_outp_to_acc(const uint32_t *__restrict__ outp, int T, int I, int C, int nq,
+                                                     uint32_t *__restrict__ acc, size_t acc_q) {
+  __shared__ uint32_t tile[64][65];
+  const int pb = blockIdx.x;                       // 64 positions
+  const int cb = blockIdx.y;                       // 64 (t, i, c) combinations
+  const int m = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int TIC = T * I * C;
+  for (int k = tid; k < 64 * 64; k += 256) {
+    const int x = k & 63, pp = k >> 6;
+    const int e = cb * 64 + x;
+    tile[pp][x] = e < TIC ? outp[((size_t)m * N + pb * 64 + pp) * TIC + e] : 0u;
+  }
+  __syncthreads();
+  for (int k = tid; k < 64 * 64; k += 256) {
+    const int pp = k & 63, x = k >> 6;
+    const int e = cb * 64 + x;
+    if (e >= TIC) continue;
+    const int c = e % C, i = (e / C) % I, t = e / (C * I);
+    const int b = i >> 1, row = i & 1;
+    if (b >= nq) continue;
+    acc[(size_t)b * acc_q + (((size_t)t * C + c) * 2 + row) * 2 * N + (size_t)m * N + pb * 64 + pp] = tile[pp][x];
+  }
+}
+
+hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *D, hipStream_t s) {
+  dim3 grid(N / 64, R / 32, T * 2 * (C / 4));
+  k_db_to_limbs<<<grid, 256, 0, s>>>(db, R, C, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
+                             uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
+                             const DevTables &t, hipStream_t s) {
+  k_sel_to_limbs<<<dim3(N / 64, R / 32, 2 * (I / 4)), 256, 0, s>>>(v, v_q, nq, R, I, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int n_ig = (I + 127) / 128, n_cg = (C + 63) / 64;
+  const unsigned blocks = (unsigned)(2 * N * n_ig * n_cg * T);
+  k_dbmul_mfma<<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int TIC = T * I * C;
+  k_outp_to_acc<<<dim3(N / 64, (TIC + 63) / 64, 2), 256, 0, s>>>(outp, T, I, C, nq, acc, acc_q);
   return hipGetLastError();
 }
 

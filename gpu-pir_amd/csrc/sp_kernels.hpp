@@ -42,6 +42,8 @@ hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int
                                 size_t right_qstride, int t_exp, int bits, const DevTables &t, hipStream_t s);
 hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t v_qstride, int nq,
                                hipStream_t s);
+// rows of 4N words at src + r * src_stride -> contiguous dst (response gather for readout).
+hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s);
 
 // Gadget 2^7 x 8 (t_gsw = t_exp = 8) round-structured kernels, 8 waves per block.
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
@@ -49,5 +51,14 @@ hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_q
 hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                           const DevTables &t, hipStream_t s);
+
+// int8-limb MFMA database multiply (C >= 32, R multiple of 32).
+//   D: database limbs [t][m][p][R/32][4][C][32] (k_db_to_limbs, one-time)
+//   A: workspace for selector limbs, 2 * N * R * 4 * I bytes (I = 2*nq padded to 32)
+//   outp: workspace, 2 * N * T * I * C words;  pw: [m][8] = 2^(7w) mod q_m
+hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *D, hipStream_t s);
+hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
+                             uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
+                             const DevTables &t, hipStream_t s);
 
 }  // namespace sp

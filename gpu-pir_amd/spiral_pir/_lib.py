@@ -67,6 +67,7 @@ SIGNATURES = {
     "sp_batch_read": [P, I, I, P],
     "sp_batch_read_responses": [P, I, P],
     "sp_batch_synchronize": [P],
+    "sp_batch_tune": [P, I, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), PI],
     "sp_batch_destroy": [P],
     "sp_client_create": [I, I, I, I, I, U64, PP],
     "sp_client_destroy": [P],

@@ -176,6 +176,13 @@ class Batch:
     def synchronize(self) -> None:
         _lib.call("sp_batch_synchronize", self._h)
 
+    def tune(self, n: int) -> dict:
+        """Autotune the database multiply for n queries (after one expansion has run)."""
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        _lib.call("sp_batch_tune", self._h, n, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"valu_ms": a.value, "mfma_ms": b.value if b.value >= 0 else None,
+                "chosen": {1: "valu", 2: "mfma"}[c.value]}
+
     def read(self, slot: int, what: int = READ_RESPONSE) -> np.ndarray:
         s = self.db.shape
         shapes = {READ_RESPONSE: (s.n2, 2, 2, 2048), READ_EXPANDED: (1 << s.g, 2, 2, 2048),

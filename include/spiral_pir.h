@@ -192,6 +192,12 @@ int sp_batch_graph_launch(sp_batch *b, int n, void *stream);
 int sp_batch_read(sp_batch *b, int slot, int what, uint32_t *host_out);
 int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out);
 int sp_batch_synchronize(sp_batch *b);
+/* Autotuner: times the database multiply of n queries with the VALU u64 and
+ * the int8-limb MFMA kernels (the latter needs 2^nu1 and 2^nu2 >= 32), keeps
+ * the faster for later runs of n queries and reports both times (ms; -1 when
+ * a variant does not apply) and the choice (1 VALU, 2 MFMA).  Run it after the
+ * batch's expansion has run once; it overwrites the accumulators. */
+int sp_batch_tune(sp_batch *b, int n, float *ms_valu, float *ms_mfma, int *chosen);
 int sp_batch_destroy(sp_batch *b);
 
 /* One query against a resident database, stage by stage on the device:

@@ -149,3 +149,164 @@ def test_batch_config2_shape(gpu_ctx):
     single.close()
     b.close()
     db.close()
+
+
+@pytest.fixture
+def force_dbmul(gpu_ctx):
+    def setv(v):
+        gpu_ctx.set_variant(2, v)
+    yield setv
+    gpu_ctx.set_variant(2, 0)
+
+
+@pytest.mark.parametrize("nu1,nu2,B", [(6, 5, 5), (5, 6, 16), (7, 5, 1)])
+def test_dbmul_mfma_matches_oracle(gpu_ctx, oracle, force_dbmul, nu1, nu2, B):
+    """The int8-limb MFMA database multiply gives the same accumulators as the oracle
+    (and so the same responses), including batches that do not fill the 32-row tiles."""
+    from spiral_pir.server import Batch, READ_DB_MUL, READ_EXPANDED, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD
+
+    s, recs, db = _setup(gpu_ctx, nu1, nu2, 910 + nu1)
+    qs = _queries(s, B, 3500 + nu2)
+    b = Batch(db, B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    force_dbmul(2)
+    b.run(B, STAGE_EXPAND)
+    b.run(B, STAGE_DB_MUL)
+    db_ntt = oracle.db_encode(recs, s.n2, nu1, nu2)
+    for slot in sorted({0, B - 1}):
+        v = b.read(slot, READ_EXPANDED)
+        sel = np.ascontiguousarray(v[0:2 * s.rows:2])
+        acc = b.read(slot, READ_DB_MUL)
+        for t in range(s.n2):
+            np.testing.assert_array_equal(acc[t], oracle.db_mul(db_ntt[t], sel, nu1, nu2, nthreads=0),
+                                          err_msg=f"slot {slot} trial {t}")
+    b.run(B, STAGE_FOLD)
+    resp = b.responses(B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        np.testing.assert_array_equal(c.decode(resp[slot]), recs[idx])
+    b.close()
+    db.close()
+
+
+def test_dbmul_mfma_extreme_residues(gpu_ctx, oracle, force_dbmul):
+    """A database of all-(q-1) residues (every limb at its maximum, 512 rows) through the
+    MFMA multiply: the int32 limb sums stay exact, against the oracle."""
+    from spiral_pir import Shape
+    from spiral_pir.server import Batch, Database, READ_DB_MUL, READ_EXPANDED, STAGE_DB_MUL, STAGE_EXPAND
+
+    s = Shape(9, 5)
+    R, C = s.rows, s.cols
+    Q0, Q1 = 268369921, 249561089
+    db_ntt = np.empty((s.n2, C, R, 2, N), np.uint32)
+    db_ntt[:, :, :, 0] = Q0 - 1
+    db_ntt[:, :, :, 1] = Q1 - 1
+    db = Database(gpu_ctx, s)
+    db.load_ntt(db_ntt)
+    qs = _queries(s, 2, 3600)
+    b = Batch(db, 2)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    force_dbmul(2)
+    b.run(2, STAGE_EXPAND)
+    b.run(2, STAGE_DB_MUL)
+    v = b.read(1, READ_EXPANDED)
+    sel = np.ascontiguousarray(v[0:2 * R:2])
+    acc = b.read(1, READ_DB_MUL)
+    np.testing.assert_array_equal(acc[0], oracle.db_mul(db_ntt[0], sel, s.nu1, s.nu2, nthreads=0))
+    b.close()
+    db.close()
+
+
+def test_autotuner_picks_and_keeps_results(gpu_ctx):
+    from spiral_pir.server import Batch, STAGE_EXPAND
+
+    s, recs, db = _setup(gpu_ctx, 6, 5, 920)
+    qs = _queries(s, 8, 3700)
+    b = Batch(db, 8)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    b.run(8)
+    ref = b.responses(8)
+    b.run(8, STAGE_EXPAND)
+    info = b.tune(8)
+    assert info["chosen"] in ("valu", "mfma") and info["valu_ms"] > 0 and info["mfma_ms"] > 0
+    b.run(8)
+    np.testing.assert_array_equal(b.responses(8), ref)
+    b.graph_launch(8)
+    np.testing.assert_array_equal(b.responses(8), ref)
+    b.close()
+    db.close()
+
+
+def _clients_queries(s, B, n_clients, seed):
+    from spiral_pir.client import Client
+
+    clients = [Client(s, seed + k) for k in range(n_clients)]
+    keys = [c.keygen() for c in clients]
+    idx = [int(x) for x in rng(seed + 100).integers(0, s.records, size=B)]
+    qs = []
+    for slot in range(B):
+        c = slot % n_clients
+        q, vf = clients[c].query(idx[slot])
+        qs.append((clients[c], idx[slot], q, keys[c][0], keys[c][1], vf))
+    return qs
+
+
+def test_dbmul_mfma_partial_row_tile(gpu_ctx, force_dbmul):
+    """48 queries = 96 selector rows: the last 32-row wave tile of the 128-row block is
+    empty and returns early; accumulators equal the VALU multiply's for every read slot."""
+    from spiral_pir.server import Batch, READ_DB_MUL, STAGE_DB_MUL, STAGE_EXPAND
+
+    s, recs, db = _setup(gpu_ctx, 9, 5, 930)
+    B = 48
+    qs = _clients_queries(s, B, 4, 3800)
+    b = Batch(db, B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    b.run(B, STAGE_EXPAND)
+    force_dbmul(1)
+    b.run(B, STAGE_DB_MUL)
+    slots = (0, 31, 47)
+    ref = {k: b.read(k, READ_DB_MUL) for k in slots}
+    force_dbmul(2)
+    b.run(B, STAGE_DB_MUL)
+    for k in slots:
+        np.testing.assert_array_equal(b.read(k, READ_DB_MUL), ref[k], err_msg=f"slot {k}")
+    b.close()
+    db.close()
+
+
+def test_tuned_batch_64_at_9_6(gpu_ctx):
+    """The benchmark's batched configuration: (9, 6), 64 queries from 8 clients, the
+    autotuned database multiply, graph replays.  Every slot decodes, the graph result
+    equals the VALU-forced eager result, and the bulk response readout equals the
+    per-slot readout."""
+    import torch
+    from spiral_pir.server import Batch, READ_RESPONSE, STAGE_EXPAND
+
+    s, recs, db = _setup(gpu_ctx, 9, 6, 1001)
+    B = 64
+    qs = _clients_queries(s, B, 8, 3900)
+    b = Batch(db, B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    b.run(B, STAGE_EXPAND)
+    info = b.tune(B)
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        b.graph_launch(B, stream)
+    b.synchronize()
+    got = b.responses(B)
+    for slot in (0, 17, 63):
+        np.testing.assert_array_equal(b.read(slot, READ_RESPONSE), got[slot], err_msg=f"slot {slot}")
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        np.testing.assert_array_equal(c.decode(got[slot]), recs[idx], err_msg=f"slot {slot} ({info})")
+    gpu_ctx.set_variant(2, 1)
+    try:
+        b.run(B)
+        np.testing.assert_array_equal(b.responses(B), got)
+    finally:
+        gpu_ctx.set_variant(2, 0)
+    b.close()
+    db.close()
