@@ -25,6 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-pir_amd"))
 
 METRIC = "PIR queries/s + effective DB-scan GB/s, batched vs 1-query; p50/p99 latency"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per k_db_mul launch for one query, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs scaled by a plain copy kernel's readings (FETCH x2.000, WRITE x1.000);
+# profiles/db_mul_hbm_traffic_nu9_6.md.  Measured shapes only.
+DB_MUL_TRAFFIC_BYTES = {(9, 6): (2142755 + 8192) * 1024}
 
 
 def db_mul_algorithmic_bytes(shape) -> int:
@@ -255,7 +259,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": DB_MUL_TRAFFIC_BYTES.get((shape.nu1, shape.nu2)),
+            "traffic_unit": "bytes per launch",
         },
         "batched": None if batched is None else {k: v for k, v in batched.items() if k != "decoded_all_correct"},
         "ingest_s": t_ingest,

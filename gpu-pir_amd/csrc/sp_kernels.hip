@@ -1179,15 +1179,17 @@ __global__ __launch_bounds__(256) void k_db_to_limbs(const uint32_t *__restrict_
 }
 
 // One block = 8 waves = a 128 (i) x 64 (c) output tile of one (t, m, p).
-// Blocks of one XCD take consecutive trials of the same (m, p) so the
-// selector limbs are read from HBM about once.
+// Workgroups are dealt to the 8 XCDs round-robin (hardware block b runs on
+// XCD b % 8); each XCD is given one contiguous range of logical tiles, so the
+// tiles of one (m, p) for all trials and column groups share an L2 and the
+// selector limbs are read from HBM about once.  gridDim.x is a multiple of 8.
 __global__ __launch_bounds__(512) void k_dbmul_mfma(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
                                                     uint32_t *__restrict__ outp, int T, int R, int C, int I,
                                                     int n_ig, int n_cg, ModConst mc0, ModConst mc1,
                                                     const uint32_t *__restrict__ pw) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  // block id -> (t fastest, then c group, i group, position, residue)
-  int bid = blockIdx.x;
+  // logical tile -> (t fastest, then c group, i group, position, residue)
+  int bid = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
   const int t = bid % T; bid /= T;
   const int cg = bid % n_cg; bid /= n_cg;
   const int ig = bid % n_ig; bid /= n_ig;
@@ -1280,7 +1282,7 @@ hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int n_ig = (I + 127) / 128, n_cg = (C + 63) / 64;
-  const unsigned blocks = (unsigned)(2 * N * n_ig * n_cg * T);
+  const unsigned blocks = (unsigned)(2 * N * n_ig * n_cg * T);  // 2N is a multiple of 8
   k_dbmul_mfma<<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
   e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -32,6 +32,23 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def kat():
+    """The reference's known-answer vectors for this path, committed as data
+    (tests/golden/reference_kat.npz, written by tests/golden/make_reference_kat.py)."""
+    with np.load(os.path.join(ROOT, "tests", "golden", "reference_kat.npz")) as d:
+        return {k: d[k] for k in d.files}
+
+
+def kat_expansion_keys(kat):
+    """Left/right key arrays for the (6, 2) round-0 vector: only right level 0 is read
+    by round 0; the other levels are zero."""
+    right = np.zeros((5, 2, 8, 2, 2048), np.uint32)
+    right[0] = kat["expand_right_keys0"][0]
+    left = np.zeros((7, 2, 8, 2, 2048), np.uint32)
+    return left, right
+
+
+@pytest.fixture(scope="session")
 def ref_data():
     """Directory of the reference's fixture files (present only where the reference is mounted)."""
     if not os.path.isdir(os.path.join(REF_DATA, "test_data")):

@@ -83,3 +83,42 @@ def test_db_mul_kat_inputs_are_unusable_without_db(ref_data):
     multiply is pinned by the closed form and brute force instead."""
     assert os.path.exists(os.path.join(ref_data, "data", "eric_first_dim.dat"))
     assert not os.path.exists(os.path.join(ref_data, "data", "eric_db.dat"))
+
+
+# ---- the same vectors from tests/golden/reference_kat.npz (run everywhere) ----
+
+def test_golden_vectors_are_the_reference_files(oracle, ref_data, kat):
+    """reference_kat.npz holds exactly what the reference's files hold."""
+    np.testing.assert_array_equal(kat["raw_poly"], oracle.read_raw_polys(_td(ref_data, "raw_poly.dat"))[0])
+    np.testing.assert_array_equal(kat["ntt_poly"], oracle.read_ntt_polys(_td(ref_data, "ntt_poly.dat"))[0])
+    np.testing.assert_array_equal(
+        kat["folding_keys"],
+        oracle.matrices_to_keys(oracle.read_ntt_polys(_td(ref_data, "client_folding_keys.dat")), 2, 16))
+    np.testing.assert_array_equal(
+        kat["expand_right_keys0"],
+        oracle.matrices_to_keys(oracle.read_ntt_polys(_td(ref_data, "client_expand_right_keys.dat")), 2, 8)[:1])
+    np.testing.assert_array_equal(
+        kat["fold_output"], oracle.read_raw_polys(_td(ref_data, "client_fold_output.dat")).reshape(2, 2, 2048))
+    np.testing.assert_array_equal(
+        kat["expand_output_1round"],
+        oracle.read_ntt_polys(_td(ref_data, "client_expand_output_1round.dat")).reshape(2, 2, 2, 2048))
+
+
+def test_golden_ntt(oracle, kat):
+    np.testing.assert_array_equal(oracle.to_ntt(kat["raw_poly"]), kat["ntt_poly"])
+    np.testing.assert_array_equal(oracle.from_ntt(kat["ntt_poly"]), kat["raw_poly"])
+
+
+def test_golden_fold(oracle, kat):
+    keys = kat["folding_keys"]
+    got = oracle.fold(kat["fold_input"], 2, 8, keys, oracle.folding_neg(keys, 2, 8))
+    np.testing.assert_array_equal(got[0], kat["fold_output"])
+
+
+def test_golden_expansion_round0(oracle, kat):
+    from conftest import kat_expansion_keys
+
+    left, right = kat_expansion_keys(kat)
+    v = oracle.expand(kat["expand_input"], 6, 2, 8, 8, left, right, rounds=1)
+    np.testing.assert_array_equal(v[0], kat["expand_output_1round"][0])
+    np.testing.assert_array_equal(v[1], kat["expand_output_1round"][1])
