@@ -44,7 +44,8 @@ static int fail(int code, const char *fmt, ...) {
 
 extern "C" const char *sp_last_error(void) { return g_err.c_str(); }
 extern "C" const char *sp_version(void) {
-  return "spiral_hip 0.1 gfx950: wave64 NTT (Shoup/Harvey), db_mul u64-lazy VALU, fold fast+general";
+  return "spiral_hip 0.2 gfx950: wave64 NTT (Shoup/Harvey, lazy digit stages), expansion/fold with digit-owning "
+         "waves (8- and 16-wave blocks), db_mul u64-lazy VALU + int8-limb MFMA, fold fast+general";
 }
 
 // ---------------------------------------------------------------------------
@@ -184,6 +185,12 @@ extern "C" int sp_ctx_create(int device, sp_ctx **out) {
   c->tables.neg1 = c->d_neg1;
   c->tables.mc[0] = make_modconst(0);
   c->tables.mc[1] = make_modconst(1);
+  c->tables.n_cu = 256;
+  {
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n_cu > 0)
+      c->tables.n_cu = n_cu;
+  }
   *out = c;
   return SP_OK;
 }
@@ -201,7 +208,7 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
 }
 
 extern "C" int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant) {
-  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > 2)
+  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > (stage == SP_STAGE_DB_MUL ? 2 : 3))
     return fail(SP_ERR_ARG, "sp_ctx_set_variant: stage %d variant %d", stage, variant);
   ctx->variant[stage] = variant;
   return SP_OK;
@@ -326,7 +333,7 @@ extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, cons
     num_per >>= 1;
     const size_t lvl = (size_t)(nu2 - 1 - cur) * level_polys * N;
     if (fast && t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
-      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, 0, ctx->tables, s));
+      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, 0, ctx->tables, ctx->variant[SP_STAGE_FOLD], s));
     else
       HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, fast ? nullptr : kn.u32() + lvl, 0,
                                 t_gsw, gadget_bits(t_gsw), ctx->tables, s));
@@ -371,7 +378,7 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
     const bool rounds8 = t_exp == 8 && ctx->variant[SP_STAGE_EXPAND] != 1;
     if (rounds8)
       HIP_TRY(launch_expand8(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
-                             right_qstride, ctx->tables, s));
+                             right_qstride, ctx->tables, ctx->variant[SP_STAGE_EXPAND], s));
     else
       HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
                                    right_qstride, t_exp, gadget_bits(t_exp), ctx->tables, s));
@@ -705,7 +712,8 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
       num_per >>= 1;
       const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
       if (fast && b->t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
-        HIP_TRY(launch_fold8(b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off, b->f_q, ctx->tables, s));
+        HIP_TRY(launch_fold8(b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off, b->f_q, ctx->tables,
+                             ctx->variant[SP_STAGE_FOLD], s));
       else
         HIP_TRY(launch_fold_level(fast, b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off,
                                   fast ? nullptr : b->fkeys_neg + off, b->f_q, b->t_gsw, gadget_bits(b->t_gsw),

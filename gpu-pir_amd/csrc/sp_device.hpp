@@ -10,8 +10,8 @@
 //    32 coefficients in VGPRs.  No workgroup barrier is needed inside a
 //    transform: the single index transpose goes through a wave-private LDS
 //    tile (2112 words: 2 pad words per 64 keep both access patterns free of
-//    bank conflicts) and the one cross-lane butterfly stage uses a DPP
-//    neighbour swap.
+//    bank conflicts with one base address per lane, see tile_at) and the one
+//    cross-lane butterfly stage uses a DPP neighbour swap.
 //  * Butterflies are Harvey's lazy Cooley-Tukey / Gentleman-Sande forms with
 //    Shoup multiplication (w, w' = floor(w * 2^32 / q)): 3 integer multiplies
 //    per butterfly.  Forward values live in [0, 4q), inverse in [0, 2q).
@@ -89,6 +89,37 @@ __device__ __forceinline__ void ct_bf(uint32_t &x, uint32_t &y, uint32_t w, uint
   x = xr + t;
 }
 
+// Forward butterfly without the reduction of x: x' = x + t, y' = x - t + 2q
+// with t = y * w mod q in [0, 2q).  If x, y < B q then x', y' < (B + 2) q.
+__device__ __forceinline__ void ct_bf_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
+  const uint32_t t = shoup_mul(y, w, wp, q);
+  y = x - t + 2 * q;
+  x = x + t;
+}
+
+// Forward butterfly for any x < 2^32: x is first brought to [0, 2q) by a
+// Shoup multiplication by 1 (mu1 = floor(2^32 / q)); outputs in [0, 4q).
+__device__ __forceinline__ void ct_bf_full(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
+                                           uint32_t mu1) {
+  const uint32_t xr = x - __umulhi(x, mu1) * q;
+  const uint32_t t = shoup_mul(y, w, wp, q);
+  y = xr - t + 2 * q;
+  x = xr + t;
+}
+
+// Stage st (1..11) of a forward transform whose first LAZY stages skip the
+// reduction of x; stage LAZY + 1 reduces x fully, later stages are Harvey's.
+template <int LAZY>
+__device__ __forceinline__ void ct_stage_bf(int st, uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
+                                            uint32_t mu1) {
+  if (st <= LAZY)
+    ct_bf_lazy(x, y, w, wp, q);
+  else if (LAZY > 0 && st == LAZY + 1)
+    ct_bf_full(x, y, w, wp, q, mu1);
+  else
+    ct_bf(x, y, w, wp, q);
+}
+
 __device__ __forceinline__ void gs_bf(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
   // Harvey inverse butterfly: inputs [0, 2q), outputs [0, 2q).
   const uint32_t tq = 2 * q;
@@ -123,6 +154,12 @@ __device__ __forceinline__ uint32_t swap_lane_pair(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
 }
 
+// Transpose tile slot of coefficient 64*i + l (phase-A register i, lane l):
+// rows of 66 words.  A phase-A access (fixed i, all lanes) and a phase-B
+// access (lane 2L+b reading row L, column 2i+b) both touch 64 distinct banks,
+// and both are one per-lane base plus a constant offset per register.
+__device__ __forceinline__ int tile_at(int i, int l) { return 66 * i + l; }
+
 __device__ __forceinline__ uint32_t tw_a(const uint32_t *__restrict__ tw, int idx, int k) {
   return tw[idx * 2 + k];
 }
@@ -133,8 +170,13 @@ __device__ __forceinline__ uint32_t tw_b(const uint32_t *__restrict__ tw, int sl
 // Forward negacyclic NTT of one polynomial held by one wave.
 // In:  r[i] = coefficient 64*i + lane, values in [0, 4q).
 // Out: r[j] = NTT position 32*lane + j (natural Spiral order), in [0, 4q).
-__device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
-                                             uint32_t *__restrict__ lds, uint32_t q, int lane) {
+// LAZY > 0: the first LAZY stages skip the reduction of x, which is valid
+// when the inputs are below B q with B + 2 LAZY <= 16 (both moduli satisfy
+// 16 q < 2^32): up to 7 for inputs below 2q.  mu1 = floor(2^32 / q).
+template <int LAZY>
+__device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
+                                               uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane) {
+  static_assert(LAZY >= 0 && LAZY <= 7, "at most 7 lazy stages");
   // Phase A: strides 1024 .. 64 (index bits 10..6 live in the register index).
 #pragma unroll
   for (int s = 0; s < 5; s++) {
@@ -143,16 +185,17 @@ __device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *
     for (int blk = 0; blk < (1 << s); blk++) {
       const uint32_t w = tw_a(tw, (1 << s) + blk, 0), wp = tw_a(tw, (1 << s) + blk, 1);
 #pragma unroll
-      for (int i = 0; i < half; i++) ct_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+      for (int i = 0; i < half; i++)
+        ct_stage_bf<LAZY>(1 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
     }
   }
   // Transpose to phase B through the wave's LDS tile.
 #pragma unroll
-  for (int i = 0; i < 32; i++) lds[66 * i + lane] = r[i];
+  for (int i = 0; i < 32; i++) lds[tile_at(i, lane)] = r[i];
   wave_lds_sync();
   const int L = lane >> 1, b = lane & 1;
 #pragma unroll
-  for (int i = 0; i < 32; i++) r[i] = lds[66 * L + 2 * i + b];
+  for (int i = 0; i < 32; i++) r[i] = lds[tile_at(L, 2 * i + b)];
   wave_lds_sync();
   // Phase B: strides 32 .. 2 in registers.
   int slot = 0;
@@ -163,7 +206,8 @@ __device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *
     for (int blk = 0; blk < (1 << s); blk++) {
       const uint32_t w = tw_b(tw, slot + blk, 0, lane), wp = tw_b(tw, slot + blk, 1, lane);
 #pragma unroll
-      for (int i = 0; i < half; i++) ct_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+      for (int i = 0; i < half; i++)
+        ct_stage_bf<LAZY>(6 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
     }
     slot += (1 << s);
   }
@@ -182,6 +226,11 @@ __device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *
   }
 #pragma unroll
   for (int j = 0; j < 32; j++) r[j] = o[j];
+}
+
+__device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
+                                             uint32_t *__restrict__ lds, uint32_t q, int lane) {
+  ntt_fwd_wave_t<0>(r, tw, lds, q, 0u, lane);
 }
 
 // Inverse negacyclic NTT (without the factor N^-1) of one polynomial.
@@ -214,10 +263,10 @@ __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *
     }
   }
 #pragma unroll
-  for (int i = 0; i < 32; i++) lds[66 * L + 2 * i + b] = n[i];
+  for (int i = 0; i < 32; i++) lds[tile_at(L, 2 * i + b)] = n[i];
   wave_lds_sync();
 #pragma unroll
-  for (int i = 0; i < 32; i++) r[i] = lds[66 * i + lane];
+  for (int i = 0; i < 32; i++) r[i] = lds[tile_at(i, lane)];
   wave_lds_sync();
   // Phase A: strides 64 .. 1024.
 #pragma unroll

@@ -805,19 +805,6 @@ namespace sp {
 
 constexpr int DIG_WORDS = N / 4;  // 512 words per byte-packed digit polynomial
 
-__device__ __forceinline__ void load_digit_bytes(uint32_t (&r)[32], const uint32_t *__restrict__ dig, int lane,
-                                                 bool is_signed, uint32_t offset) {
-#pragma unroll
-  for (int i4 = 0; i4 < 8; i4++) {
-    const uint32_t w = dig[i4 * 64 + lane];
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const uint32_t byte = (w >> (8 * b)) & 0xFFu;
-      r[4 * i4 + b] = is_signed ? (uint32_t)((int32_t)(int8_t)byte + (int32_t)offset) : byte;
-    }
-  }
-}
-
 __device__ __forceinline__ void store_tile_p(uint32_t *__restrict__ tile, const uint32_t (&r)[32], int lane,
                                              uint32_t two_q) {
   uint4 *t4 = reinterpret_cast<uint4 *>(tile);
@@ -827,30 +814,20 @@ __device__ __forceinline__ void store_tile_p(uint32_t *__restrict__ tile, const 
                                    reduce_2q(r[4 * c + 2], two_q), reduce_2q(r[4 * c + 3], two_q));
 }
 
-// Fold level, server form C_neg = G - C:  acc[i] <- Y + from_ntt(C * NTT(g^-1 X - g^-1 Y)),
-// X = acc[num_per + i], Y = acc[i].  32 tasks = 2 residues x 2 polys x 8 digits.
-template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_fold8(uint32_t *__restrict__ accs, size_t set_stride, int n_trials,
-                                                   int num_per, const uint32_t *__restrict__ cpos,
-                                                   size_t key_qstride, const uint32_t *__restrict__ tw_base,
-                                                   ModConst mc0, ModConst mc1) {
-  constexpr int NT = NW * 64;
-  constexpr int TASKS = 32;
-  constexpr int ROUNDS = TASKS / NW;
-  constexpr int PPT = N / NT;
-  static_assert(NW == 4 || NW == 8, "4 or 8 waves");
-  __shared__ uint32_t tiles[NW][LDS_TILE];
-  __shared__ uint32_t dig[2][8][DIG_WORDS];
-  const int tid = threadIdx.x;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  const int i = blockIdx.x;
-  const int set = blockIdx.z * n_trials + blockIdx.y;
-  uint32_t *base = accs + (size_t)set * set_stride;
-  const uint32_t *X = base + (size_t)(num_per + i) * 4 * N;
-  uint32_t *Y = base + (size_t)i * 4 * N;
-  const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
+// Fold and expansion kernels come in two shapes:
+//  * NW = 8 waves (512 threads, 8 transform tiles = 67.6 KB of LDS, at most
+//    128 VGPRs): two blocks per CU, for grids that fill the chip;
+//  * NW = 16 waves (1024 threads, 16 tiles = 135 KB): one block per CU with
+//    half the passes, for grids smaller than the CU count (latency).
+// Every wave transforms one digit polynomial per pass into its own tile; after
+// a barrier every thread multiply-accumulates the pass's tiles at its own
+// positions against the key.
 
-  // Digit differences g^-1(X) - g^-1(Y), signed bytes.
+// Fold digit differences g^-1(X) - g^-1(Y) as signed bytes into dig
+// ([j][k][DIG_WORDS], 8192 words).
+template <int NT>
+__device__ __forceinline__ void fold_digits(const uint32_t *__restrict__ X, const uint32_t *__restrict__ Y,
+                                            uint32_t *__restrict__ dig, int tid) {
   for (int it = tid; it < 2 * DIG_WORDS; it += NT) {
     const int j = it / DIG_WORDS, w = it % DIG_WORDS, l = w & 63, i4 = w >> 6;
     const uint32_t *xh = X + (size_t)j * 2 * N, *xl = xh + N;
@@ -870,54 +847,105 @@ __global__ __launch_bounds__(NW * 64) void k_fold8(uint32_t *__restrict__ accs, 
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) dig[j][k][w] = packed[k];
+    for (int k = 0; k < 8; k++) dig[(j * 8 + k) * DIG_WORDS + w] = packed[k];
   }
-  __syncthreads();
+}
 
-  uint64_t acc[2][2][PPT];
+// One residue of a fold level: 16 / NW passes over the key columns owned by
+// this wave (column c = wave + NW h), one forward transform per wave and
+// pass, then a multiply-accumulate of the pass's NW tiles.
+// Returns the Montgomery-reduced sums of rows 0/1 at this thread's positions.
+template <int NW, int M>
+__device__ __forceinline__ void fold_residue(const uint32_t (&dw)[16 / NW][8], uint32_t (&sum)[2][N / (NW * 64)],
+                                             uint32_t (*tiles)[LDS_TILE], const uint32_t *__restrict__ kp,
+                                             const uint32_t *__restrict__ tw_base, const ModConst &mc, int wave,
+                                             int lane, int tid) {
+  constexpr int PPT = N / (NW * 64);
+  uint64_t acc[2][PPT];
 #pragma unroll
-  for (int m = 0; m < 2; m++)
+  for (int row = 0; row < 2; row++)
 #pragma unroll
-    for (int row = 0; row < 2; row++)
+    for (int e = 0; e < PPT; e++) acc[row][e] = 0;
 #pragma unroll
-      for (int e = 0; e < PPT; e++) acc[m][row][e] = 0;
-
-  for (int rnd = 0; rnd < ROUNDS; rnd++) {
-    {
-      const int task = rnd * NW + wave;
-      const int m = task & 1, j = (task >> 1) & 1, k = task >> 2;
-      const uint32_t q = m ? mc1.q : mc0.q;
-      uint32_t r[32];
-      load_digit_bytes(r, dig[j][k], lane, true, q);
-      ntt_fwd_wave(r, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, lane);
-      store_tile_p(tiles[wave], r, lane, 2 * q);
-    }
+  for (int h = 0; h < 16 / NW; h++) {
+    uint32_t r[32];
+#pragma unroll
+    for (int i4 = 0; i4 < 8; i4++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) r[4 * i4 + b] = (uint32_t)((int32_t)(int8_t)(dw[h][i4] >> (8 * b)) + (int32_t)mc.q);
+    const uint32_t *twm = tw_base + (M * 2) * TW_WORDS;
+    asm volatile("" : "+s"(twm));  // re-read per pass, not held in registers
+    // inputs in (q - 128, q + 128) < 2q: the first 7 stages need no reduction
+    ntt_fwd_wave_t<7>(r, twm, tiles[wave], mc.q, mc.mu1, lane);
+    store_tile_p(tiles[wave], r, lane, 2 * mc.q);
     __syncthreads();
 #pragma unroll
     for (int w2 = 0; w2 < NW; w2++) {
-      const int task = rnd * NW + w2;
-      const int m = w2 & 1, j = (task >> 1) & 1, k = task >> 2;
-      const uint32_t *k0 = kp + ((size_t)(m * 16 + j + 2 * k) * 2) * N + PPT * tid;
+      const int c = w2 + NW * h;
+      const uint32_t *k0 = kp + ((size_t)(M * 16 + c) * 2) * N + PPT * tid;
       const uint32_t *k1 = k0 + N;
       const uint32_t *tl = &tiles[w2][PPT * tid];
 #pragma unroll
       for (int e = 0; e < PPT; e++) {
         const uint32_t d = tl[e];
-        acc[m][0][e] += (uint64_t)d * k0[e];
-        acc[m][1][e] += (uint64_t)d * k1[e];
+        acc[0][e] += (uint64_t)d * k0[e];
+        acc[1][e] += (uint64_t)d * k1[e];
       }
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int row = 0; row < 2; row++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) sum[row][e] = redc64(acc[row][e], mc.q, mc.qinv_neg);
+}
+
+// Fold level, server form C_neg = G - C:  acc[i] <- Y + from_ntt(C * NTT(g^-1 X - g^-1 Y)),
+// X = acc[num_per + i], Y = acc[i].  The 16 digit polynomials (2 polys x 8
+// digits, key column c = j + 2k) are extracted once into tiles 0-3; wave w
+// keeps columns w + NW h packed in registers (4 coefficients per word) and
+// transforms each once per residue.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void k_fold8(uint32_t *__restrict__ accs, size_t set_stride, int n_trials, int num_per,
+             const uint32_t *__restrict__ cpos, size_t key_qstride, const uint32_t *__restrict__ tw_base,
+             ModConst mc0, ModConst mc1) {
+  constexpr int NT = NW * 64;
+  constexpr int PPT = N / NT;
+  constexpr int H = 16 / NW;
+  static_assert(NW == 8 || NW == 16, "8 or 16 waves");
+  __shared__ uint32_t tiles[NW][LDS_TILE];
+  uint32_t *dig = &tiles[0][0];  // digit bytes, 8192 words over tiles 0-3
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int i = blockIdx.x;
+  const int set = blockIdx.z * n_trials + blockIdx.y;
+  uint32_t *base = accs + (size_t)set * set_stride;
+  const uint32_t *X = base + (size_t)(num_per + i) * 4 * N;
+  uint32_t *Y = base + (size_t)i * 4 * N;
+  const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
+
+  fold_digits<NT>(X, Y, dig, tid);
+  __syncthreads();
+  uint32_t dw[H][8];
+#pragma unroll
+  for (int h = 0; h < H; h++) {
+    const int c = wave + NW * h, j = c & 1, k = c >> 1;
+#pragma unroll
+    for (int i4 = 0; i4 < 8; i4++) dw[h][i4] = dig[(j * 8 + k) * DIG_WORDS + i4 * 64 + lane];
+  }
+  __syncthreads();  // the digit bytes are dead from here on
+
+  uint32_t sum[2][2][PPT];  // [m][row][e]
+  fold_residue<NW, 0>(dw, sum[0], tiles, kp, tw_base, mc0, wave, lane, tid);
+  fold_residue<NW, 1>(dw, sum[1], tiles, kp, tw_base, mc1, wave, lane, tid);
   // Sums of polynomial (row, m) into tile 2m + row, then INTT + CRT + Y.
 #pragma unroll
-  for (int m = 0; m < 2; m++) {
-    const ModConst &mc = m ? mc1 : mc0;
+  for (int m = 0; m < 2; m++)
 #pragma unroll
     for (int row = 0; row < 2; row++)
 #pragma unroll
-      for (int e = 0; e < PPT; e++) tiles[2 * m + row][PPT * tid + e] = redc64(acc[m][row][e], mc.q, mc.qinv_neg);
-  }
+      for (int e = 0; e < PPT; e++) tiles[2 * m + row][PPT * tid + e] = sum[m][row][e];
   __syncthreads();
   uint32_t s[32];
   if (wave < 4) {
@@ -956,34 +984,15 @@ __global__ __launch_bounds__(NW * 64) void k_fold8(uint32_t *__restrict__ accs, 
   }
 }
 
-// Expansion step (gpu-pir/expansion.cuh:87-166) for element i of round r,
-// gadget 2^7 x 8: 16 tasks = 2 residues x 8 digits of tau_t(a).
-template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
-                                                     int max_bits, const uint32_t *__restrict__ left,
-                                                     const uint32_t *__restrict__ right, size_t left_qstride,
-                                                     size_t right_qstride, const uint32_t *__restrict__ tw_base,
-                                                     ModConst mc0, ModConst mc1) {
-  constexpr int NT = NW * 64;
-  constexpr int TASKS = 16;
-  constexpr int ROUNDS = TASKS / NW;
-  constexpr int PPT = N / NT;
-  static_assert(NW == 4 || NW == 8, "4 or 8 waves");
-  __shared__ uint32_t tiles[NW][LDS_TILE];
-  __shared__ uint32_t dig[8][DIG_WORDS];
-  __shared__ uint32_t as_e[2][N];
-  const int i = blockIdx.x;
-  if (stop > 0 && r > stop && (i & 1)) return;
-  if (stop > 0 && r == stop && (i & 1) && (i / 2) >= max_bits) return;
-  const int tid = threadIdx.x;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
-  const uint32_t t = (uint32_t)(N >> r) + 1u;
-  const bool use_left = (r != 0) && ((i & 1) == 0);
-  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
-  uint32_t *xm = tiles[2];  // tiles 2 and 3 (2 * LDS_TILE >= 2 N words) are free in phase 1
-
-  // Phase 1: waves 0/1 take v_i.a to CRT halves; the others stage v_i.as_e.
+// Expansion phases 1-2 for element v_i: waves 0/1 take v_i.a out of the NTT
+// domain (one residue each, CRT halves into xm = tiles 2-3); then all threads
+// form the CRT value, apply tau_t (a negated zero becomes Q) and write the 8
+// digit bytes of every coefficient into dig = tiles 4-5.
+template <int NT>
+__device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, uint32_t (*tiles)[LDS_TILE],
+                                              uint32_t t, const uint32_t *__restrict__ tw_base, const ModConst &mc0,
+                                              const ModConst &mc1, int wave, int lane, int tid) {
+  uint32_t *xm = tiles[2];
   if (wave < 2) {
     const int m = wave;
     const ModConst mc = m ? mc1 : mc0;
@@ -993,11 +1002,8 @@ __global__ __launch_bounds__(NW * 64) void k_expand8(uint32_t *__restrict__ v, s
 #pragma unroll
     for (int ii = 0; ii < 32; ii++) xm[m * N + 64 * ii + lane] = canon(shoup_mul(rr[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   }
-  for (int u = tid - 128; u >= 0 && u < N / 2; u += NT - 128)
-    reinterpret_cast<uint4 *>(&as_e[0][0])[u] = reinterpret_cast<const uint4 *>(vi + 2 * N)[u];
   __syncthreads();
-  // Phase 2: CRT, raw automorphism (a negated zero becomes Q), digit bytes.
-  uint8_t *db = reinterpret_cast<uint8_t *>(&dig[0][0]);
+  uint8_t *db = reinterpret_cast<uint8_t *>(tiles[4]);
   for (int z = tid; z < N; z += NT) {
     uint64_t x = (uint64_t)xm[z] * Q1 + (uint64_t)xm[N + z] * Q0;
     x = (x >= QQ) ? x - QQ : x;
@@ -1009,77 +1015,144 @@ __global__ __launch_bounds__(NW * 64) void k_expand8(uint32_t *__restrict__ v, s
     for (int k = 0; k < 8; k++) db[((size_t)k * DIG_WORDS + word) * 4 + byte] = (uint8_t)((y >> (7 * k)) & 127u);
   }
   __syncthreads();
+}
 
-  uint64_t acc[2][2][PPT];
+// Expansion, multiply-accumulate of residue m at this thread's PPT positions
+// (P index p0 + e) over the 8 digit tiles tiles[tile0 + k], then
+// v_i += W g^-1(tau a) + (0, NTT(tau as_e)) for residue m.  tau(as_e) in the
+// NTT domain is a permutation of positions; it is gathered before `bar`
+// (a block barrier), after which the residue is written.
+template <int PPT>
+__device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
+                                              uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,
+                                              uint32_t q, uint32_t qinv) {
+  uint32_t ase[PPT];
 #pragma unroll
-  for (int m = 0; m < 2; m++)
-#pragma unroll
-    for (int row = 0; row < 2; row++)
-#pragma unroll
-      for (int e = 0; e < PPT; e++) acc[m][row][e] = 0;
-
-  for (int rnd = 0; rnd < ROUNDS; rnd++) {
-    {
-      const int task = rnd * NW + wave;
-      const int m = task & 1, k = task >> 1;
-      const uint32_t q = m ? mc1.q : mc0.q;
-      uint32_t rr[32];
-      load_digit_bytes(rr, dig[k], lane, false, 0);
-      ntt_fwd_wave(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, lane);
-      store_tile_p(tiles[wave], rr, lane, 2 * q);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int w2 = 0; w2 < NW; w2++) {
-      const int task = rnd * NW + w2;
-      const int m = w2 & 1, k = task >> 1;
-      const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + PPT * tid;
-      const uint32_t *k1 = k0 + N;
-      const uint32_t *tl = &tiles[w2][PPT * tid];
-#pragma unroll
-      for (int e = 0; e < PPT; e++) {
-        const uint32_t d = tl[e];
-        acc[m][0][e] += (uint64_t)d * k0[e];
-        acc[m][1][e] += (uint64_t)d * k1[e];
-      }
-    }
-    __syncthreads();
+  for (int e = 0; e < PPT; e++) {
+    const uint32_t z = (uint32_t)z_of_p(p0 + e);
+    const uint32_t ev = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
+    ase[e] = vi[(size_t)(2 + m) * N + p_of_z((int)brv11((ev - 1) >> 1))];
   }
-  // v_i += W g^-1(tau a) + (0, NTT(tau as_e)), thread-owned positions.
+  uint64_t acc[2][PPT];
 #pragma unroll
-  for (int m = 0; m < 2; m++) {
-    const ModConst &mc = m ? mc1 : mc0;
+  for (int row = 0; row < 2; row++)
 #pragma unroll
-    for (int row = 0; row < 2; row++) {
-      uint32_t *dst = vi + (size_t)(row * 2 + m) * N + PPT * tid;
+    for (int e = 0; e < PPT; e++) acc[row][e] = 0;
 #pragma unroll
-      for (int e = 0; e < PPT; e++) {
-        uint32_t x = dst[e] + redc64(acc[m][row][e], mc.q, mc.qinv_neg);
-        if (row) {
-          const uint32_t z = (uint32_t)z_of_p(PPT * tid + e);
-          const uint32_t ev = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
-          x += as_e[m][p_of_z((int)brv11((ev - 1) >> 1))];
-        }
-        x = min(x, x - 2 * mc.q);
-        dst[e] = canon(x, mc.q);
-      }
+  for (int k = 0; k < 8; k++) {
+    const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + p0;
+    const uint32_t *k1 = k0 + N;
+    const uint32_t *tl = &tiles[tile0 + k][p0];
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      const uint32_t d = tl[e];
+      acc[0][e] += (uint64_t)d * k0[e];
+      acc[1][e] += (uint64_t)d * k1[e];
+    }
+  }
+  __syncthreads();  // tiles read and row 1 of residue m gathered by every thread
+#pragma unroll
+  for (int row = 0; row < 2; row++) {
+    uint32_t *dst = vi + (size_t)(row * 2 + m) * N + p0;
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      uint32_t x = dst[e] + redc64(acc[row][e], q, qinv);
+      if (row) x += ase[e];
+      x = min(x, x - 2 * q);
+      dst[e] = canon(x, q);
     }
   }
 }
 
+// Expansion step (gpu-pir/expansion.cuh:87-166) for element i of round r,
+// gadget 2^7 x 8.  After phases 1-2, wave w keeps digit k = w mod 8 packed in
+// registers.  NW = 8: two passes (residue 0, then 1), each one transform per
+// wave and an update of that residue by all threads.  NW = 16: waves 8-15
+// take residue 1 in the same pass; threads 0-511 update residue 0 and
+// 512-1023 residue 1.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits,
+               const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
+               size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
+  constexpr int NT = NW * 64;
+  static_assert(NW == 8 || NW == 16, "8 or 16 waves");
+  __shared__ uint32_t tiles[NW][LDS_TILE];
+  const int i = blockIdx.x;
+  if (stop > 0 && r > stop && (i & 1)) return;
+  if (stop > 0 && r == stop && (i & 1) && (i / 2) >= max_bits) return;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  const bool use_left = (r != 0) && ((i & 1) == 0);
+  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
+
+  expand_digits<NT>(vi, tiles, t, tw_base, mc0, mc1, wave, lane, tid);
+  uint32_t dw[8];  // digit wave mod 8: byte b of word i4 is coefficient 64 (4 i4 + b) + lane
+  const uint32_t *dig = tiles[4];
+#pragma unroll
+  for (int i4 = 0; i4 < 8; i4++) dw[i4] = dig[(wave & 7) * DIG_WORDS + i4 * 64 + lane];
+  __syncthreads();  // the digit bytes are dead from here on
+
+#pragma unroll
+  for (int pass = 0; pass < 16 / NW; pass++) {
+    const int m = NW == 16 ? (wave >> 3) : pass;  // residue transformed by this wave
+    {
+      const uint32_t q = m ? mc1.q : mc0.q;
+      uint32_t rr[32];
+#pragma unroll
+      for (int i4 = 0; i4 < 8; i4++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) rr[4 * i4 + b] = (dw[i4] >> (8 * b)) & 0xFFu;
+      const uint32_t *twm = tw_base + (m * 2) * TW_WORDS;
+      asm volatile("" : "+s"(twm));  // re-read per pass, not held in registers
+      // digits < 128: the first 7 stages need no reduction
+      ntt_fwd_wave_t<7>(rr, twm, tiles[wave], q, m ? mc1.mu1 : mc0.mu1, lane);
+      store_tile_p(tiles[wave], rr, lane, 2 * q);
+    }
+    __syncthreads();
+    // residue updated by this thread, and its positions
+    const int mu = NW == 16 ? (tid >> 9) : pass;
+    const int p0 = 4 * (tid & 511);
+    expand_update<4>(vi, W, tiles, NW == 16 ? 8 * mu : 0, mu, p0, t, mu ? mc1.q : mc0.q,
+                     mu ? mc1.qinv_neg : mc0.qinv_neg);
+  }
+}
+
+// Waves per block for a grid of `blocks` working blocks: 16 (one block per CU,
+// half the passes) while the grid does not fill the CUs, else 8 (two per CU).
+static int fold_expand_waves(long blocks, const DevTables &t, int variant) {
+  if (variant == 2) return 8;
+  if (variant == 3) return 16;
+  return blocks <= t.n_cu ? 16 : 8;
+}
+
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
-                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, hipStream_t s) {
+                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, hipStream_t s) {
   dim3 grid(num_per, n_trials, n_queries);
-  k_fold8<8><<<grid, 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0], t.mc[1]);
+  if (fold_expand_waves((long)num_per * n_trials * n_queries, t, variant) == 16)
+    k_fold8<16><<<grid, 1024, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0],
+                                      t.mc[1]);
+  else
+    k_fold8<8><<<grid, 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 
 hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
-                          const DevTables &t, hipStream_t s) {
+                          const DevTables &t, int variant, hipStream_t s) {
   dim3 grid(num_out, n_queries);
-  k_expand8<8><<<grid, 512, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride, t.tw,
-                                    t.mc[0], t.mc[1]);
+  // working blocks per query: odd i past stop_round exit at once
+  long work = num_out;
+  if (stop > 0 && r > stop) work = num_out / 2;
+  else if (stop > 0 && r == stop) work = num_out / 2 + (num_out / 2 < max_bits ? num_out / 2 : max_bits);
+  if (fold_expand_waves(work * n_queries, t, variant) == 16)
+    k_expand8<16><<<grid, 1024, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride,
+                                        t.tw, t.mc[0], t.mc[1]);
+  else
+    k_expand8<8><<<grid, 512, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride,
+                                      t.tw, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@ struct DevTables {
   const uint32_t *tw;    // [m][dir][TW_WORDS]
   const uint32_t *neg1;  // [LOGN][m][2][N]  (w, w') in layout P
   ModConst mc[2];
+  int n_cu;  // compute units of the device (launch-shape choices)
 };
 
 inline const uint32_t *tw_of(const DevTables &t, int m, int inv) { return t.tw + (size_t)(m * 2 + inv) * TW_WORDS; }
@@ -45,12 +46,13 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
 // rows of 4N words at src + r * src_stride -> contiguous dst (response gather for readout).
 hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s);
 
-// Gadget 2^7 x 8 (t_gsw = t_exp = 8) round-structured kernels, 8 waves per block.
+// Gadget 2^7 x 8 (t_gsw = t_exp = 8) fold and expansion kernels with digit-owning waves.
+// variant: 0 = by grid size, 2 = 8-wave blocks, 3 = 16-wave blocks.
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
-                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, hipStream_t s);
+                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, hipStream_t s);
 hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
-                          const DevTables &t, hipStream_t s);
+                          const DevTables &t, int variant, hipStream_t s);
 
 // int8-limb MFMA database multiply (C >= 32, R multiple of 32).
 //   D: database limbs [t][m][p][R/32][4][C][32] (k_db_to_limbs, one-time)

@@ -60,7 +60,9 @@ int sp_ctx_destroy(sp_ctx *ctx);
 int sp_ctx_synchronize(sp_ctx *ctx);
 /* Kernel variant per stage, for tests and the autotuner; 0 = automatic.
  *   SP_STAGE_EXPAND, SP_STAGE_FOLD: 1 = one block of 4 waves per item (any
- *     gadget), 2 = round-structured 8-wave kernel (gadget 2^7 x 8 only)
+ *     gadget); for gadget 2^7 x 8 only: 2 = 8-wave blocks (two per CU),
+ *     3 = 16-wave blocks (one per CU, half the passes); automatic picks 3
+ *     when the launch has no more working blocks than the device has CUs
  *   SP_STAGE_DB_MUL: 1 = VALU u64 multiply-accumulate, 2 = int8-limb MFMA */
 #define SP_STAGE_EXPAND 0
 #define SP_STAGE_FOLD 1

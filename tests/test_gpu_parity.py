@@ -194,7 +194,8 @@ def variant(gpu_ctx, request):
     gpu_ctx.set_variant(stage, 0)
 
 
-@pytest.mark.parametrize("variant", [(1, 1), (1, 2)], indirect=True, ids=["fold-4wave", "fold-rounds8"])
+@pytest.mark.parametrize("variant", [(1, 1), (1, 2), (1, 3)], indirect=True,
+                         ids=["fold-4wave", "fold-8wave", "fold-16wave"])
 @pytest.mark.parametrize("nu2", [1, 2, 4])
 @pytest.mark.parametrize("general", [False, True])
 def test_fold(gpu_ctx, oracle, nu2, general, variant):
@@ -212,7 +213,8 @@ def test_fold(gpu_ctx, oracle, nu2, general, variant):
     np.testing.assert_array_equal(got[0], exp[0])
 
 
-@pytest.mark.parametrize("variant", [(0, 1), (0, 2)], indirect=True, ids=["expand-4wave", "expand-rounds8"])
+@pytest.mark.parametrize("variant", [(0, 1), (0, 2), (0, 3)], indirect=True,
+                         ids=["expand-4wave", "expand-8wave", "expand-16wave"])
 @pytest.mark.parametrize("nu1,nu2", [(6, 2), (3, 1)])
 def test_expansion_random_keys(gpu_ctx, oracle, nu1, nu2, variant):
     from spiral_pir import ops
