@@ -1065,8 +1065,11 @@ __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const u
 }
 
 // Expansion step (gpu-pir/expansion.cuh:87-166) for element i of round r,
-// gadget 2^7 x 8.  After phases 1-2, wave w keeps digit k = w mod 8 packed in
-// registers.  NW = 8: two passes (residue 0, then 1), each one transform per
+// gadget 2^7 x 8.  The grid holds only working steps: below stop_round block b
+// is step i = b; from stop_round on, blocks [0, 2^r) are the even steps 2b and
+// the rest the odd steps 2(b - 2^r) + 1 that stop_round still computes (odd
+// steps past it are skipped, expansion.cuh:92-97).  After phases 1-2, wave w
+// keeps digit k = w mod 8 packed in registers.  NW = 8: two passes (residue 0, then 1), each one transform per
 // wave and an update of that residue by all threads.  NW = 16: waves 8-15
 // take residue 1 in the same pass; threads 0-511 update residue 0 and
 // 512-1023 residue 1.
@@ -1078,9 +1081,8 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
   constexpr int NT = NW * 64;
   static_assert(NW == 8 || NW == 16, "8 or 16 waves");
   __shared__ uint32_t tiles[NW][LDS_TILE];
-  const int i = blockIdx.x;
-  if (stop > 0 && r > stop && (i & 1)) return;
-  if (stop > 0 && r == stop && (i & 1) && (i / 2) >= max_bits) return;
+  const int b = blockIdx.x, half = 1 << r;
+  const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
@@ -1142,12 +1144,12 @@ hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_q
 hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                           const DevTables &t, int variant, hipStream_t s) {
-  dim3 grid(num_out, n_queries);
-  // working blocks per query: odd i past stop_round exit at once
-  long work = num_out;
+  // working steps per query (k_expand8 maps them back to i)
+  int work = num_out;
   if (stop > 0 && r > stop) work = num_out / 2;
   else if (stop > 0 && r == stop) work = num_out / 2 + (num_out / 2 < max_bits ? num_out / 2 : max_bits);
-  if (fold_expand_waves(work * n_queries, t, variant) == 16)
+  dim3 grid(work, n_queries);
+  if (fold_expand_waves((long)work * n_queries, t, variant) == 16)
     k_expand8<16><<<grid, 1024, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride,
                                         t.tw, t.mc[0], t.mc[1]);
   else
