@@ -122,6 +122,140 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
             "db_mul_autotune": tuned, "decoded_all_correct": ok, "slots_decoded_wrong": bad}
 
 
+def _make_db(ctx, Database, Shape, nu1: int, nu2: int, seed: int):
+    shape = Shape(nu1, nu2)
+    records = np.random.default_rng(seed).integers(0, 256, size=(shape.records, shape.n2, 2048), dtype=np.uint8)
+    db = Database(ctx, shape)
+    db.load_bytes(records)
+    return shape, records, db
+
+
+def _client_queries(Client, shape, n: int, n_clients: int, seed: int):
+    """n queries from n_clients seeded clients: (client, (left, right), index, query, v_folding)."""
+    clients = [Client(shape, seed + k) for k in range(n_clients)]
+    keys = [c.keygen() for c in clients]
+    idx = [int(x) for x in np.random.default_rng(seed + 777).integers(0, shape.records, size=n)]
+    out = []
+    for j in range(n):
+        c = j % n_clients
+        q, vf = clients[c].query(idx[j])
+        out.append((clients[c], keys[c], idx[j], q, vf))
+    return out
+
+
+def run_group(torch, mods, ctx, main_db, steps: int, stream) -> tuple[dict, dict]:
+    """Three databases of different shapes, 48/32/16 queries, one graph launch per step."""
+    Batch, Client, Database, Group, Shape, STAGE_EXPAND = (mods[k] for k in
+                                                           ("Batch", "Client", "Database", "Group", "Shape",
+                                                            "STAGE_EXPAND"))
+    dbs = {"9,6": main_db}
+    t0 = time.perf_counter()
+    dbs["10,7"] = _make_db(ctx, Database, Shape, 10, 7, 1002)
+    dbs["8,4"] = _make_db(ctx, Database, Shape, 8, 4, 1003)
+    t_ingest = time.perf_counter() - t0
+    plan = (("9,6", 48, 3100), ("10,7", 32, 3200), ("8,4", 16, 3300))
+    lanes = []
+    for name, n, seed in plan:
+        shape, records, db = dbs[name]
+        qs = _client_queries(Client, shape, n, 4, seed)
+        b = Batch(db, n)
+        for slot, (cl, (left, right), idx, q, vf) in enumerate(qs):
+            b.load(slot, q, left, right, vf)
+        b.run(n, STAGE_EXPAND, stream.cuda_stream)
+        tuned = b.tune(n)
+        lanes.append((name, n, shape, records, b, qs, tuned))
+    g = Group([ln[4] for ln in lanes])
+    ns = [ln[1] for ln in lanes]
+    g.launch(ns, stream.cuda_stream)  # captures the group graph
+    g.launch(ns, stream.cuda_stream)
+    g.synchronize()
+    ok = True
+    for name, n, shape, records, b, qs, tuned in lanes:
+        resp = b.responses(n)
+        ok &= all(np.array_equal(cl.decode(resp[j]), records[idx]) for j, (cl, kk, idx, q, vf) in enumerate(qs))
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        g.launch(ns, stream.cuda_stream)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    total = sum(ns)
+    out = {"workload": "96 queries split 48/32/16 over resident (9,6)/(10,7)/(8,4) databases "
+                       "(2 GiB / 8 GiB / 128 MiB), one hipGraph launch with one branch per database",
+           "ms_per_launch": ms, "queries_per_s": total * 1000.0 / ms, "steps": steps,
+           "db_mul_autotune": {ln[0]: ln[6] for ln in lanes}, "decoded_all_correct": bool(ok),
+           "extra_ingest_s": t_ingest}
+    g.close()
+    for ln in lanes:
+        ln[4].close()
+    return out, dbs
+
+
+def run_serving(torch, mods, dbs, rates, seconds: float) -> dict:
+    """Poisson arrivals, 70/30 over the resident (9,6)/(10,7) databases, continuous batching."""
+    from spiral_pir.serving import Server, drive, latency_stats, pinned_u32, poisson_times
+
+    Client, Query = mods["Client"], mods["Query"]
+    lanes = {"9,6": dbs["9,6"], "10,7": dbs["10,7"]}
+    srv = Server({k: v[2] for k, v in lanes.items()}, max_batch=128, buffers=2, weights={"9,6": 0.5, "10,7": 0.5})
+    pools = {}
+    t0 = time.perf_counter()
+    for li, (name, (shape, records, db)) in enumerate(lanes.items()):
+        qs = _client_queries(Client, shape, 16, 4, 4000 + 100 * li)
+        pool = []
+        for j, (cl, (left, right), idx, q, vf) in enumerate(qs):
+            cid = (name, j % 4)
+            if cid not in srv.lanes[name].keys:
+                srv.register(name, cid, left, right)
+            one = Query(db)
+            one.load(q, left, right, vf)
+            one.run()
+            ref = one.read()
+            one.close()
+            pq, pv = pinned_u32(q.size), pinned_u32(vf.size)
+            pq[:] = q.reshape(-1)
+            pv[:] = vf.reshape(-1)
+            pool.append((cid, pq, pv, ref, bool(np.array_equal(cl.decode(ref), records[idx]))))
+        pools[name] = pool
+        srv.warmup(name, pool[0][0], pool[0][1], pool[0][2])
+    t_setup = time.perf_counter() - t0
+    srv.start()
+    rng = np.random.default_rng(3000)
+    results = []
+    exact = all(p[4] for pool in pools.values() for p in pool)
+    for rate in rates:
+        times = poisson_times(rate, seconds, rng)
+        arrivals = []
+        for t in times:
+            name = "9,6" if rng.random() < 0.7 else "10,7"
+            k = int(rng.integers(0, len(pools[name])))
+            cid, pq, pv, ref, _ = pools[name][k]
+            arrivals.append((float(t), name, cid, pq, pv, (name, k)))
+        reqs = drive(srv, arrivals)
+        for r in reqs:
+            r.wait(timeout=120)
+        same = all(np.array_equal(r.response, pools[r.tag[0]][r.tag[1]][3]) for r in reqs)
+        exact &= same
+        st = latency_stats(reqs)
+        by_lane = {}
+        for name in lanes:
+            sub = [r for r in reqs if r.lane == name]
+            if sub:
+                ls = latency_stats(sub)
+                by_lane[name] = {"completed": ls["completed"], "p50_ms": ls["p50_ms"], "p99_ms": ls["p99_ms"]}
+        results.append({"offered_qps": rate, **st, "by_database": by_lane, "responses_equal_alone": bool(same)})
+    srv.stop()
+    out = {"workload": "Poisson arrivals, 70/30 over resident (9,6)/(10,7) databases, resident client keys, "
+                       "continuous batching (<=128 per shape, 2 buffers per shape, graph replay per size class)",
+           "seconds_per_rate": seconds, "rates": results, "admission": srv.admission,
+           "batches_launched": {k: ln.launched_batches for k, ln in srv.lanes.items()},
+           "setup_s": t_setup, "all_responses_exact_and_decoded": bool(exact)}
+    srv.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +269,11 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=64, help="batched leg: queries per graph launch (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-eager", action="store_true", help="batched leg without hipGraph replay")
+    ap.add_argument("--no-group", action="store_true", help="skip the three-database group launch")
+    ap.add_argument("--group-steps", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-rates", default="200,1000,2000", help="offered queries/s, comma separated")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -147,7 +286,7 @@ def main() -> int:
     import torch
     from spiral_pir import Shape, version
     from spiral_pir.client import Client
-    from spiral_pir.server import (Batch, Context, Database, Query, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD,
+    from spiral_pir.server import (Batch, Context, Database, Group, Query, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD,
                                    STAGES_ALL)
 
     torch.cuda.init()
@@ -185,20 +324,24 @@ def main() -> int:
     # Timed region: K whole answers, events around the database multiply.
     K = args.steps
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
+    step_ev[0].record(stream)
     for k in range(K):
         query.run(STAGE_EXPAND, sptr)
         ev[k][0].record(stream)
         query.run(STAGE_DB_MUL, sptr)
         ev[k][1].record(stream)
         query.run(STAGE_FOLD, sptr)
+        step_ev[k + 1].record(stream)
     stop.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     ms_per_step = total_ms / K
     dbmul_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    step_ms = np.array([step_ev[k].elapsed_time(step_ev[k + 1]) for k in range(K)])
 
     # Stage breakdown (separate pass, events between every stage).
     reps = max(3, min(K, 10))
@@ -223,6 +366,21 @@ def main() -> int:
         batched = run_batched(torch, Batch, Client, db, shape, records, args.batch, args.batch_steps, stream,
                               eager=args.batch_eager)
         correct = correct and batched["decoded_all_correct"]
+    mods = {"Batch": Batch, "Client": Client, "Database": Database, "Group": Group, "Shape": Shape, "Query": Query,
+            "STAGE_EXPAND": STAGE_EXPAND}
+    group = serving = None
+    if not args.no_group or not args.no_serve:
+        group, dbs = run_group(torch, mods, ctx, (shape, records, db), args.group_steps, stream)
+        correct = correct and group["decoded_all_correct"]
+        if args.no_group:
+            group = None
+        if not args.no_serve:
+            rates = [float(x) for x in args.serve_rates.split(",") if x]
+            serving = run_serving(torch, mods, dbs, rates, args.serve_seconds)
+            correct = correct and serving["all_responses_exact_and_decoded"]
+        for name, (_, _, d) in dbs.items():
+            if d is not db:
+                d.close()
 
     alg_bytes = db_mul_algorithmic_bytes(shape)
     achieved = alg_bytes / (dbmul_ms * 1e-3) / 1e9
@@ -250,6 +408,7 @@ def main() -> int:
         "db_scan_gbps": db_scan_gbps,
         "db_mul_ms": dbmul_ms,
         "stage_ms": stage_ms,
+        "single_query_latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "decoded_record_correct": correct,
         "single_query_decoded_correct": single_correct,
         "roofline": {
@@ -263,6 +422,8 @@ def main() -> int:
             "traffic_unit": "bytes per launch",
         },
         "batched": None if batched is None else {k: v for k, v in batched.items() if k != "decoded_all_correct"},
+        "group": group,
+        "serving": serving,
         "ingest_s": t_ingest,
         "library": version(),
     }

@@ -529,6 +529,7 @@ struct sp_batch {
   hipEvent_t done = nullptr;      // recorded after every run
   std::vector<std::pair<int, hipGraphExec_t>> graphs;
   std::vector<hipGraph_t> graph_defs;
+  uint32_t *bulk = nullptr;       // bulk-load staging: [cap][4N] queries then [cap][f_q] v_folding
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
@@ -545,7 +546,8 @@ static void batch_free(sp_batch *b) {
   if (!b) return;
   for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
   for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
-  uint32_t *bufs[] = {b->query_ct, b->v, b->left, b->right, b->fkeys, b->fkeys_neg, b->acc, b->stage, b->mfma_out};
+  uint32_t *bufs[] = {b->query_ct, b->v,     b->left,     b->right, b->fkeys,
+                      b->fkeys_neg, b->acc, b->stage, b->mfma_out, b->bulk};
   for (uint32_t *p : bufs)
     if (p) (void)hipFree(p);
   if (b->mfma_a) (void)hipFree(b->mfma_a);
@@ -651,6 +653,104 @@ static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const u
 extern "C" int sp_batch_load(sp_batch *b, int slot, const uint32_t *host_query, const uint32_t *host_left,
                              const uint32_t *host_right, const uint32_t *host_v_folding) {
   return batch_load(b, slot, host_query, host_left, host_right, host_v_folding, nullptr);
+}
+
+// ---- device-resident client keys ------------------------------------------------
+struct sp_keys {
+  sp_db *db = nullptr;
+  int t_gsw = 0, t_exp = 0;
+  size_t l_words = 0, r_words = 0;
+  uint32_t *left = nullptr, *right = nullptr;  // layout P, 2^32-scaled: as a batch slot's keys
+};
+
+extern "C" int sp_keys_create(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_left, const uint32_t *host_right,
+                              sp_keys **out) {
+  if (!db || !host_left || !host_right || !out) return fail(SP_ERR_ARG, "sp_keys_create: NULL");
+  *out = nullptr;
+  if (t_exp < 2 || t_exp > 56 || (t_exp & 1) || t_gsw < 2 || t_gsw > 28)
+    return fail(SP_ERR_ARG, "sp_keys_create: t_gsw %d / t_exp %d unsupported", t_gsw, t_exp);
+  int g = 0, stop = 0;
+  int rc = sp_expansion_depth(db->nu1, db->nu2, t_gsw, &g, &stop);
+  if (rc) return rc;
+  sp_ctx *ctx = db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t kl = (size_t)2 * t_exp * 2 * N;
+  sp_keys *k = new sp_keys();
+  k->db = db; k->t_gsw = t_gsw; k->t_exp = t_exp;
+  k->l_words = (size_t)g * kl;
+  k->r_words = (size_t)(stop + 1) * kl;
+  DevBuf stage;
+  hipError_t e = hipMalloc(&k->left, k->l_words * 4);
+  if (e == hipSuccess) e = hipMalloc(&k->right, k->r_words * 4);
+  if (e == hipSuccess) e = stage.alloc(std::max(k->l_words, k->r_words) * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(stage.p, host_left, k->l_words * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_nat_to_p(stage.u32(), k->left, k->l_words / N, 2 * t_exp, 1, ctx->tables, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(stage.p, host_right, k->r_words * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_nat_to_p(stage.u32(), k->right, k->r_words / N, 2 * t_exp, 1, ctx->tables, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    if (k->left) (void)hipFree(k->left);
+    if (k->right) (void)hipFree(k->right);
+    delete k;
+    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_keys_create: %s", hipGetErrorString(e));
+  }
+  *out = k;
+  return SP_OK;
+}
+
+extern "C" int sp_keys_destroy(sp_keys *k) {
+  if (!k) return SP_OK;
+  (void)hipSetDevice(k->db->ctx->device);
+  (void)hipDeviceSynchronize();
+  if (k->left) (void)hipFree(k->left);
+  if (k->right) (void)hipFree(k->right);
+  delete k;
+  return SP_OK;
+}
+
+// Slots [first, first + n): query i (host, [4N], NTT natural) from
+// host_queries[i], its resident keys keys[i], its v_folding (host,
+// [nu2][2][2 t_gsw][2][N]) from host_v_folding[i].  The transfers go to a
+// device staging area (asynchronous DMA when the host buffers are pinned),
+// one layout conversion per array covers all n queries, keys are copied
+// device to device.
+extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                                  sp_keys *const *keys, const uint32_t *const *host_v_folding) {
+  if (!b || !host_queries || !keys || !host_v_folding) return fail(SP_ERR_ARG, "sp_batch_load_many: NULL");
+  if (n < 1 || first < 0 || first + n > b->cap)
+    return fail(SP_ERR_ARG, "sp_batch_load_many: slots [%d, %d) of %d", first, first + n, b->cap);
+  for (int i = 0; i < n; i++) {
+    const sp_keys *k = keys[i];
+    if (!host_queries[i] || !host_v_folding[i]) return fail(SP_ERR_ARG, "sp_batch_load_many: query %d NULL", i);
+    if (!k || k->db != b->db || k->t_exp != b->t_exp || k->t_gsw != b->t_gsw || k->l_words != b->l_q ||
+        k->r_words != b->r_q)
+      return fail(SP_ERR_ARG, "sp_batch_load_many: keys %d do not match this batch's database and gadget", i);
+  }
+  sp_ctx *ctx = b->db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));  // the previous run must be done with the slots
+  const size_t qw = (size_t)4 * N;
+  if (!b->bulk) {
+    hipError_t e = hipMalloc(&b->bulk, (size_t)b->cap * (qw + b->f_q) * 4);
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "bulk staging: %s", hipGetErrorString(e));
+  }
+  uint32_t *sq = b->bulk, *sf = b->bulk + (size_t)b->cap * qw;
+  const size_t f0 = (size_t)first;
+  for (int i = 0; i < n; i++) {
+    HIP_TRY(hipMemcpyAsync(sq + (size_t)i * qw, host_queries[i], qw * 4, hipMemcpyHostToDevice, b->io));
+    HIP_TRY(hipMemcpyAsync(sf + (size_t)i * b->f_q, host_v_folding[i], b->f_q * 4, hipMemcpyHostToDevice, b->io));
+  }
+  HIP_TRY(launch_nat_to_p(sq, b->query_ct + f0 * qw, (size_t)n * 4, 1, 0, ctx->tables, b->io));
+  HIP_TRY(launch_nat_to_p(sf, b->fkeys + f0 * b->f_q, (size_t)n * b->f_q / N, 4 * b->t_gsw, 1, ctx->tables, b->io));
+  for (int i = 0; i < n; i++) {
+    HIP_TRY(hipMemcpyAsync(b->left + (f0 + i) * b->l_q, keys[i]->left, b->l_q * 4, hipMemcpyDeviceToDevice, b->io));
+    HIP_TRY(hipMemcpyAsync(b->right + (f0 + i) * b->r_q, keys[i]->right, b->r_q * 4, hipMemcpyDeviceToDevice, b->io));
+  }
+  HIP_TRY(hipStreamSynchronize(b->io));
+  b->fold_mode = 1;
+  return SP_OK;
 }
 
 static int mfma_workspace(sp_batch *b) {
@@ -775,6 +875,129 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   return SP_OK;
 }
 
+// ---- several batches (each on its own database) in one launch --------------------
+struct sp_group {
+  std::vector<sp_batch *> b;
+  hipStream_t fork = nullptr;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
+  std::vector<std::pair<std::vector<int>, hipGraphExec_t>> graphs;  // ns -> executable graph
+  std::vector<hipGraph_t> graph_defs;
+  hipEvent_t done = nullptr;
+};
+
+static void group_free(sp_group *g) {
+  for (auto &ge : g->graphs) (void)hipGraphExecDestroy(ge.second);
+  for (auto &gd : g->graph_defs) (void)hipGraphDestroy(gd);
+  for (auto e : g->ev_join)
+    if (e) (void)hipEventDestroy(e);
+  if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+  if (g->done) (void)hipEventDestroy(g->done);
+  if (g->fork) (void)hipStreamDestroy(g->fork);
+  delete g;
+}
+
+extern "C" int sp_group_create(sp_batch *const *batches, int count, sp_group **out) {
+  if (!batches || !out || count < 1 || count > 64) return fail(SP_ERR_ARG, "sp_group_create: bad argument");
+  *out = nullptr;
+  const int dev = batches[0] ? batches[0]->db->ctx->device : -1;
+  for (int i = 0; i < count; i++) {
+    if (!batches[i] || batches[i]->db->ctx->device != dev) return fail(SP_ERR_ARG, "sp_group_create: batch %d", i);
+    for (int j = 0; j < i; j++)
+      if (batches[j] == batches[i]) return fail(SP_ERR_ARG, "sp_group_create: batch %d repeated", i);
+  }
+  HIP_TRY(hipSetDevice(dev));
+  sp_group *g = new sp_group();
+  g->b.assign(batches, batches + count);
+  g->ev_join.assign(count, nullptr);
+  hipError_t e = hipStreamCreateWithFlags(&g->fork, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->done, hipEventDisableTiming);
+  for (int i = 0; i < count && e == hipSuccess; i++) e = hipEventCreateWithFlags(&g->ev_join[i], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    group_free(g);
+    return fail(SP_ERR_HIP, "sp_group_create: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return SP_OK;
+}
+
+extern "C" int sp_group_destroy(sp_group *g) {
+  if (!g) return SP_OK;
+  (void)hipSetDevice(g->b[0]->db->ctx->device);
+  (void)hipDeviceSynchronize();
+  group_free(g);
+  return SP_OK;
+}
+
+// All stages of slots [0, ns[i]) of every batch i with ns[i] > 0, as one graph
+// whose branches (one per batch) run concurrently; captured on first use for
+// this ns and replayed on `stream`.
+extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
+  if (!g || !ns) return fail(SP_ERR_ARG, "sp_group_launch: NULL");
+  const int count = (int)g->b.size();
+  std::vector<int> key(ns, ns + count);
+  int active = 0;
+  for (int i = 0; i < count; i++) {
+    if (key[i] < 0 || key[i] > g->b[i]->cap) return fail(SP_ERR_ARG, "sp_group_launch: ns[%d] = %d", i, key[i]);
+    if (key[i] > 0 && g->b[i]->fold_mode < 0) return fail(SP_ERR_STATE, "sp_group_launch: batch %d not loaded", i);
+    active += key[i] > 0;
+  }
+  if (!active) return SP_OK;
+  HIP_TRY(hipSetDevice(g->b[0]->db->ctx->device));
+  hipGraphExec_t exec = nullptr;
+  for (auto &ge : g->graphs)
+    if (ge.first == key) exec = ge.second;
+  if (!exec) {
+    for (int i = 0; i < count; i++) {  // one-time preparations stay out of the graph
+      sp_batch *b = g->b[i];
+      if (key[i] > 0 && dbmul_variant(b, key[i]) == 2) {
+        int rc = db_limbs(b->db, b->cap_stream);
+        if (!rc) rc = mfma_workspace(b);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(b->cap_stream));
+      }
+    }
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(g->fork, hipStreamCaptureModeThreadLocal));
+    int rc = SP_OK;
+    hipError_t e = hipEventRecord(g->ev_fork, g->fork);
+    for (int i = 0; i < count && !rc && e == hipSuccess; i++) {
+      if (key[i] == 0) continue;
+      sp_batch *b = g->b[i];
+      e = hipStreamWaitEvent(b->cap_stream, g->ev_fork, 0);
+      if (e == hipSuccess) rc = batch_enqueue(b, key[i], 7, b->cap_stream);
+      if (!rc && e == hipSuccess) e = hipEventRecord(g->ev_join[i], b->cap_stream);
+      if (!rc && e == hipSuccess) e = hipStreamWaitEvent(g->fork, g->ev_join[i], 0);
+    }
+    hipError_t e2 = hipStreamEndCapture(g->fork, &graph);
+    if (rc || e != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      return fail(SP_ERR_HIP, "sp_group_launch capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(SP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    g->graphs.push_back({key, exec});
+    g->graph_defs.push_back(graph);
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIP_TRY(hipGraphLaunch(exec, s));
+  HIP_TRY(hipEventRecord(g->done, s));
+  for (int i = 0; i < count; i++)
+    if (key[i] > 0) HIP_TRY(hipEventRecord(g->b[i]->done, s));
+  return SP_OK;
+}
+
+extern "C" int sp_group_synchronize(sp_group *g) {
+  if (!g) return fail(SP_ERR_ARG, "sp_group_synchronize: NULL");
+  HIP_TRY(hipEventSynchronize(g->done));
+  return SP_OK;
+}
+
 // Times the database multiply of n queries with each variant (VALU u64 and,
 // where the shape allows, int8-limb MFMA) on the batch's io stream, keeps
 // the faster one for n.  Needs the expansion (stage 1) to have run.
@@ -881,6 +1104,15 @@ extern "C" int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out) {
     HIP_TRY(hipStreamSynchronize(b->io));
   }
   return SP_OK;
+}
+
+extern "C" int sp_batch_poll(sp_batch *b, int *done) {
+  if (!b || !done) return fail(SP_ERR_ARG, "sp_batch_poll: NULL");
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  const hipError_t e = hipEventQuery(b->done);
+  if (e == hipSuccess) { *done = 1; return SP_OK; }
+  if (e == hipErrorNotReady) { *done = 0; return SP_OK; }
+  return fail(SP_ERR_HIP, "sp_batch_poll: %s", hipGetErrorString(e));
 }
 
 extern "C" int sp_batch_synchronize(sp_batch *b) {

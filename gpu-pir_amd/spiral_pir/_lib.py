@@ -67,8 +67,16 @@ SIGNATURES = {
     "sp_batch_read": [P, I, I, P],
     "sp_batch_read_responses": [P, I, P],
     "sp_batch_synchronize": [P],
+    "sp_batch_poll": [P, PI],
     "sp_batch_tune": [P, I, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), PI],
     "sp_batch_destroy": [P],
+    "sp_keys_create": [P, I, I, P, P, PP],
+    "sp_keys_destroy": [P],
+    "sp_batch_load_many": [P, I, I, P, P, P],
+    "sp_group_create": [P, I, PP],
+    "sp_group_launch": [P, PI, P],
+    "sp_group_synchronize": [P],
+    "sp_group_destroy": [P],
     "sp_client_create": [I, I, I, I, I, U64, PP],
     "sp_client_destroy": [P],
     "sp_client_sizes": [P, PSZ, PSZ, PSZ, PSZ, PSZ],

@@ -176,12 +176,37 @@ class Batch:
     def synchronize(self) -> None:
         _lib.call("sp_batch_synchronize", self._h)
 
+    def poll(self) -> bool:
+        """True when the last run has finished (does not wait)."""
+        d = ctypes.c_int()
+        _lib.call("sp_batch_poll", self._h, ctypes.byref(d))
+        return bool(d.value)
+
     def tune(self, n: int) -> dict:
         """Autotune the database multiply for n queries (after one expansion has run)."""
         a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
         _lib.call("sp_batch_tune", self._h, n, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return {"valu_ms": a.value, "mfma_ms": b.value if b.value >= 0 else None,
                 "chosen": {1: "valu", 2: "mfma"}[c.value]}
+
+    def load_many(self, first: int, queries, keys: "list[Keys]", v_folding) -> None:
+        """Slots [first, first + len(keys)): queries[i] ([2][2][2048] u32), resident keys[i],
+        v_folding[i] (u32, fold_words).  Arrays in pinned memory make the transfers
+        asynchronous DMA; they are not retained after the call."""
+        n = len(keys)
+        s = self.db.shape
+        qs = [np.ascontiguousarray(q, dtype=np.uint32) for q in queries]
+        vfs = [np.ascontiguousarray(v, dtype=np.uint32) for v in v_folding]
+        if len(qs) != n or len(vfs) != n:
+            raise ValueError("load_many: one query, key set and v_folding per slot")
+        for q, v in zip(qs, vfs):
+            if q.size != s.query_words or v.size != s.fold_words:
+                raise ValueError("load_many: query or v_folding of the wrong size")
+        P = ctypes.c_void_p
+        qa = (P * n)(*[q.ctypes.data for q in qs])
+        va = (P * n)(*[v.ctypes.data for v in vfs])
+        ka = (P * n)(*[k.handle.value for k in keys])
+        _lib.call("sp_batch_load_many", self._h, first, n, qa, ka, va)
 
     def read(self, slot: int, what: int = READ_RESPONSE) -> np.ndarray:
         s = self.db.shape
@@ -205,6 +230,52 @@ class Batch:
     @property
     def handle(self):
         return self._h
+
+
+class Keys:
+    """One client's expansion keys, resident on the device for one database shape."""
+
+    def __init__(self, db: Database, left: np.ndarray, right: np.ndarray):
+        s = db.shape
+        self.db = db
+        h = ctypes.c_void_p()
+        lft = np.ascontiguousarray(left, dtype=np.uint32)
+        rgt = np.ascontiguousarray(right, dtype=np.uint32)
+        _lib.call("sp_keys_create", db.handle, s.t_gsw, s.t_exp, _ptr(lft), _ptr(rgt), ctypes.byref(h))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("sp_keys_destroy", self._h)
+            self._h = None
+
+
+class Group:
+    """Several batches (each against its own database) answered by one graph launch
+    whose branches run concurrently."""
+
+    def __init__(self, batches: "list[Batch]"):
+        self.batches = list(batches)
+        h = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(self.batches))(*[b.handle.value for b in self.batches])
+        _lib.call("sp_group_create", arr, len(self.batches), ctypes.byref(h))
+        self._h = h
+
+    def launch(self, ns: "list[int]", stream=None) -> None:
+        arr = (ctypes.c_int * len(ns))(*ns)
+        _lib.call("sp_group_launch", self._h, arr, ctypes.c_void_p(stream or 0))
+
+    def synchronize(self) -> None:
+        _lib.call("sp_group_synchronize", self._h)
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("sp_group_destroy", self._h)
+            self._h = None
 
 
 def answer(db: Database, query, left, right, v_folding) -> np.ndarray:

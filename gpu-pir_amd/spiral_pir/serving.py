@@ -1,0 +1,327 @@
+"""Continuous-batching PIR server over databases resident in HBM.
+
+Requests (a query ciphertext and its GSW fold vectors, from a client whose
+expansion keys are resident on the device) arrive at any time.  A scheduler
+thread keeps, per database ("lane"), a few batch buffers of `capacity` slots,
+each with its own stream.  Whenever a buffer is free and requests are waiting,
+it takes up to `capacity` of them, loads them with one bulk call, and replays
+the captured stage chain of the smallest size class that holds them.  Requests
+complete when their batch finishes.  Buffers of one lane and of different lanes
+run on the device at the same time, so queries that arrive while others are
+being answered are admitted as soon as a buffer frees up, and their work
+overlaps on the GPU.
+
+Admission is by free device memory: the slot capacity of each lane is sized at
+start-up from the free HBM (hipMemGetInfo) and the device bytes one slot needs;
+requests beyond the free slots wait in the lane's queue.
+
+The reference has no serving layer (it answers one query from test files);
+this module is the host-side loop over the library's batch, keys and graph
+entry points.
+"""
+from __future__ import annotations
+
+import collections
+import threading
+import time
+
+import numpy as np
+
+from .params import POLY_LEN, Shape
+from .server import STAGE_EXPAND, Batch, Database, Keys
+
+SIZE_CLASSES = (1, 2, 4, 8, 16, 32, 64, 128)
+
+
+def slot_device_bytes(shape: Shape) -> int:
+    """Device bytes one batch slot holds: query, expanded v, copy of the client's
+    keys, fold vectors, accumulators, staging, and its share of the int8-limb
+    multiply workspaces (selector limbs and products of its two rows)."""
+    n = POLY_LEN
+    v = (1 << shape.g) * 4 * n * 4
+    keys = (shape.left_words + shape.right_words) * 4
+    fold = shape.fold_words * 4
+    acc = shape.n2 * shape.cols * 4 * n * 4
+    query = 4 * n * 4
+    staging = (4 * n + shape.fold_words) * 4
+    mfma = 2 * n * shape.rows * 4 * 2 + 2 * n * shape.n2 * 2 * shape.cols * 4
+    return v + keys + fold + acc + query + staging + mfma
+
+
+def size_class(n: int, capacity: int) -> int:
+    for c in SIZE_CLASSES:
+        if c >= n:
+            return min(c, capacity)
+    return capacity
+
+
+class Request:
+    """One query in flight; `wait()` returns its n2 response ciphertexts."""
+
+    __slots__ = ("lane", "client", "query", "v_folding", "tag", "t_submit", "t_launch", "t_done", "response",
+                 "error", "_ev")
+
+    def __init__(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray, tag=None):
+        self.lane = lane
+        self.client = client
+        self.query = query
+        self.v_folding = v_folding
+        self.tag = tag
+        self.t_submit = time.perf_counter()
+        self.t_launch = None
+        self.t_done = None
+        self.response = None
+        self.error = None
+        self._ev = threading.Event()
+
+    def wait(self, timeout: float | None = None) -> np.ndarray:
+        if not self._ev.wait(timeout):
+            raise TimeoutError("request not answered in time")
+        if self.error is not None:
+            raise self.error
+        return self.response
+
+    @property
+    def done(self) -> bool:
+        return self._ev.is_set()
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_submit
+
+
+class _Lane:
+    def __init__(self, name: str, db: Database, capacity: int, buffers: int):
+        import torch
+
+        self.name = name
+        self.db = db
+        self.capacity = capacity
+        self.batches = [Batch(db, capacity) for _ in range(buffers)]
+        self.streams = [torch.cuda.Stream() for _ in range(buffers)]
+        self.free = collections.deque(range(buffers))
+        self.inflight: dict[int, list[Request]] = {}
+        self.pending: collections.deque[Request] = collections.deque()
+        self.keys: dict = {}
+        self.ready: set = set()  # (buffer, size class) autotuned
+        self.launched_batches = 0
+        self.launched_queries = 0
+
+    def close(self) -> None:
+        for b in self.batches:
+            b.close()
+        for k in self.keys.values():
+            k.close()
+
+
+class Server:
+    """Continuous batching over resident databases `dbs` (name -> Database).
+
+    capacity per lane = min(max_batch, share of `hbm_fraction` of the free HBM
+    / (buffers x slot_device_bytes)); `weights` splits the memory between lanes
+    (default: equally)."""
+
+    def __init__(self, dbs: dict[str, Database], max_batch: int = 128, buffers: int = 2,
+                 hbm_fraction: float = 0.8, weights: dict[str, float] | None = None,
+                 poll_interval: float = 1e-4):
+        import torch
+
+        free, total = torch.cuda.mem_get_info()
+        weights = weights or {name: 1.0 for name in dbs}
+        wsum = sum(weights[name] for name in dbs)
+        self.admission = {"free_bytes_at_start": int(free), "total_bytes": int(total), "hbm_fraction": hbm_fraction,
+                          "lanes": {}}
+        self.lanes: dict[str, _Lane] = {}
+        for name, db in dbs.items():
+            per_slot = slot_device_bytes(db.shape)
+            share = free * hbm_fraction * weights[name] / wsum
+            cap = min(max_batch, int(share // (buffers * per_slot)))
+            if cap < 1:
+                raise MemoryError(f"lane {name}: {share / 2**30:.1f} GiB of free HBM does not hold one slot "
+                                  f"({per_slot / 2**20:.0f} MiB) per buffer")
+            self.lanes[name] = _Lane(name, db, cap, buffers)
+            self.admission["lanes"][name] = {"capacity": cap, "buffers": buffers, "slot_bytes": per_slot}
+        self.poll_interval = poll_interval
+        self._queue: collections.deque[Request] = collections.deque()
+        self._lock = threading.Lock()
+        self._wake = threading.Event()
+        self._stop = False
+        self._thread: threading.Thread | None = None
+        self._error: BaseException | None = None
+
+    # ---- set-up ---------------------------------------------------------------
+    def register(self, lane: str, client, left: np.ndarray, right: np.ndarray) -> None:
+        """Make a client's expansion keys resident for `lane`."""
+        ln = self.lanes[lane]
+        ln.keys[client] = Keys(ln.db, left, right)
+
+    def warmup(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray) -> None:
+        """Fill every buffer of `lane` with copies of one query and autotune and
+        capture every size class, so that serving starts without first-use costs."""
+        ln = self.lanes[lane]
+        keys = ln.keys[client]
+        for buf, batch in enumerate(ln.batches):
+            stream = ln.streams[buf].cuda_stream
+            batch.load_many(0, [query] * ln.capacity, [keys] * ln.capacity, [v_folding] * ln.capacity)
+            for c in sorted({size_class(n, ln.capacity) for n in range(1, ln.capacity + 1)}):
+                self._prepare(ln, buf, c)
+                batch.graph_launch(c, stream)
+                batch.synchronize()
+
+    # ---- requests -------------------------------------------------------------
+    def submit(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray, tag=None) -> Request:
+        if lane not in self.lanes:
+            raise KeyError(f"no database lane {lane!r}")
+        if client not in self.lanes[lane].keys:
+            raise KeyError(f"client {client!r} has no resident keys for lane {lane!r}")
+        r = Request(lane, client, query, v_folding, tag)
+        with self._lock:
+            self._queue.append(r)
+        self._wake.set()
+        return r
+
+    # ---- the loop -------------------------------------------------------------
+    def start(self) -> None:
+        if self._thread is not None:
+            raise RuntimeError("server already running")
+        self._stop = False
+        self._thread = threading.Thread(target=self._run, name="pir-serving", daemon=True)
+        self._thread.start()
+
+    def stop(self) -> None:
+        """Answer everything submitted so far, then stop the loop."""
+        self._stop = True
+        self._wake.set()
+        if self._thread is not None:
+            self._thread.join()
+            self._thread = None
+        if self._error is not None:
+            raise self._error
+
+    def close(self) -> None:
+        if self._thread is not None:
+            self.stop()
+        for ln in self.lanes.values():
+            ln.close()
+
+    def _run(self) -> None:
+        try:
+            while True:
+                with self._lock:
+                    arrivals = list(self._queue)
+                    self._queue.clear()
+                for r in arrivals:
+                    self.lanes[r.lane].pending.append(r)
+                progressed = bool(arrivals)
+                busy = False
+                for ln in self.lanes.values():
+                    progressed |= self._complete(ln)
+                    while ln.free and ln.pending:
+                        self._launch(ln)
+                        progressed = True
+                    busy |= bool(ln.inflight or ln.pending)
+                if self._stop and not busy:
+                    with self._lock:
+                        if not self._queue:
+                            return
+                if not progressed:
+                    self._wake.wait(self.poll_interval)
+                    self._wake.clear()
+        except BaseException as e:  # fail every waiting request, surface the error on stop()
+            self._error = e
+            for ln in self.lanes.values():
+                for reqs in ln.inflight.values():
+                    for r in reqs:
+                        r.error = e
+                        r._ev.set()
+                for r in ln.pending:
+                    r.error = e
+                    r._ev.set()
+
+    def _prepare(self, ln: _Lane, buf: int, c: int) -> None:
+        if (buf, c) in ln.ready:
+            return
+        batch = ln.batches[buf]
+        batch.run(c, STAGE_EXPAND, ln.streams[buf].cuda_stream)
+        batch.tune(c)
+        ln.ready.add((buf, c))
+
+    def _launch(self, ln: _Lane) -> None:
+        buf = ln.free.popleft()
+        n = min(ln.capacity, len(ln.pending))
+        reqs = [ln.pending.popleft() for _ in range(n)]
+        batch = ln.batches[buf]
+        try:
+            batch.load_many(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs],
+                            [r.v_folding for r in reqs])
+            c = size_class(n, ln.capacity)
+            self._prepare(ln, buf, c)
+            batch.graph_launch(c, ln.streams[buf].cuda_stream)
+        except Exception as e:
+            for r in reqs:
+                r.error = e
+                r._ev.set()
+            ln.free.append(buf)
+            return
+        t = time.perf_counter()
+        for r in reqs:
+            r.t_launch = t
+        ln.inflight[buf] = reqs
+        ln.launched_batches += 1
+        ln.launched_queries += n
+
+    def _complete(self, ln: _Lane) -> bool:
+        done_any = False
+        for buf in list(ln.inflight):
+            batch = ln.batches[buf]
+            if not batch.poll():
+                continue
+            reqs = ln.inflight.pop(buf)
+            resp = batch.responses(len(reqs))
+            t = time.perf_counter()
+            for i, r in enumerate(reqs):
+                r.response = resp[i]
+                r.t_done = t
+                r._ev.set()
+            ln.free.append(buf)
+            done_any = True
+        return done_any
+
+
+def pinned_u32(n_words: int) -> np.ndarray:
+    """A host array in pinned memory (asynchronous DMA for bulk loads)."""
+    import torch
+
+    return torch.empty(n_words, dtype=torch.int32, pin_memory=True).numpy().view(np.uint32)
+
+
+def poisson_times(rate: float, duration: float, rng: np.random.Generator) -> np.ndarray:
+    """Arrival offsets (s) of a Poisson process of `rate` per second over `duration` s."""
+    n = max(1, int(rate * duration * 1.5) + 10)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    return t[t < duration]
+
+
+def drive(server: Server, arrivals: list[tuple[float, str, object, np.ndarray, np.ndarray, object]]) -> list[Request]:
+    """Submit (offset_s, lane, client, query, v_folding, tag) at their offsets from now
+    (sleeping, then spinning for the last 200 us) and return the requests."""
+    t0 = time.perf_counter()
+    out = []
+    for off, lane, client, q, vf, tag in arrivals:
+        while True:
+            dt = t0 + off - time.perf_counter()
+            if dt <= 0:
+                break
+            if dt > 2e-4:
+                time.sleep(dt - 2e-4)
+        out.append(server.submit(lane, client, q, vf, tag))
+    return out
+
+
+def latency_stats(reqs: list[Request]) -> dict:
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    t_first = min(r.t_submit for r in reqs)
+    t_last = max(r.t_done for r in reqs)
+    return {"completed": len(reqs), "sustained_qps": len(reqs) / (t_last - t_first),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}

@@ -194,6 +194,8 @@ int sp_batch_graph_launch(sp_batch *b, int n, void *stream);
 int sp_batch_read(sp_batch *b, int slot, int what, uint32_t *host_out);
 int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out);
 int sp_batch_synchronize(sp_batch *b);
+/* *done = 1 when the batch's last run has finished, 0 while it runs (no wait). */
+int sp_batch_poll(sp_batch *b, int *done);
 /* Autotuner: times the database multiply of n queries with the VALU u64 and
  * the int8-limb MFMA kernels (the latter needs 2^nu1 and 2^nu2 >= 32), keeps
  * the faster for later runs of n queries and reports both times (ms; -1 when
@@ -201,6 +203,36 @@ int sp_batch_synchronize(sp_batch *b);
  * batch's expansion has run once; it overwrites the accumulators. */
 int sp_batch_tune(sp_batch *b, int n, float *ms_valu, float *ms_mfma, int *chosen);
 int sp_batch_destroy(sp_batch *b);
+
+/* Client expansion keys resident on the device, for one database shape and
+ * gadget (the serving path keeps them for the client's session).  host_left /
+ * host_right as for sp_expand.  No reference counterpart: the reference's
+ * PublicParameters (gpu-pir/structures.cuh:173-177) live in its one test
+ * query's storage. */
+typedef struct sp_keys sp_keys;
+int sp_keys_create(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_left, const uint32_t *host_right,
+                   sp_keys **out);
+int sp_keys_destroy(sp_keys *k);
+/* Loads slots [first, first + n) of a batch in one go: host_queries[i] the
+ * query ciphertext of slot first + i ([4N], NTT natural order), keys[i] its
+ * resident keys (same database and gadget as the batch; copied device to
+ * device), host_v_folding[i] its fold vectors ([nu2][2][2 t_gsw][2][N]).
+ * Waits for the batch's previous run; returns when the host buffers may be
+ * reused (pinned buffers make the transfers asynchronous DMA). */
+int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *host_queries, sp_keys *const *keys,
+                       const uint32_t *const *host_v_folding);
+
+/* Several batches (each against its own database, e.g. of different shapes)
+ * answered by one launch: sp_group_launch runs all stages of slots
+ * [0, ns[i]) of every batch i with ns[i] > 0 as one hipGraph whose branches
+ * (one per batch) run concurrently, captured on first use for this ns and
+ * replayed on `stream`.  The batches' own readouts and loads synchronize with
+ * it.  No reference counterpart (the reference answers one query at a time). */
+typedef struct sp_group sp_group;
+int sp_group_create(sp_batch *const *batches, int count, sp_group **out);
+int sp_group_launch(sp_group *g, const int *ns, void *stream);
+int sp_group_synchronize(sp_group *g);
+int sp_group_destroy(sp_group *g);
 
 /* One query against a resident database, stage by stage on the device:
  *   query        NTT Ciphertext (natural order)

@@ -1,0 +1,150 @@
+"""The serving loop's host logic on the CPU: size classes, slot sizing, Poisson
+arrivals, batching order and error handling, with stand-in batch objects in
+place of device batches (no GPU is touched)."""
+from __future__ import annotations
+
+import collections
+import threading
+
+import numpy as np
+import pytest
+
+
+class FakeStream:
+    cuda_stream = 0
+
+
+class FakeBatch:
+    """Answers each slot with a vector filled with the query's first word."""
+
+    def __init__(self, fail_load: bool = False, fail_after: int | None = None):
+        self.fail_load = fail_load
+        self.fail_after = fail_after
+        self.loaded = []
+        self.launches = []
+        self.tuned = []
+
+    def load_many(self, first, queries, keys, v_folding):
+        if self.fail_load or (self.fail_after is not None and len(self.launches) >= self.fail_after):
+            raise RuntimeError("injected load failure")
+        assert first == 0 and len(queries) == len(keys) == len(v_folding)
+        self.loaded = [int(q[0]) for q in queries]
+
+    def run(self, n, stages, stream=None):
+        pass
+
+    def tune(self, n):
+        self.tuned.append(n)
+        return {}
+
+    def graph_launch(self, n, stream=None):
+        self.launches.append((n, len(self.loaded)))
+
+    def poll(self):
+        return True
+
+    def responses(self, n):
+        return np.stack([np.full(4, v, np.uint32) for v in self.loaded[:n]])
+
+
+def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
+    from spiral_pir.serving import Server, _Lane
+
+    srv = Server.__new__(Server)
+    srv.lanes = {}
+    for name, batches in batches_per_lane.items():
+        ln = _Lane.__new__(_Lane)
+        ln.name, ln.db, ln.capacity = name, None, capacity
+        ln.batches = batches
+        ln.streams = [FakeStream() for _ in batches]
+        ln.free = collections.deque(range(len(batches)))
+        ln.inflight, ln.pending, ln.keys, ln.ready = {}, collections.deque(), {"c": object()}, set()
+        ln.launched_batches = ln.launched_queries = 0
+        srv.lanes[name] = ln
+    srv.admission = {}
+    srv.poll_interval = 1e-4
+    srv._queue = collections.deque()
+    srv._lock = threading.Lock()
+    srv._wake = threading.Event()
+    srv._stop = False
+    srv._thread = None
+    srv._error = None
+    return srv
+
+
+def _q(v):
+    return np.array([v, 0, 0, 0], np.uint32)
+
+
+def test_size_classes():
+    from spiral_pir.serving import size_class
+
+    assert [size_class(n, 128) for n in (1, 2, 3, 5, 9, 33, 100, 128)] == [1, 2, 4, 8, 16, 64, 128, 128]
+    assert size_class(5, 6) == 6 and size_class(6, 6) == 6 and size_class(3, 6) == 4
+
+
+def test_slot_bytes_grow_with_shape():
+    from spiral_pir import Shape
+    from spiral_pir.serving import slot_device_bytes
+
+    a, b = slot_device_bytes(Shape(9, 6)), slot_device_bytes(Shape(10, 7))
+    assert 60 * 2**20 < a < b < 200 * 2**20
+
+
+def test_poisson_rate():
+    from spiral_pir.serving import poisson_times
+
+    t = poisson_times(1000.0, 5.0, np.random.default_rng(3))
+    assert abs(len(t) - 5000) < 300 and np.all(np.diff(t) > 0) and t[-1] < 5.0
+
+
+def test_requests_are_batched_and_answered_in_order():
+    srv = _server({"x": [FakeBatch(), FakeBatch()]}, capacity=4)
+    reqs = [srv.submit("x", "c", _q(i), np.zeros(2, np.uint32), tag=i) for i in range(10)]
+    srv.start()
+    for r in reqs:
+        assert int(r.wait(5)[0]) == r.tag
+    srv.stop()
+    ln = srv.lanes["x"]
+    assert ln.launched_queries == 10 and ln.launched_batches == 3  # 4 + 4 + 2
+    assert sorted(n for b in ln.batches for n, _ in b.launches) == [2, 4, 4]
+
+
+def test_load_failure_fails_its_requests_only():
+    bad, good = FakeBatch(fail_load=True), FakeBatch()
+    srv = _server({"bad": [bad], "good": [good]}, capacity=8)
+    r_bad = srv.submit("bad", "c", _q(1), np.zeros(2, np.uint32))
+    r_good = srv.submit("good", "c", _q(2), np.zeros(2, np.uint32))
+    srv.start()
+    with pytest.raises(RuntimeError, match="injected"):
+        r_bad.wait(5)
+    assert int(r_good.wait(5)[0]) == 2
+    # the lane recovers its buffer and keeps serving
+    r_again = srv.submit("good", "c", _q(3), np.zeros(2, np.uint32))
+    assert int(r_again.wait(5)[0]) == 3
+    srv.stop()
+    assert srv.lanes["bad"].free and not srv.lanes["bad"].inflight
+
+
+def test_unknown_lane_or_client_is_rejected():
+    srv = _server({"x": [FakeBatch()]})
+    with pytest.raises(KeyError):
+        srv.submit("y", "c", _q(0), np.zeros(2, np.uint32))
+    with pytest.raises(KeyError):
+        srv.submit("x", "nobody", _q(0), np.zeros(2, np.uint32))
+
+
+def test_wait_times_out_when_never_answered():
+    from spiral_pir.serving import Request
+
+    r = Request("x", "c", _q(0), np.zeros(2, np.uint32))
+    with pytest.raises(TimeoutError):
+        r.wait(0.01)
+
+
+def test_stop_drains_pending_requests():
+    srv = _server({"x": [FakeBatch()]}, capacity=2)
+    srv.start()
+    reqs = [srv.submit("x", "c", _q(i), np.zeros(2, np.uint32), tag=i) for i in range(7)]
+    srv.stop()
+    assert all(r.done and int(r.response[0]) == r.tag for r in reqs)
