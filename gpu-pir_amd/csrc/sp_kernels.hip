@@ -278,8 +278,10 @@ __global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ b
 // First-dimension DB multiply (single query), internal layouts:
 //   db  [T][m][R][C][P]      sel j at sel + j*sel_stride: [row][m][P]
 //   out [T][C][row][m][P]
-// Block = 4 waves splitting the R rows; a lane owns 2 positions and CG
-// columns, accumulating sel*db lazily in u64 (folded every 128 rows).
+// Block = WPB waves splitting the R rows; a lane owns 2 positions and CG
+// columns, accumulating sel*db lazily in u64 (folded every 128 rows).  The
+// database is streamed once with non-temporal loads (no reuse to cache);
+// the selectors are reused by every column group and trial and stay cached.
 // Blocks are numbered so that the 8 XCDs each own a quarter of the
 // (residue, position-block) selector groups, keeping selector re-reads in
 // one XCD's L2.
@@ -289,12 +291,15 @@ __device__ __forceinline__ uint32_t mod_u64(uint64_t x, const ModConst &mc) {
   return raw_mod((uint32_t)(x >> 32), (uint32_t)x, mc);
 }
 
-template <int CG>
-__global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db, const uint32_t *__restrict__ sel_all,
-                                                size_t sel_stride, size_t sel_qstride, uint32_t *__restrict__ out_all,
-                                                size_t out_qstride, int nq, int T, int R, int C, ModConst mc0,
-                                                ModConst mc1) {
-  __shared__ uint32_t red[4][CG][2][128];
+typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
+
+template <int CG, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_db_mul(const uint32_t *__restrict__ db,
+                                                     const uint32_t *__restrict__ sel_all, size_t sel_stride,
+                                                     size_t sel_qstride, uint32_t *__restrict__ out_all,
+                                                     size_t out_qstride, int nq, int T, int R, int C, ModConst mc0,
+                                                     ModConst mc1) {
+  __shared__ uint32_t red[WPB][CG][2][128];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ncg = C / CG;
   const int bid = blockIdx.x;
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db,
   const int m = sg >> 4, pb = sg & 15;
   const ModConst mc = m ? mc1 : mc0;
   const int p0 = pb * 128 + lane * 2;
-  const int rows_per_wave = R / 4;
+  const int rows_per_wave = R / WPB;
   const int j0 = wave * rows_per_wave;
 
   uint64_t acc[CG][2][2];
@@ -335,7 +340,7 @@ __global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db,
       const uint32_t *dp = dbase + (size_t)j * C * N;
 #pragma unroll
       for (int c = 0; c < CG; c++) {
-        const uint2 d = *reinterpret_cast<const uint2 *>(dp + (size_t)c * N);
+        const v2u32 d = __builtin_nontemporal_load(reinterpret_cast<const v2u32 *>(dp + (size_t)c * N));
         acc[c][0][0] += (uint64_t)s0.x * d.x;
         acc[c][1][0] += (uint64_t)s0.y * d.y;
         acc[c][0][1] += (uint64_t)s1.x * d.x;
@@ -360,14 +365,11 @@ __global__ __launch_bounds__(256) void k_db_mul(const uint32_t *__restrict__ db,
       red[wave][c][row][lane * 2 + 1] = mod_u64(acc[c][1][row], mc);
     }
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < CG; k++) {
-    const int idx = k * 256 + threadIdx.x;
+  for (int idx = threadIdx.x; idx < CG * 256; idx += WPB * 64) {
     const int c = idx >> 8, row = (idx >> 7) & 1, pos = idx & 127;
-    uint32_t s = red[0][c][row][pos] + red[1][c][row][pos];
-    s = canon(s, mc.q);
-    s = canon(s + red[2][c][row][pos], mc.q);
-    s = canon(s + red[3][c][row][pos], mc.q);
+    uint32_t s = red[0][c][row][pos];
+#pragma unroll
+    for (int w = 1; w < WPB; w++) s = canon(s + red[w][c][row][pos], mc.q);
     out[((((size_t)tt * C + cg * CG + c) * 2 + row) * 2 + m) * N + pb * 128 + pos] = s;
   }
 }
@@ -722,14 +724,24 @@ hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_str
                          hipStream_t s) {
   const int cg = C >= 8 ? 8 : C;
   const unsigned blocks = (unsigned)(32 * T * (C / cg) * nq);
-#define SP_DBMUL(CGV)                                                                                        \
-  k_db_mul<CGV><<<blocks, 256, 0, s>>>(db, sel, sel_stride, sel_qstride, out, out_qstride, nq, T, R, C, t.mc[0], \
-                                      t.mc[1])
-  switch (cg) {
-    case 8: SP_DBMUL(8); break;
-    case 4: SP_DBMUL(4); break;
-    case 2: SP_DBMUL(2); break;
-    default: SP_DBMUL(1); break;
+  // 8 waves split the rows (R >= 8); the smallest databases (R = 4) use 4.
+#define SP_DBMUL(CGV, W)                                                                                      \
+  k_db_mul<CGV, W><<<blocks, W * 64, 0, s>>>(db, sel, sel_stride, sel_qstride, out, out_qstride, nq, T, R, C, \
+                                             t.mc[0], t.mc[1])
+  if (R >= 8) {
+    switch (cg) {
+      case 8: SP_DBMUL(8, 8); break;
+      case 4: SP_DBMUL(4, 8); break;
+      case 2: SP_DBMUL(2, 8); break;
+      default: SP_DBMUL(1, 8); break;
+    }
+  } else {
+    switch (cg) {
+      case 8: SP_DBMUL(8, 4); break;
+      case 4: SP_DBMUL(4, 4); break;
+      case 2: SP_DBMUL(2, 4); break;
+      default: SP_DBMUL(1, 4); break;
+    }
   }
 #undef SP_DBMUL
   return hipGetLastError();
