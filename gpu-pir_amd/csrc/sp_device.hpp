@@ -108,14 +108,17 @@ __device__ __forceinline__ void ct_bf_full(uint32_t &x, uint32_t &y, uint32_t w,
 }
 
 // Stage st (1..11) of a forward transform whose first LAZY stages skip the
-// reduction of x; stage LAZY + 1 reduces x fully, later stages are Harvey's.
-template <int LAZY>
+// reduction of x; stage LAZY + 1 reduces x fully; later stages are lazy again
+// when TAIL (values then grow from 4q by 2q per stage), else Harvey's.
+template <int LAZY, bool TAIL>
 __device__ __forceinline__ void ct_stage_bf(int st, uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
                                             uint32_t mu1) {
   if (st <= LAZY)
     ct_bf_lazy(x, y, w, wp, q);
   else if (LAZY > 0 && st == LAZY + 1)
     ct_bf_full(x, y, w, wp, q, mu1);
+  else if (TAIL)
+    ct_bf_lazy(x, y, w, wp, q);
   else
     ct_bf(x, y, w, wp, q);
 }
@@ -171,12 +174,17 @@ __device__ __forceinline__ uint32_t tw_b(const uint32_t *__restrict__ tw, int sl
 // In:  r[i] = coefficient 64*i + lane, values in [0, 4q).
 // Out: r[j] = NTT position 32*lane + j (natural Spiral order), in [0, 4q).
 // LAZY > 0: the first LAZY stages skip the reduction of x, which is valid
-// when the inputs are below B q with B + 2 LAZY <= 16 (both moduli satisfy
-// 16 q < 2^32): up to 7 for inputs below 2q.  mu1 = floor(2^32 / q).
-template <int LAZY>
+// while the values stay below 2^32: inputs below B q grow to (B + 2 LAZY) q,
+// and both moduli satisfy 16 q + 128 < 2^32 (so 8 stages for digits below
+// 128, 7 for inputs below 2q).  Stage LAZY + 1 reduces x fully (outputs
+// < 4q); with TAIL the remaining 11 - LAZY - 1 stages are lazy too and the
+// outputs are below (4 + 2 (10 - LAZY)) q, else Harvey's bound 4q holds.
+// mu1 = floor(2^32 / q).
+template <int LAZY, bool TAIL = false>
 __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                                uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane) {
-  static_assert(LAZY >= 0 && LAZY <= 7, "at most 7 lazy stages");
+  static_assert(LAZY >= 0 && LAZY <= 8, "at most 8 lazy stages");
+  static_assert(!TAIL || (LAZY > 0 && 4 + 2 * (10 - LAZY) <= 15), "lazy tail bound");
   // Phase A: strides 1024 .. 64 (index bits 10..6 live in the register index).
 #pragma unroll
   for (int s = 0; s < 5; s++) {
@@ -186,7 +194,7 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
       const uint32_t w = tw_a(tw, (1 << s) + blk, 0), wp = tw_a(tw, (1 << s) + blk, 1);
 #pragma unroll
       for (int i = 0; i < half; i++)
-        ct_stage_bf<LAZY>(1 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
+        ct_stage_bf<LAZY, TAIL>(1 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
     }
   }
   // Transpose to phase B through the wave's LDS tile.
@@ -207,7 +215,7 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
       const uint32_t w = tw_b(tw, slot + blk, 0, lane), wp = tw_b(tw, slot + blk, 1, lane);
 #pragma unroll
       for (int i = 0; i < half; i++)
-        ct_stage_bf<LAZY>(6 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
+        ct_stage_bf<LAZY, TAIL>(6 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
     }
     slot += (1 << s);
   }
@@ -220,7 +228,7 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
     const uint32_t recv = swap_lane_pair(send);
     uint32_t x = b ? recv : r[k];
     uint32_t y = b ? r[16 + k] : recv;
-    ct_bf(x, y, tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
+    ct_stage_bf<LAZY, TAIL>(11, x, y, tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q, mu1);
     o[2 * k] = x;
     o[2 * k + 1] = y;
   }

@@ -817,13 +817,18 @@ namespace sp {
 
 constexpr int DIG_WORDS = N / 4;  // 512 words per byte-packed digit polynomial
 
-__device__ __forceinline__ void store_tile_p(uint32_t *__restrict__ tile, const uint32_t (&r)[32], int lane,
-                                             uint32_t two_q) {
+// Tile store for transform outputs below 2^32: each value is reduced to
+// [0, 2q) by a Shoup multiplication by 1.
+__device__ __forceinline__ void store_tile_p_full(uint32_t *__restrict__ tile, const uint32_t (&r)[32], int lane,
+                                                  uint32_t q, uint32_t mu1) {
   uint4 *t4 = reinterpret_cast<uint4 *>(tile);
 #pragma unroll
-  for (int c = 0; c < 8; c++)
-    t4[c * 64 + lane] = make_uint4(reduce_2q(r[4 * c], two_q), reduce_2q(r[4 * c + 1], two_q),
-                                   reduce_2q(r[4 * c + 2], two_q), reduce_2q(r[4 * c + 3], two_q));
+  for (int c = 0; c < 8; c++) {
+    uint32_t v[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) v[e] = r[4 * c + e] - __umulhi(r[4 * c + e], mu1) * q;
+    t4[c * 64 + lane] = make_uint4(v[0], v[1], v[2], v[3]);
+  }
 }
 
 // Fold and expansion kernels come in two shapes:
@@ -887,9 +892,9 @@ __device__ __forceinline__ void fold_residue(const uint32_t (&dw)[16 / NW][8], u
       for (int b = 0; b < 4; b++) r[4 * i4 + b] = (uint32_t)((int32_t)(int8_t)(dw[h][i4] >> (8 * b)) + (int32_t)mc.q);
     const uint32_t *twm = tw_base + (M * 2) * TW_WORDS;
     asm volatile("" : "+s"(twm));  // re-read per pass, not held in registers
-    // inputs in (q - 128, q + 128) < 2q: the first 7 stages need no reduction
-    ntt_fwd_wave_t<7>(r, twm, tiles[wave], mc.q, mc.mu1, lane);
-    store_tile_p(tiles[wave], r, lane, 2 * mc.q);
+    // inputs in (q - 128, q + 128) < 2q: stages 1-7 and 9-11 skip the reduction (outputs < 10q)
+    ntt_fwd_wave_t<7, true>(r, twm, tiles[wave], mc.q, mc.mu1, lane);
+    store_tile_p_full(tiles[wave], r, lane, mc.q, mc.mu1);
     __syncthreads();
 #pragma unroll
     for (int w2 = 0; w2 < NW; w2++) {
@@ -1121,9 +1126,10 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
         for (int b = 0; b < 4; b++) rr[4 * i4 + b] = (dw[i4] >> (8 * b)) & 0xFFu;
       const uint32_t *twm = tw_base + (m * 2) * TW_WORDS;
       asm volatile("" : "+s"(twm));  // re-read per pass, not held in registers
-      // digits < 128: the first 7 stages need no reduction
-      ntt_fwd_wave_t<7>(rr, twm, tiles[wave], q, m ? mc1.mu1 : mc0.mu1, lane);
-      store_tile_p(tiles[wave], rr, lane, 2 * q);
+      // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
+      const uint32_t mu1 = m ? mc1.mu1 : mc0.mu1;
+      ntt_fwd_wave_t<8, true>(rr, twm, tiles[wave], q, mu1, lane);
+      store_tile_p_full(tiles[wave], rr, lane, q, mu1);
     }
     __syncthreads();
     // residue updated by this thread, and its positions
@@ -1136,16 +1142,20 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
 
 // Waves per block for a grid of `blocks` working blocks: 16 (one block per CU,
 // half the passes) while the grid does not fill the CUs, else 8 (two per CU).
-static int fold_expand_waves(long blocks, const DevTables &t, int variant) {
+// The fold takes 16 at every size: its 16-wave block (two passes, 16 tiles
+// per multiply-accumulate) measured faster than two 8-wave blocks per CU
+// also on full grids (64 queries at (9,6): 3.19 vs 3.61 ms per fold stage).
+static int fold_expand_waves(long blocks, const DevTables &t, int variant, bool fold) {
   if (variant == 2) return 8;
   if (variant == 3) return 16;
+  if (fold) return 16;
   return blocks <= t.n_cu ? 16 : 8;
 }
 
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
                         const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, hipStream_t s) {
   dim3 grid(num_per, n_trials, n_queries);
-  if (fold_expand_waves((long)num_per * n_trials * n_queries, t, variant) == 16)
+  if (fold_expand_waves((long)num_per * n_trials * n_queries, t, variant, true) == 16)
     k_fold8<16><<<grid, 1024, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0],
                                       t.mc[1]);
   else
@@ -1161,7 +1171,7 @@ hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, in
   if (stop > 0 && r > stop) work = num_out / 2;
   else if (stop > 0 && r == stop) work = num_out / 2 + (num_out / 2 < max_bits ? num_out / 2 : max_bits);
   dim3 grid(work, n_queries);
-  if (fold_expand_waves((long)work * n_queries, t, variant) == 16)
+  if (fold_expand_waves((long)work * n_queries, t, variant, false) == 16)
     k_expand8<16><<<grid, 1024, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride,
                                         t.tw, t.mc[0], t.mc[1]);
   else
