@@ -208,7 +208,8 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
 }
 
 extern "C" int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant) {
-  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > (stage == SP_STAGE_DB_MUL ? 2 : 3))
+  if (!ctx || stage < 0 || stage > 2 || variant < 0 ||
+      variant > (stage == SP_STAGE_DB_MUL ? 2 : stage == SP_STAGE_FOLD ? 3 : 4))
     return fail(SP_ERR_ARG, "sp_ctx_set_variant: stage %d variant %d", stage, variant);
   ctx->variant[stage] = variant;
   return SP_OK;
@@ -361,24 +362,50 @@ extern "C" int sp_expansion_depth(int nu1, int nu2, int t_gsw, int *g, int *stop
 
 // Runs `rounds` expansion rounds on internal-layout v (n_queries sets; query
 // b's ciphertexts at v + b*q_stride, its keys at left/right + b*stride).
+// Largest number of working steps of any expansion round (digit scratch rows).
+static int expansion_max_work(int nu1, int nu2, int t_gsw) {
+  int g = 0, stop = 0;
+  if (sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop)) return 0;
+  int mx = 0;
+  for (int r = 0; r < g; r++) mx = std::max(mx, expand_split_work(r, stop, t_gsw * nu2));
+  return mx;
+}
+
+// Expansion kernel family for a round: 1 = one 4-wave block per step (any
+// gadget), 2/3 = fused 8/16-wave step kernels, 4 = split digits + update
+// kernels (needs the digit scratch).  Automatic: the split form when scratch
+// is there, else the fused one.
+static int expansion_family(const sp_ctx *ctx, int t_exp, const uint32_t *dig) {
+  const int var = ctx->variant[SP_STAGE_EXPAND];
+  if (t_exp != 8 || var == 1) return 1;
+  if (var == 2 || var == 3) return var;
+  return dig ? 4 : 2;
+}
+
 static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_queries, int nu1, int nu2, int t_gsw,
                          int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t left_qstride,
-                         size_t right_qstride, int rounds, hipStream_t s) {
+                         size_t right_qstride, int rounds, uint32_t *dig, size_t dig_qstride, hipStream_t s) {
   int g = 0, stop = 0;
   int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
   if (rc) return rc;
   if (rounds < 0 || rounds > g) rounds = g;
   const size_t key_level = (size_t)2 * t_exp * 2 * N;
+  const int family = expansion_family(ctx, t_exp, dig);
+  if (family == 4 && !dig) return fail(SP_ERR_STATE, "split expansion without digit scratch");
   for (int r = 0; r < rounds; r++) {
     const int num_in = 1 << r;
-    HIP_TRY(launch_expand_neg1(v, q_stride, n_queries, num_in, ctx->tables.neg1 + (size_t)r * 2 * 2 * N,
-                               ctx->tables, s));
+    const uint32_t *neg1_r = ctx->tables.neg1 + (size_t)r * 2 * 2 * N;
     const uint32_t *lr = left_p + (size_t)r * key_level;
     const uint32_t *rr = right_p + (size_t)std::min(r, stop) * key_level;
-    const bool rounds8 = t_exp == 8 && ctx->variant[SP_STAGE_EXPAND] != 1;
-    if (rounds8)
+    if (family == 4) {
+      HIP_TRY(launch_expand_split(v, q_stride, n_queries, r, stop, t_gsw * nu2, lr, rr, left_qstride, right_qstride,
+                                  neg1_r, dig, dig_qstride, ctx->tables, s));
+      continue;
+    }
+    HIP_TRY(launch_expand_neg1(v, q_stride, n_queries, num_in, neg1_r, ctx->tables, s));
+    if (family != 1)
       HIP_TRY(launch_expand8(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
-                             right_qstride, ctx->tables, ctx->variant[SP_STAGE_EXPAND], s));
+                             right_qstride, ctx->tables, family == 3 ? 3 : ctx->variant[SP_STAGE_EXPAND], s));
     else
       HIP_TRY(launch_expand_action(v, q_stride, n_queries, r, 2 * num_in, stop, t_gsw * nu2, lr, rr, left_qstride,
                                    right_qstride, t_exp, gadget_bits(t_exp), ctx->tables, s));
@@ -405,7 +432,10 @@ extern "C" int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_g
   HIP_TRY(launch_nat_to_p(dev_v, v.u32(), n_ct * 4, 1, 0, ctx->tables, s));
   HIP_TRY(launch_nat_to_p(dev_left, kl.u32(), (size_t)g * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
   HIP_TRY(launch_nat_to_p(dev_right, kr.u32(), (size_t)(stop + 1) * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
-  rc = run_expansion(ctx, v.u32(), 0, 1, nu1, nu2, t_gsw, t_exp, kl.u32(), kr.u32(), 0, 0, rounds, s);
+  DevBuf dig;
+  if (t_exp == 8 && (ctx->variant[SP_STAGE_EXPAND] == 0 || ctx->variant[SP_STAGE_EXPAND] == 4))
+    HIP_TRY(dig.alloc((size_t)expansion_max_work(nu1, nu2, t_gsw) * 8 * 512 * 4));
+  rc = run_expansion(ctx, v.u32(), 0, 1, nu1, nu2, t_gsw, t_exp, kl.u32(), kr.u32(), 0, 0, rounds, dig.u32(), 0, s);
   if (rc) return rc;
   HIP_TRY(launch_p_to_nat(v.u32(), dev_v, n_ct * 4, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -530,6 +560,8 @@ struct sp_batch {
   std::vector<std::pair<int, hipGraphExec_t>> graphs;
   std::vector<hipGraph_t> graph_defs;
   uint32_t *bulk = nullptr;       // bulk-load staging: [cap][4N] queries then [cap][f_q] v_folding
+  uint32_t *dig = nullptr;        // split-expansion digit scratch [cap][dig_q]
+  size_t dig_q = 0;
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
@@ -546,8 +578,8 @@ static void batch_free(sp_batch *b) {
   if (!b) return;
   for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
   for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
-  uint32_t *bufs[] = {b->query_ct, b->v,     b->left,     b->right, b->fkeys,
-                      b->fkeys_neg, b->acc, b->stage, b->mfma_out, b->bulk};
+  uint32_t *bufs[] = {b->query_ct,  b->v,   b->left,  b->right,    b->fkeys, b->fkeys_neg,
+                      b->acc, b->stage, b->mfma_out, b->bulk, b->dig};
   for (uint32_t *p : bufs)
     if (p) (void)hipFree(p);
   if (b->mfma_a) (void)hipFree(b->mfma_a);
@@ -593,6 +625,10 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   A(&b->fkeys, cap * b->f_q);
   A(&b->acc, cap * b->acc_q);
   A(&b->stage, b->stage_words);
+  if (t_exp == 8) {  // digit scratch of the split expansion
+    b->dig_q = (size_t)expansion_max_work(db->nu1, db->nu2, t_gsw) * 8 * 512;
+    A(&b->dig, cap * b->dig_q);
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->io, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
@@ -786,7 +822,7 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
   if (stages & 1) {
     HIP_TRY(launch_copy_queries(b->query_ct, 4 * N, b->v, b->v_q, n, s));
     int rc = run_expansion(ctx, b->v, b->v_q, n, db->nu1, db->nu2, b->t_gsw, b->t_exp, b->left, b->right, b->l_q,
-                           b->r_q, -1, s);
+                           b->r_q, -1, b->dig, b->dig_q, s);
     if (rc) return rc;
   }
   if (stages & 2) {

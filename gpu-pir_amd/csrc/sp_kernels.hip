@@ -1140,6 +1140,156 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
   }
 }
 
+// ---- split expansion round: digits kernel + update kernel -------------------
+// Round r, gadget 2^7 x 8, in two launches so that one step spreads over
+// three blocks (one for its digits, one per residue for its transforms).
+//
+// k_expand_digits: one 2-wave block per element i of the round.  For
+// i >= 2^r the element is formed first as v[i] = v_neg1[r] * v[i - 2^r] (all
+// four polynomials, written back; the reference computes the product for
+// every i, expansion.cuh:171-219).  For a working step, wave m then takes
+// residue m of v_i.a out of the NTT domain, and the block forms the CRT
+// value, applies tau_t (a negated zero becomes Q) and writes the 8 digit
+// bytes of every coefficient to dig[b] (b = the step's index in the working
+// order of k_expand8, layout as the digit bytes there).  Grid x: [0, work)
+// the working steps, then `extra` elements that only take the v_neg1
+// product: the odd i >= 2^r skipped at this round, i = odd_start + 2e + 1.
+__global__ __launch_bounds__(128) void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
+                                                       int work, int odd_start, const uint32_t *__restrict__ neg1_r,
+                                                       uint32_t *__restrict__ dig, size_t dig_qstride,
+                                                       const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                       ModConst mc1) {
+  __shared__ uint32_t xm[2 * N];
+  __shared__ uint32_t tiles[2][LDS_TILE];
+  const int b = blockIdx.x, half = 1 << r;
+  const bool step = b < work;
+  const int i = !step ? odd_start + 2 * (b - work) + 1
+                      : ((stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b);
+  const int tid = threadIdx.x;
+  const int m = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const ModConst mc = m ? mc1 : mc0;
+  uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
+  uint32_t *vi = vq + (size_t)i * 4 * N;
+  uint32_t a[32];
+  if (i >= half) {
+    // wave m: rows 0 and 1 of residue m of v_neg1 * v[i - 2^r]
+    const uint32_t *src = vq + (size_t)(i - half) * 4 * N;
+    const uint4 *w4 = reinterpret_cast<const uint4 *>(neg1_r + (size_t)m * 2 * N);
+    const uint4 *wp4 = reinterpret_cast<const uint4 *>(neg1_r + (size_t)m * 2 * N + N);
+#pragma unroll
+    for (int row = 0; row < 2; row++) {
+      const uint4 *s4 = reinterpret_cast<const uint4 *>(src + (size_t)(row * 2 + m) * N);
+      uint4 *d4 = reinterpret_cast<uint4 *>(vi + (size_t)(row * 2 + m) * N);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint4 x = s4[c * 64 + lane], w = w4[c * 64 + lane], wp = wp4[c * 64 + lane];
+        uint4 y;
+        y.x = canon(shoup_mul(x.x, w.x, wp.x, mc.q), mc.q);
+        y.y = canon(shoup_mul(x.y, w.y, wp.y, mc.q), mc.q);
+        y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
+        y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
+        d4[c * 64 + lane] = y;
+        if (row == 0) { a[4 * c] = y.x; a[4 * c + 1] = y.y; a[4 * c + 2] = y.z; a[4 * c + 3] = y.w; }
+      }
+    }
+  } else {
+    load_p(a, vi + (size_t)m * N, lane);
+  }
+  if (!step) return;  // uniform per block
+  ntt_inv_wave(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, lane);
+#pragma unroll
+  for (int ii = 0; ii < 32; ii++) xm[m * N + 64 * ii + lane] = canon(shoup_mul(a[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  __syncthreads();
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  uint8_t *db = reinterpret_cast<uint8_t *>(&tiles[0][0]);  // 4096 words over tiles 0-1
+  for (int z = tid; z < N; z += 128) {
+    uint64_t x = (uint64_t)xm[z] * Q1 + (uint64_t)xm[N + z] * Q0;
+    x = (x >= QQ) ? x - QQ : x;
+    const uint32_t zt = (uint32_t)z * t;
+    const uint32_t dst = zt & (N - 1);
+    const uint64_t y = ((zt >> LOGN) & 1) ? QQ - x : x;
+    const uint32_t word = (dst >> 8) * 64 + (dst & 63), byte = (dst >> 6) & 3;
+#pragma unroll
+    for (int k = 0; k < 8; k++) db[((size_t)k * DIG_WORDS + word) * 4 + byte] = (uint8_t)((y >> (7 * k)) & 127u);
+  }
+  __syncthreads();
+  const uint4 *src4 = reinterpret_cast<const uint4 *>(&tiles[0][0]);
+  uint4 *dst4 = reinterpret_cast<uint4 *>(dig + (size_t)blockIdx.y * dig_qstride + (size_t)b * 8 * DIG_WORDS);
+  for (int u = tid; u < 8 * DIG_WORDS / 4; u += 128) dst4[u] = src4[u];
+}
+
+// k_expand_update: one 8-wave block per (working step b, query, residue m):
+// wave k transforms digit k of the step for residue m; then every thread
+// updates residue m of v_i at its 4 positions (expand_update).
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, const uint32_t *__restrict__ dig,
+                     size_t dig_qstride, const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+                     size_t left_qstride, size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0,
+                     ModConst mc1) {
+  __shared__ uint32_t tiles[8][LDS_TILE];
+  const int b = blockIdx.x, half = 1 << r, m = blockIdx.z;
+  const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  const bool use_left = (r != 0) && ((i & 1) == 0);
+  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
+  const uint32_t q = m ? mc1.q : mc0.q, mu1 = m ? mc1.mu1 : mc0.mu1, qinv = m ? mc1.qinv_neg : mc0.qinv_neg;
+  const uint32_t *dg = dig + (size_t)blockIdx.y * dig_qstride + ((size_t)b * 8 + wave) * DIG_WORDS;
+  uint32_t rr[32];
+#pragma unroll
+  for (int i4 = 0; i4 < 8; i4++) {
+    const uint32_t w = dg[i4 * 64 + lane];
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) rr[4 * i4 + bb] = (w >> (8 * bb)) & 0xFFu;
+  }
+  // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
+  ntt_fwd_wave_t<8, true>(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
+  store_tile_p_full(tiles[wave], rr, lane, q, mu1);
+  __syncthreads();
+  expand_update<4>(vi, W, tiles, 0, m, 4 * tid, t, q, qinv);
+}
+
+// Working steps of round r (grid of k_expand8 / k_expand_update), and the
+// skipped odd elements >= 2^r that still take the v_neg1 product.
+static void round_plan(int r, int stop, int max_bits, int &work, int &odd_start, int &extra) {
+  const int half = 1 << r;
+  work = 2 * half;
+  odd_start = 0;
+  extra = 0;
+  if (stop > 0 && r > stop) {
+    work = half;
+    odd_start = half;
+    extra = half / 2;
+  } else if (stop > 0 && r == stop) {
+    work = half + (half < max_bits ? half : max_bits);
+    odd_start = half > 2 * max_bits ? half : 2 * max_bits;
+    extra = odd_start < 2 * half ? (2 * half - odd_start) / 2 : 0;
+  }
+}
+
+int expand_split_work(int r, int stop, int max_bits) {
+  int work, odd_start, extra;
+  round_plan(r, stop, max_bits, work, odd_start, extra);
+  return work;
+}
+
+hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int stop, int max_bits,
+                               const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
+                               const uint32_t *neg1_r, uint32_t *dig, size_t dig_qstride, const DevTables &t,
+                               hipStream_t s) {
+  int work, odd_start, extra;
+  round_plan(r, stop, max_bits, work, odd_start, extra);
+  k_expand_digits<<<dim3(work + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, work, odd_start, neg1_r, dig,
+                                                                dig_qstride, t.tw, t.mc[0], t.mc[1]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_expand_update<<<dim3(work, n_queries, 2), 512, 0, s>>>(v, q_stride, r, stop, dig, dig_qstride, left, right,
+                                                           left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
 // Waves per block for a grid of `blocks` working blocks: 16 (one block per CU,
 // half the passes) while the grid does not fill the CUs, else 8 (two per CU).
 // The fold takes 16 at every size: its 16-wave block (two passes, 16 tiles
@@ -1167,9 +1317,9 @@ hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, in
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                           const DevTables &t, int variant, hipStream_t s) {
   // working steps per query (k_expand8 maps them back to i)
-  int work = num_out;
-  if (stop > 0 && r > stop) work = num_out / 2;
-  else if (stop > 0 && r == stop) work = num_out / 2 + (num_out / 2 < max_bits ? num_out / 2 : max_bits);
+  int work, odd_start, extra;
+  round_plan(r, stop, max_bits, work, odd_start, extra);
+  (void)num_out;
   dim3 grid(work, n_queries);
   if (fold_expand_waves((long)work * n_queries, t, variant, false) == 16)
     k_expand8<16><<<grid, 1024, 0, s>>>(v, q_stride, r, stop, max_bits, left, right, left_qstride, right_qstride,

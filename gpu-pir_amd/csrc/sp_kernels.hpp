@@ -54,6 +54,15 @@ hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, in
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                           const DevTables &t, int variant, hipStream_t s);
 
+// Expansion round in two launches (digits, then per-residue transforms and
+// update); also forms v[2^r + i] = v_neg1[r] * v[i] (no separate neg1 launch).
+//   dig: scratch, per query dig_qstride >= expand_split_work(r, ...) * 8 * 512 words
+int expand_split_work(int r, int stop, int max_bits);
+hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int stop, int max_bits,
+                               const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
+                               const uint32_t *neg1_r, uint32_t *dig, size_t dig_qstride, const DevTables &t,
+                               hipStream_t s);
+
 // int8-limb MFMA database multiply (C >= 32, R multiple of 32).
 //   D: database limbs [t][m][p][R/32][4][C][32] (k_db_to_limbs, one-time)
 //   A: workspace for selector limbs, 2 * N * R * 4 * I bytes (I = 2*nq padded to 32)
