@@ -208,8 +208,7 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
 }
 
 extern "C" int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant) {
-  if (!ctx || stage < 0 || stage > 2 || variant < 0 ||
-      variant > (stage == SP_STAGE_DB_MUL ? 2 : stage == SP_STAGE_FOLD ? 3 : 4))
+  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > (stage == SP_STAGE_DB_MUL ? 2 : 4))
     return fail(SP_ERR_ARG, "sp_ctx_set_variant: stage %d variant %d", stage, variant);
   ctx->variant[stage] = variant;
   return SP_OK;
@@ -329,12 +328,15 @@ extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, cons
     HIP_TRY(kn.alloc(level_polys * nu2 * N * 4));
     HIP_TRY(launch_nat_to_p(dev_v_folding_neg, kn.u32(), level_polys * nu2, 2 * mx, 1, ctx->tables, s));
   }
+  DevBuf part;  // split-fold partial sums: (2^nu2 / 2) pairs x [m][h][row][N]
+  if (fast && t_gsw == 8) HIP_TRY(part.alloc((size_t)((1 << nu2) / 2) * 8 * N * 4));
   int num_per = 1 << nu2;
   for (int cur = 0; cur < nu2; cur++) {
     num_per >>= 1;
     const size_t lvl = (size_t)(nu2 - 1 - cur) * level_polys * N;
     if (fast && t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
-      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, 0, ctx->tables, ctx->variant[SP_STAGE_FOLD], s));
+      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, 0, ctx->tables, ctx->variant[SP_STAGE_FOLD],
+                           part.u32(), 0, s));
     else
       HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, fast ? nullptr : kn.u32() + lvl, 0,
                                 t_gsw, gadget_bits(t_gsw), ctx->tables, s));
@@ -562,6 +564,7 @@ struct sp_batch {
   uint32_t *bulk = nullptr;       // bulk-load staging: [cap][4N] queries then [cap][f_q] v_folding
   uint32_t *dig = nullptr;        // split-expansion digit scratch [cap][dig_q]
   size_t dig_q = 0;
+  uint32_t *part = nullptr;       // split-fold partial sums [cap][n2][C/2][m][h][row][N]
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
@@ -578,8 +581,8 @@ static void batch_free(sp_batch *b) {
   if (!b) return;
   for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
   for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
-  uint32_t *bufs[] = {b->query_ct,  b->v,   b->left,  b->right,    b->fkeys, b->fkeys_neg,
-                      b->acc, b->stage, b->mfma_out, b->bulk, b->dig};
+  uint32_t *bufs[] = {b->query_ct, b->v,     b->left,     b->right, b->fkeys, b->fkeys_neg,
+                      b->acc,      b->stage, b->mfma_out, b->bulk,  b->dig,   b->part};
   for (uint32_t *p : bufs)
     if (p) (void)hipFree(p);
   if (b->mfma_a) (void)hipFree(b->mfma_a);
@@ -629,6 +632,7 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
     b->dig_q = (size_t)expansion_max_work(db->nu1, db->nu2, t_gsw) * 8 * 512;
     A(&b->dig, cap * b->dig_q);
   }
+  if (t_gsw == 8) A(&b->part, cap * db->n2 * (size_t)(C / 2) * 8 * N);  // split-fold partial sums
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->io, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
@@ -849,7 +853,7 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
       const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
       if (fast && b->t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
         HIP_TRY(launch_fold8(b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off, b->f_q, ctx->tables,
-                             ctx->variant[SP_STAGE_FOLD], s));
+                             ctx->variant[SP_STAGE_FOLD], b->part, (size_t)(C / 2) * 8 * N, s));
       else
         HIP_TRY(launch_fold_level(fast, b->acc, (size_t)C * 4 * N, db->n2, n, num_per, b->fkeys + off,
                                   fast ? nullptr : b->fkeys_neg + off, b->f_q, b->t_gsw, gadget_bits(b->t_gsw),

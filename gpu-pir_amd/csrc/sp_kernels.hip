@@ -841,30 +841,30 @@ __device__ __forceinline__ void store_tile_p_full(uint32_t *__restrict__ tile, c
 // positions against the key.
 
 // Fold digit differences g^-1(X) - g^-1(Y) as signed bytes into dig
-// ([j][k][DIG_WORDS], 8192 words).
-template <int NT>
+// ([j][k - K0][DIG_WORDS] for digits K0 .. K0 + NK - 1 of both polys j).
+template <int NT, int K0 = 0, int NK = 8>
 __device__ __forceinline__ void fold_digits(const uint32_t *__restrict__ X, const uint32_t *__restrict__ Y,
                                             uint32_t *__restrict__ dig, int tid) {
   for (int it = tid; it < 2 * DIG_WORDS; it += NT) {
     const int j = it / DIG_WORDS, w = it % DIG_WORDS, l = w & 63, i4 = w >> 6;
     const uint32_t *xh = X + (size_t)j * 2 * N, *xl = xh + N;
     const uint32_t *yh = Y + (size_t)j * 2 * N, *yl = yh + N;
-    uint32_t packed[8];
+    uint32_t packed[NK];
 #pragma unroll
-    for (int k = 0; k < 8; k++) packed[k] = 0;
+    for (int k = 0; k < NK; k++) packed[k] = 0;
 #pragma unroll
     for (int b = 0; b < 4; b++) {
       const int z = 64 * (4 * i4 + b) + l;
       const uint64_t x = ((uint64_t)xh[z] << 32) | xl[z];
       const uint64_t y = ((uint64_t)yh[z] << 32) | yl[z];
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int d = (int)((x >> (7 * k)) & 127u) - (int)((y >> (7 * k)) & 127u);
+      for (int k = 0; k < NK; k++) {
+        const int d = (int)((x >> (7 * (K0 + k))) & 127u) - (int)((y >> (7 * (K0 + k))) & 127u);
         packed[k] |= ((uint32_t)d & 0xFFu) << (8 * b);
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) dig[(j * 8 + k) * DIG_WORDS + w] = packed[k];
+    for (int k = 0; k < NK; k++) dig[(j * NK + k) * DIG_WORDS + w] = packed[k];
   }
 }
 
@@ -917,6 +917,49 @@ __device__ __forceinline__ void fold_residue(const uint32_t (&dw)[16 / NW][8], u
     for (int e = 0; e < PPT; e++) sum[row][e] = redc64(acc[row][e], mc.q, mc.qinv_neg);
 }
 
+// Fold epilogue: tiles 2m + row hold the NTT-domain sums (P layout, < q) of
+// row `row` mod q_m; waves 0-3 take them out of the NTT domain, combine the
+// residues by CRT and add the result to Y (raw hi/lo planes) mod Q.
+__device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t *__restrict__ Y,
+                                          const uint32_t *__restrict__ tw_base, const ModConst &mc0,
+                                          const ModConst &mc1, int wave, int lane) {
+  uint32_t s[32];
+  if (wave < 4) {
+    const int m = wave >> 1;
+    const ModConst mc = m ? mc1 : mc0;
+    load_p(s, tiles[wave], lane);
+    wave_lds_sync();
+    ntt_inv_wave(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+#pragma unroll
+    for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  }
+  __syncthreads();
+  if (wave < 4) {
+    const int m = wave >> 1;
+    uint32_t *xm = tiles[wave];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) xm[ii * 64 + lane] = s[16 * (1 - m) + ii];
+  }
+  __syncthreads();
+  if (wave < 4) {
+    const int m = wave >> 1, row = wave & 1;
+    const uint32_t *xo = tiles[wave ^ 2];
+    uint32_t *yh = Y + (size_t)row * 2 * N, *yl = yh + N;
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+      const int z = 64 * (16 * m + ii) + lane;
+      const uint32_t own = s[16 * m + ii], oth = xo[ii * 64 + lane];
+      const uint32_t x0 = m ? oth : own, x1 = m ? own : oth;
+      uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
+      x = (x >= QQ) ? x - QQ : x;
+      x += ((uint64_t)yh[z] << 32) | yl[z];
+      x = (x >= QQ) ? x - QQ : x;
+      yh[z] = (uint32_t)(x >> 32);
+      yl[z] = (uint32_t)x;
+    }
+  }
+}
+
 // Fold level, server form C_neg = G - C:  acc[i] <- Y + from_ntt(C * NTT(g^-1 X - g^-1 Y)),
 // X = acc[num_per + i], Y = acc[i].  The 16 digit polynomials (2 polys x 8
 // digits, key column c = j + 2k) are extracted once into tiles 0-3; wave w
@@ -964,41 +1007,101 @@ void k_fold8(uint32_t *__restrict__ accs, size_t set_stride, int n_trials, int n
 #pragma unroll
       for (int e = 0; e < PPT; e++) tiles[2 * m + row][PPT * tid + e] = sum[m][row][e];
   __syncthreads();
-  uint32_t s[32];
-  if (wave < 4) {
-    const int m = wave >> 1;
-    const ModConst mc = m ? mc1 : mc0;
-    load_p(s, tiles[wave], lane);
-    wave_lds_sync();
-    ntt_inv_wave(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+  fold_tail(tiles, Y, tw_base, mc0, mc1, wave, lane);
+}
+
+// ---- split fold level: partial products + finish ----------------------------
+// k_fold_part: one 8-wave block per (pair i, trial, query, residue m, column
+// half h): digits 4h .. 4h + 3 of both polys (key columns c = 8h .. 8h + 7,
+// c = j + 2k), one transform per wave, multiply-accumulate of the 8 tiles,
+// and the Montgomery-reduced partial sums of both rows (NTT domain, layout P,
+// < q) to part[set][i][m][h][row][P] (set = query * n_trials + trial).
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_trials, int num_per,
+                 const uint32_t *__restrict__ cpos, size_t key_qstride, uint32_t *__restrict__ part,
+                 size_t part_set_stride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
+  constexpr int PPT = 4;
+  __shared__ uint32_t tiles[8][LDS_TILE];
+  uint32_t *dig = &tiles[0][0];  // 8 digit polynomials, 4096 words over tiles 0-1
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int i = blockIdx.x;
+  const int trial = blockIdx.y >> 2, m = (blockIdx.y >> 1) & 1, h = blockIdx.y & 1;
+  const int set = blockIdx.z * n_trials + trial;
+  const uint32_t *base = accs + (size_t)set * set_stride;
+  const uint32_t *X = base + (size_t)(num_per + i) * 4 * N;
+  const uint32_t *Y = base + (size_t)i * 4 * N;
+  const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
+  if (h == 0)
+    fold_digits<512, 0, 4>(X, Y, dig, tid);
+  else
+    fold_digits<512, 4, 4>(X, Y, dig, tid);
+  __syncthreads();
+  const int c = 8 * h + wave, j = c & 1, kl = (c >> 1) - 4 * h;
+  uint32_t dw[8];
 #pragma unroll
-    for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  for (int i4 = 0; i4 < 8; i4++) dw[i4] = dig[(j * 4 + kl) * DIG_WORDS + i4 * 64 + lane];
+  __syncthreads();  // the digit bytes are dead from here on
+  const uint32_t q = m ? mc1.q : mc0.q, mu1 = m ? mc1.mu1 : mc0.mu1, qinv = m ? mc1.qinv_neg : mc0.qinv_neg;
+  {
+    uint32_t r[32];
+#pragma unroll
+    for (int i4 = 0; i4 < 8; i4++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) r[4 * i4 + b] = (uint32_t)((int32_t)(int8_t)(dw[i4] >> (8 * b)) + (int32_t)q);
+    // inputs in (q - 128, q + 128) < 2q: stages 1-7 and 9-11 skip the reduction (outputs < 10q)
+    ntt_fwd_wave_t<7, true>(r, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
+    store_tile_p_full(tiles[wave], r, lane, q, mu1);
   }
   __syncthreads();
-  if (wave < 4) {
-    const int m = wave >> 1;
-    uint32_t *xm = tiles[wave];
+  uint64_t acc[2][PPT];
 #pragma unroll
-    for (int ii = 0; ii < 16; ii++) xm[ii * 64 + lane] = s[16 * (1 - m) + ii];
-  }
-  __syncthreads();
-  if (wave < 4) {
-    const int m = wave >> 1, row = wave & 1;
-    const uint32_t *xo = tiles[wave ^ 2];
-    uint32_t *yh = Y + (size_t)row * 2 * N, *yl = yh + N;
+  for (int row = 0; row < 2; row++)
 #pragma unroll
-    for (int ii = 0; ii < 16; ii++) {
-      const int z = 64 * (16 * m + ii) + lane;
-      const uint32_t own = s[16 * m + ii], oth = xo[ii * 64 + lane];
-      const uint32_t x0 = m ? oth : own, x1 = m ? own : oth;
-      uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
-      x = (x >= QQ) ? x - QQ : x;
-      x += ((uint64_t)yh[z] << 32) | yl[z];
-      x = (x >= QQ) ? x - QQ : x;
-      yh[z] = (uint32_t)(x >> 32);
-      yl[z] = (uint32_t)x;
+    for (int e = 0; e < PPT; e++) acc[row][e] = 0;
+#pragma unroll
+  for (int w2 = 0; w2 < 8; w2++) {
+    const uint32_t *k0 = kp + ((size_t)(m * 16 + 8 * h + w2) * 2) * N + PPT * tid;
+    const uint32_t *k1 = k0 + N;
+    const uint32_t *tl = &tiles[w2][PPT * tid];
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      const uint32_t d = tl[e];
+      acc[0][e] += (uint64_t)d * k0[e];
+      acc[1][e] += (uint64_t)d * k1[e];
     }
   }
+  uint32_t *dst = part + (size_t)set * part_set_stride + ((((size_t)i * 2 + m) * 2 + h) * 2) * N + PPT * tid;
+#pragma unroll
+  for (int row = 0; row < 2; row++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) dst[(size_t)row * N + e] = redc64(acc[row][e], q, qinv);
+}
+
+// k_fold_finish: one 4-wave block per (pair i, trial, query): tile 2m + row =
+// part[m][0][row] + part[m][1][row] mod q_m, then the fold epilogue on Y.
+__global__ __launch_bounds__(256) void k_fold_finish(uint32_t *__restrict__ accs, size_t set_stride, int n_trials,
+                                                     const uint32_t *__restrict__ part, size_t part_set_stride,
+                                                     const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                     ModConst mc1) {
+  __shared__ uint32_t tiles[4][LDS_TILE];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int i = blockIdx.x;
+  const int set = blockIdx.z * n_trials + blockIdx.y;
+  uint32_t *Y = accs + (size_t)set * set_stride + (size_t)i * 4 * N;
+  const uint32_t *pp = part + (size_t)set * part_set_stride + (size_t)i * 8 * N;  // [m][h][row][P]
+  for (int u = tid; u < N; u += 256) {  // 4 tiles x N/4 uint4
+    const int w = u / (N / 4), k4 = u % (N / 4);
+    const int m = w >> 1, row = w & 1;
+    const uint32_t q = m ? mc1.q : mc0.q;
+    const uint4 a = reinterpret_cast<const uint4 *>(pp + (size_t)((m * 2 + 0) * 2 + row) * N)[k4];
+    const uint4 b = reinterpret_cast<const uint4 *>(pp + (size_t)((m * 2 + 1) * 2 + row) * N)[k4];
+    reinterpret_cast<uint4 *>(tiles[w])[k4] =
+        make_uint4(canon(a.x + b.x, q), canon(a.y + b.y, q), canon(a.z + b.z, q), canon(a.w + b.w, q));
+  }
+  __syncthreads();
+  fold_tail(tiles, Y, tw_base, mc0, mc1, wave, lane);
 }
 
 // Expansion phases 1-2 for element v_i: waves 0/1 take v_i.a out of the NTT
@@ -1303,9 +1406,24 @@ static int fold_expand_waves(long blocks, const DevTables &t, int variant, bool 
 }
 
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
-                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, hipStream_t s) {
+                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, uint32_t *part,
+                        size_t part_set_stride, hipStream_t s) {
   dim3 grid(num_per, n_trials, n_queries);
-  if (fold_expand_waves((long)num_per * n_trials * n_queries, t, variant, true) == 16)
+  const long pairs = (long)num_per * n_trials * n_queries;
+  // Split form: four partial blocks per pair, then a finish launch; the
+  // automatic choice whenever the scratch is there (measured at (9,6): 0.16
+  // vs 0.26 ms for one query, 2.75 vs 3.17 ms for 64 queries against the
+  // 16-wave blocks).
+  if (part && (variant == 4 || variant == 0)) {
+    k_fold_part<<<dim3(num_per, n_trials * 4, n_queries), 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos,
+                                                                        key_qstride, part, part_set_stride, t.tw,
+                                                                        t.mc[0], t.mc[1]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_fold_finish<<<grid, 256, 0, s>>>(accs, set_stride, n_trials, part, part_set_stride, t.tw, t.mc[0], t.mc[1]);
+    return hipGetLastError();
+  }
+  if (fold_expand_waves(pairs, t, variant, true) == 16)
     k_fold8<16><<<grid, 1024, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0],
                                       t.mc[1]);
   else

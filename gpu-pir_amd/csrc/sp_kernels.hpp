@@ -47,9 +47,12 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
 hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s);
 
 // Gadget 2^7 x 8 (t_gsw = t_exp = 8) fold and expansion kernels with digit-owning waves.
-// variant: 0 = by grid size, 2 = 8-wave blocks, 3 = 16-wave blocks.
+// variant: 0 = by grid size, 2 = 8-wave blocks, 3 = 16-wave blocks, 4 = split
+// (fold only: partial blocks per residue and column half, then a finish
+// launch; part: scratch of part_set_stride >= num_per * 8 * N words per set).
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
-                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, hipStream_t s);
+                        const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, uint32_t *part,
+                        size_t part_set_stride, hipStream_t s);
 hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                           const DevTables &t, int variant, hipStream_t s);

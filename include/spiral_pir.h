@@ -61,9 +61,10 @@ int sp_ctx_synchronize(sp_ctx *ctx);
 /* Kernel variant per stage, for tests and the autotuner; 0 = automatic.
  *   SP_STAGE_EXPAND, SP_STAGE_FOLD: 1 = one block of 4 waves per item (any
  *     gadget); for gadget 2^7 x 8 only: 2 = 8-wave blocks (two per CU),
- *     3 = 16-wave blocks (one per CU, half the passes); expansion only:
- *     4 = each round in two launches (digits; per-residue transforms and
- *     update).  Automatic: 3 for the fold, 4 for the expansion
+ *     3 = 16-wave blocks (one per CU, half the passes), 4 = split in two
+ *     launches (expansion: digits, then per-residue transforms and update;
+ *     fold: partial products per residue and key-column half, then finish).
+ *     Automatic: 4 for both
  *   SP_STAGE_DB_MUL: 1 = VALU u64 multiply-accumulate, 2 = int8-limb MFMA */
 #define SP_STAGE_EXPAND 0
 #define SP_STAGE_FOLD 1

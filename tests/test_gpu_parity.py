@@ -194,8 +194,8 @@ def variant(gpu_ctx, request):
     gpu_ctx.set_variant(stage, 0)
 
 
-@pytest.mark.parametrize("variant", [(1, 1), (1, 2), (1, 3)], indirect=True,
-                         ids=["fold-4wave", "fold-8wave", "fold-16wave"])
+@pytest.mark.parametrize("variant", [(1, 1), (1, 2), (1, 3), (1, 4)], indirect=True,
+                         ids=["fold-4wave", "fold-8wave", "fold-16wave", "fold-split"])
 @pytest.mark.parametrize("nu2", [1, 2, 4])
 @pytest.mark.parametrize("general", [False, True])
 def test_fold(gpu_ctx, oracle, nu2, general, variant):
