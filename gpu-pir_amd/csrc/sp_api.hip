@@ -345,6 +345,30 @@ extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, cons
   return SP_OK;
 }
 
+extern "C" int sp_regev_to_gsw(sp_ctx *ctx, const uint32_t *dev_v, size_t n_ct, int nu2, int t_gsw, int t_conv,
+                               const uint32_t *dev_conv, uint32_t *dev_v_folding, void *stream) {
+  if (!ctx || !dev_v || !dev_conv || !dev_v_folding) return fail(SP_ERR_ARG, "sp_regev_to_gsw: NULL");
+  if (nu2 < 1 || nu2 > 12 || t_gsw < 2 || t_gsw > 28)
+    return fail(SP_ERR_ARG, "sp_regev_to_gsw: nu2 %d, t_gsw %d", nu2, t_gsw);
+  if (t_conv != SP_T_CONV) return fail(SP_ERR_ARG, "sp_regev_to_gsw: t_conv %d unsupported (%d)", t_conv, SP_T_CONV);
+  const size_t need = (size_t)2 * nu2 * t_gsw;
+  if (n_ct < need) return fail(SP_ERR_ARG, "sp_regev_to_gsw: %zu ciphertexts, %zu needed", n_ct, need);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const size_t conv_polys = (size_t)2 * (2 * t_conv) * 2;
+  const size_t fold_polys = (size_t)nu2 * 2 * (2 * t_gsw) * 2;
+  DevBuf v, kc, out;
+  HIP_TRY(v.alloc(need * 4 * N * 4));
+  HIP_TRY(kc.alloc(conv_polys * N * 4));
+  HIP_TRY(out.alloc(fold_polys * N * 4));
+  HIP_TRY(launch_nat_to_p(dev_v, v.u32(), need * 4, 1, 0, ctx->tables, s));
+  HIP_TRY(launch_nat_to_p(dev_conv, kc.u32(), conv_polys, 2 * (2 * t_conv), 1, ctx->tables, s));
+  HIP_TRY(launch_regev_to_gsw(false, v.u32(), 0, 1, nu2, t_gsw, kc.u32(), 0, out.u32(), 0, nullptr, ctx->tables, s));
+  HIP_TRY(launch_p_to_nat(out.u32(), dev_v_folding, fold_polys, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SP_OK;
+}
+
 static int ceil_log2(uint64_t x) {
   int v = 0;
   uint64_t p = 1;
@@ -565,6 +589,10 @@ struct sp_batch {
   uint32_t *dig = nullptr;        // split-expansion digit scratch [cap][dig_q]
   size_t dig_q = 0;
   uint32_t *part = nullptr;       // split-fold partial sums [cap][n2][C/2][m][h][row][N]
+  uint32_t *conv = nullptr;       // conversion keys of packed slots [cap][m][8][row][P], 2^32-scaled
+  size_t conv_q = 0;
+  std::vector<int> packed;        // per slot: 1 = packed query (v_folding converted on the device)
+  int *d_packed = nullptr;        // device copy of `packed`
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
@@ -581,11 +609,12 @@ static void batch_free(sp_batch *b) {
   if (!b) return;
   for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
   for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
-  uint32_t *bufs[] = {b->query_ct, b->v,     b->left,     b->right, b->fkeys, b->fkeys_neg,
-                      b->acc,      b->stage, b->mfma_out, b->bulk,  b->dig,   b->part};
+  uint32_t *bufs[] = {b->query_ct, b->v,        b->left, b->right, b->fkeys, b->fkeys_neg, b->acc,
+                      b->stage,    b->mfma_out, b->bulk, b->dig,   b->part,  b->conv};
   for (uint32_t *p : bufs)
     if (p) (void)hipFree(p);
   if (b->mfma_a) (void)hipFree(b->mfma_a);
+  if (b->d_packed) (void)hipFree(b->d_packed);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->io) (void)hipStreamDestroy(b->io);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -615,6 +644,8 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   b->r_q = (size_t)(stop + 1) * kl;
   b->f_q = (size_t)db->nu2 * kf;
   b->acc_q = (size_t)db->n2 * C * 4 * N;
+  b->conv_q = (size_t)2 * (2 * SP_T_CONV) * 2 * N;
+  b->packed.assign((size_t)capacity, 0);
   b->stage_words = std::max<size_t>({b->l_q, b->r_q, b->f_q, b->acc_q, b->v_q});
   const size_t cap = (size_t)capacity;
   hipError_t e = hipSuccess;
@@ -628,6 +659,8 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   A(&b->fkeys, cap * b->f_q);
   A(&b->acc, cap * b->acc_q);
   A(&b->stage, b->stage_words);
+  if (e == hipSuccess) e = hipMalloc(&b->d_packed, cap * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(b->d_packed, 0, cap * sizeof(int));
   if (t_exp == 8) {  // digit scratch of the split expansion
     b->dig_q = (size_t)expansion_max_work(db->nu1, db->nu2, t_gsw) * 8 * 512;
     A(&b->dig, cap * b->dig_q);
@@ -662,6 +695,16 @@ static int upload_p(sp_batch *b, const uint32_t *host, uint32_t *dst, size_t n_p
   return SP_OK;
 }
 
+// Records the form of slots [first, first + n) (1 packed, 0 with fold vectors)
+// on the host and, for the conversion kernel, on the device (io stream).
+static int set_slot_form(sp_batch *b, int first, int n, int packed) {
+  for (int i = 0; i < n; i++) b->packed[(size_t)first + i] = packed;
+  HIP_TRY(hipMemcpyAsync(b->d_packed + first, b->packed.data() + first, (size_t)n * sizeof(int),
+                         hipMemcpyHostToDevice, b->io));
+  HIP_TRY(hipStreamSynchronize(b->io));
+  return SP_OK;
+}
+
 static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const uint32_t *host_left,
                       const uint32_t *host_right, const uint32_t *host_v_folding, const uint32_t *host_v_folding_neg) {
   if (!b || !host_query || !host_left || !host_right || !host_v_folding) return fail(SP_ERR_ARG, "load: NULL");
@@ -686,6 +729,7 @@ static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const u
                        4 * b->t_gsw, 1)))
       return rc;
   }
+  if ((rc = set_slot_form(b, slot, 1, 0))) return rc;
   b->fold_mode = mode;
   return SP_OK;
 }
@@ -701,6 +745,7 @@ struct sp_keys {
   int t_gsw = 0, t_exp = 0;
   size_t l_words = 0, r_words = 0;
   uint32_t *left = nullptr, *right = nullptr;  // layout P, 2^32-scaled: as a batch slot's keys
+  uint32_t *conv = nullptr;                    // conversion key [m][2 t_conv][row][P], 2^32-scaled
 };
 
 extern "C" int sp_keys_create(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_left, const uint32_t *host_right,
@@ -745,7 +790,30 @@ extern "C" int sp_keys_destroy(sp_keys *k) {
   (void)hipDeviceSynchronize();
   if (k->left) (void)hipFree(k->left);
   if (k->right) (void)hipFree(k->right);
+  if (k->conv) (void)hipFree(k->conv);
   delete k;
+  return SP_OK;
+}
+
+extern "C" int sp_keys_set_conversion(sp_keys *k, int t_conv, const uint32_t *host_conv) {
+  if (!k || !host_conv) return fail(SP_ERR_ARG, "sp_keys_set_conversion: NULL");
+  if (t_conv != SP_T_CONV) return fail(SP_ERR_ARG, "sp_keys_set_conversion: t_conv %d unsupported (%d)", t_conv, SP_T_CONV);
+  sp_ctx *ctx = k->db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t polys = (size_t)2 * (2 * t_conv) * 2;
+  if (!k->conv) {
+    hipError_t e = hipMalloc(&k->conv, polys * N * 4);
+    if (e != hipSuccess) {
+      k->conv = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_keys_set_conversion: %s",
+                  hipGetErrorString(e));
+    }
+  }
+  DevBuf stage;
+  HIP_TRY(stage.alloc(polys * N * 4));
+  HIP_TRY(hipMemcpyAsync(stage.p, host_conv, polys * N * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(launch_nat_to_p(stage.u32(), k->conv, polys, 2 * (2 * t_conv), 1, ctx->tables, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return SP_OK;
 }
 
@@ -789,8 +857,56 @@ extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t 
     HIP_TRY(hipMemcpyAsync(b->right + (f0 + i) * b->r_q, keys[i]->right, b->r_q * 4, hipMemcpyDeviceToDevice, b->io));
   }
   HIP_TRY(hipStreamSynchronize(b->io));
+  int rc = set_slot_form(b, first, n, 0);
+  if (rc) return rc;
   b->fold_mode = 1;
   return SP_OK;
+}
+
+extern "C" int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                                    sp_keys *const *keys) {
+  if (!b || !host_queries || !keys) return fail(SP_ERR_ARG, "sp_batch_load_packed: NULL");
+  if (n < 1 || first < 0 || first + n > b->cap)
+    return fail(SP_ERR_ARG, "sp_batch_load_packed: slots [%d, %d) of %d", first, first + n, b->cap);
+  for (int i = 0; i < n; i++) {
+    const sp_keys *k = keys[i];
+    if (!host_queries[i]) return fail(SP_ERR_ARG, "sp_batch_load_packed: query %d NULL", i);
+    if (!k || k->db != b->db || k->t_exp != b->t_exp || k->t_gsw != b->t_gsw || k->l_words != b->l_q ||
+        k->r_words != b->r_q)
+      return fail(SP_ERR_ARG, "sp_batch_load_packed: keys %d do not match this batch's database and gadget", i);
+    if (!k->conv) return fail(SP_ERR_ARG, "sp_batch_load_packed: keys %d have no conversion key", i);
+  }
+  sp_ctx *ctx = b->db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));  // the previous run must be done with the slots
+  const size_t qw = (size_t)4 * N;
+  hipError_t e = hipSuccess;
+  if (!b->bulk) e = hipMalloc(&b->bulk, (size_t)b->cap * (qw + b->f_q) * 4);
+  if (e == hipSuccess && !b->conv) e = hipMalloc(&b->conv, (size_t)b->cap * b->conv_q * 4);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "packed-load buffers: %s", hipGetErrorString(e));
+  const size_t f0 = (size_t)first;
+  for (int i = 0; i < n; i++)
+    HIP_TRY(hipMemcpyAsync(b->bulk + (size_t)i * qw, host_queries[i], qw * 4, hipMemcpyHostToDevice, b->io));
+  HIP_TRY(launch_nat_to_p(b->bulk, b->query_ct + f0 * qw, (size_t)n * 4, 1, 0, ctx->tables, b->io));
+  for (int i = 0; i < n; i++) {
+    HIP_TRY(hipMemcpyAsync(b->left + (f0 + i) * b->l_q, keys[i]->left, b->l_q * 4, hipMemcpyDeviceToDevice, b->io));
+    HIP_TRY(hipMemcpyAsync(b->right + (f0 + i) * b->r_q, keys[i]->right, b->r_q * 4, hipMemcpyDeviceToDevice, b->io));
+    HIP_TRY(hipMemcpyAsync(b->conv + (f0 + i) * b->conv_q, keys[i]->conv, b->conv_q * 4, hipMemcpyDeviceToDevice,
+                           b->io));
+  }
+  HIP_TRY(hipStreamSynchronize(b->io));
+  int rc = set_slot_form(b, first, n, 1);
+  if (rc) return rc;
+  b->fold_mode = 1;
+  return SP_OK;
+}
+
+// Whether any of slots [0, n) holds a packed query (the run then converts).
+static int slots_packed(const sp_batch *b, int n) {
+  for (int i = 0; i < n; i++)
+    if (b->packed[i]) return 1;
+  return 0;
 }
 
 static int mfma_workspace(sp_batch *b) {
@@ -823,11 +939,15 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
   sp_db *db = b->db;
   sp_ctx *ctx = db->ctx;
   const int R = 1 << db->nu1, C = 1 << db->nu2;
+  const int packed = slots_packed(b, n);
   if (stages & 1) {
     HIP_TRY(launch_copy_queries(b->query_ct, 4 * N, b->v, b->v_q, n, s));
     int rc = run_expansion(ctx, b->v, b->v_q, n, db->nu1, db->nu2, b->t_gsw, b->t_exp, b->left, b->right, b->l_q,
                            b->r_q, -1, b->dig, b->dig_q, s);
     if (rc) return rc;
+    if (packed)
+      HIP_TRY(launch_regev_to_gsw(true, b->v, b->v_q, n, db->nu2, b->t_gsw, b->conv, b->conv_q, b->fkeys, b->f_q,
+                                  b->d_packed, ctx->tables, s));
   }
   if (stages & 2) {
     const int var = dbmul_variant(b, n);
@@ -880,10 +1000,12 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   if (!b) return fail(SP_ERR_ARG, "sp_batch_graph_launch: NULL");
   if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_graph_launch: n = %d, capacity %d", n, b->cap);
   if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_graph_launch: nothing loaded");
+  const int packed = slots_packed(b, n);
+  const int gkey = n | (packed << 20);  // packed slots add the conversion to the graph
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   hipGraphExec_t exec = nullptr;
   for (auto &ge : b->graphs)
-    if (ge.first == n) exec = ge.second;
+    if (ge.first == gkey) exec = ge.second;
   if (!exec) {
     // One-time preparations stay out of the graph.
     if (dbmul_variant(b, n) == 2) {
@@ -906,7 +1028,7 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
       (void)hipGraphDestroy(graph);
       return fail(SP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    b->graphs.push_back({n, exec});
+    b->graphs.push_back({gkey, exec});
     b->graph_defs.push_back(graph);
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -983,6 +1105,8 @@ extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
     if (key[i] > 0 && g->b[i]->fold_mode < 0) return fail(SP_ERR_STATE, "sp_group_launch: batch %d not loaded", i);
     active += key[i] > 0;
   }
+  for (int i = 0; i < count; i++)  // the graph also depends on which batches convert packed queries
+    key.push_back(key[i] > 0 ? slots_packed(g->b[i], key[i]) : 0);
   if (!active) return SP_OK;
   HIP_TRY(hipSetDevice(g->b[0]->db->ctx->device));
   hipGraphExec_t exec = nullptr;
@@ -1112,6 +1236,11 @@ static int batch_read(sp_batch *b, int slot, int what, uint32_t *host_out) {
     HIP_TRY(hipMemcpyAsync(host_out, b->stage, n * N * 4, hipMemcpyDeviceToHost, b->io));
   } else if (what == 3) {
     HIP_TRY(hipMemcpyAsync(host_out, acc, b->acc_q * 4, hipMemcpyDeviceToHost, b->io));
+  } else if (what == 4) {
+    const size_t n = b->f_q / N;
+    HIP_TRY(launch_p_to_nat_unscale(b->fkeys + (size_t)slot * b->f_q, b->stage, n, 4 * b->t_gsw, db->ctx->tables,
+                                    b->io));
+    HIP_TRY(hipMemcpyAsync(host_out, b->stage, b->f_q * 4, hipMemcpyDeviceToHost, b->io));
   } else {
     return fail(SP_ERR_ARG, "read: what = %d", what);
   }
@@ -1184,6 +1313,11 @@ extern "C" int sp_query_load(sp_query *q, const uint32_t *host_query, const uint
                              const uint32_t *host_v_folding_neg) {
   if (!q) return fail(SP_ERR_ARG, "sp_query_load: NULL");
   return batch_load(q->b, 0, host_query, host_left, host_right, host_v_folding, host_v_folding_neg);
+}
+
+extern "C" int sp_query_load_packed(sp_query *q, const uint32_t *host_query, sp_keys *keys) {
+  if (!q || !host_query || !keys) return fail(SP_ERR_ARG, "sp_query_load_packed: NULL");
+  return sp_batch_load_packed(q->b, 0, 1, &host_query, &keys);
 }
 
 extern "C" int sp_query_run(sp_query *q, int stages, void *stream) {

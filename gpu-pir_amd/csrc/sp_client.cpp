@@ -385,6 +385,97 @@ extern "C" int sp_client_query(sp_client *c, uint64_t index, uint32_t *host_quer
   return SP_OK;
 }
 
+// Conversion key (Spiral's RegevToGSW key; the reference declares it as
+// PublicParameters.conversion, gpu-pir/structures.cuh:173-177, 2 x 2 t_conv):
+// CiphertextHalf[2][2 t_conv] = u32 [m][col][row][N], standard form.  With
+// bits = ceil(56 / t_conv), column 2k encrypts s^2 * 2^(bits k) and column
+// 2k + 1 encrypts -s * 2^(bits k), so that for a ciphertext (a, b) of mu,
+// key * g^-1(a, b) (rdim 2, digit k of poly j at row j + 2k) encrypts
+// s^2 a - s b = -s mu - s e: the row-0 GSW column of mu.
+extern "C" int sp_client_conversion_key(sp_client *c, int t_conv, uint32_t *host_conv) {
+  if (!c || !host_conv) return cfail(SP_ERR_ARG, "sp_client_conversion_key: NULL");
+  if (t_conv < 2 || t_conv > 28) return cfail(SP_ERR_ARG, "sp_client_conversion_key: t_conv out of range");
+  const int bits = gadget_bits(t_conv);
+  const int mx = 2 * t_conv;
+  std::vector<uint32_t> ct(4 * N), mu(2 * N);
+  for (int col = 0; col < mx; col++) {
+    const int k = col >> 1;
+    for (int m = 0; m < 2; m++) {
+      const uint64_t q = QM[m];
+      const uint64_t gk = powmod(2, (uint64_t)bits * k, q);
+      for (int z = 0; z < N; z++) {
+        const uint64_t s = c->s_ntt[m * N + z];
+        const uint64_t v = (col & 1) ? (q - s) % q : s * s % q;
+        mu[m * N + z] = (uint32_t)(v * gk % q);
+      }
+    }
+    c->encrypt(ct.data(), mu.data());
+    for (int m = 0; m < 2; m++) {
+      uint32_t *o = host_conv + ((size_t)m * mx + col) * 2 * N;
+      memcpy(o, ct.data() + m * N, N * 4);
+      memcpy(o + N, ct.data() + (2 + m) * N, N * 4);
+    }
+  }
+  return SP_OK;
+}
+
+// Packed query for record `index` (= i1 * 2^nu2 + i2): one Regev ciphertext
+// (NTT Ciphertext) carrying both the first-dimension selector and the bits of
+// i2 for the server-side Regev -> GSW conversion.  Plaintext mu:
+//   coefficient 2 i1               = floor(Q/p) * 2^-g                 mod Q
+//   coefficient 2 (l t_gsw + j) + 1 = bit l of i2 * 2^(bits j) * 2^-(stop+1) mod Q
+// Expansion doubles a coefficient in every round it passes: the even slots
+// pass all g rounds, the odd (GSW) slots rounds 0 .. stop_round.
+extern "C" int sp_client_query_packed(sp_client *c, uint64_t index, uint32_t *host_query) {
+  if (!c || !host_query) return cfail(SP_ERR_ARG, "sp_client_query_packed: NULL");
+  if (index >> (c->nu1 + c->nu2)) return cfail(SP_ERR_ARG, "sp_client_query_packed: index out of range");
+  if ((uint64_t)c->t_gsw * c->nu2 > ((uint64_t)1 << c->stop))
+    return cfail(SP_ERR_ARG, "sp_client_query_packed: GSW bits do not fit the odd slots");
+  const uint64_t i1 = index >> c->nu2, i2 = index & ((1ull << c->nu2) - 1);
+  const uint64_t inv2 = (QQ + 1) / 2;
+  std::vector<uint64_t> mu(N, 0);
+  mu[2 * i1] = (uint64_t)((u128)(QQ / 256) * powmod(inv2, c->g, QQ) % QQ);
+  const int bits = gadget_bits(c->t_gsw);
+  const uint64_t inv_odd = powmod(inv2, c->stop + 1, QQ);
+  for (int l = 0; l < c->nu2; l++) {
+    if (!((i2 >> l) & 1)) continue;
+    for (int j = 0; j < c->t_gsw; j++) {
+      const uint64_t gj = powmod(2, (uint64_t)bits * j, QQ);
+      mu[2 * (l * c->t_gsw + j) + 1] = (uint64_t)((u128)gj * inv_odd % QQ);
+    }
+  }
+  std::vector<uint32_t> mu_ntt(2 * N);
+  raw_to_ntt(mu.data(), mu_ntt.data());
+  c->encrypt(host_query, mu_ntt.data());
+  return SP_OK;
+}
+
+// Phase b - a*s mod Q (u64 [N], coefficient order) of one NTT Ciphertext
+// (natural layout [row][m][N], standard form): the plaintext plus the noise.
+extern "C" int sp_client_phase(const sp_client *c, const uint32_t *host_ct, uint64_t *host_phase) {
+  if (!c || !host_ct || !host_phase) return cfail(SP_ERR_ARG, "sp_client_phase: NULL");
+  std::vector<uint32_t> d(2 * N);
+  for (int m = 0; m < 2; m++) {
+    const uint32_t q = QM[m];
+    for (int k = 0; k < N; k++) {
+      const uint32_t a = host_ct[m * N + k], b = host_ct[(2 + m) * N + k];
+      if (a >= q || b >= q) return cfail(SP_ERR_ARG, "sp_client_phase: residue out of range");
+      const uint64_t as = (uint64_t)a * c->s_ntt[m * N + k] % q;
+      d[m * N + k] = (uint32_t)((b + q - as) % q);
+    }
+    ntt().inv(d.data() + m * N, m);
+  }
+  for (int z = 0; z < N; z++) host_phase[z] = crt(d[z], d[N + z]);
+  return SP_OK;
+}
+
+// The ternary secret s (int32 [N], coefficient order), for noise measurements.
+extern "C" int sp_client_secret(const sp_client *c, int32_t *host_s) {
+  if (!c || !host_s) return cfail(SP_ERR_ARG, "sp_client_secret: NULL");
+  memcpy(host_s, c->s.data(), N * sizeof(int32_t));
+  return SP_OK;
+}
+
 // Decode n2 raw response ciphertexts into the record's n2 * 2048 bytes.
 extern "C" int sp_client_decode(sp_client *c, const uint32_t *host_response, uint8_t *host_record) {
   if (!c || !host_response || !host_record) return cfail(SP_ERR_ARG, "sp_client_decode: NULL");

@@ -229,6 +229,19 @@ __global__ __launch_bounds__(256) void k_p_to_nat(const uint32_t *__restrict__ i
   }
 }
 
+// P -> natural for 2^32-scaled residues: out = in * 2^-32 mod q (canonical).
+__global__ __launch_bounds__(256) void k_p_to_nat_unscale(const uint32_t *__restrict__ in,
+                                                          uint32_t *__restrict__ out, size_t n, int m_div,
+                                                          ModConst mc0, ModConst mc1) {
+  const size_t total = n * N;
+  for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
+    const size_t k = g / N;
+    const int z = (int)(g % N);
+    const ModConst &mc = ((k / m_div) & 1) ? mc1 : mc0;
+    out[g] = redc64((uint64_t)in[k * N + p_of_z(z)], mc.q, mc.qinv_neg);
+  }
+}
+
 // Reference Database layout [c][j][m][z] -> internal [m][j][c][p] (one trial).
 __global__ __launch_bounds__(256) void k_db_from_ref(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                                      int R, int C) {
@@ -708,6 +721,12 @@ hipError_t launch_p_to_nat(const uint32_t *in, uint32_t *out, size_t n, hipStrea
   return hipGetLastError();
 }
 
+hipError_t launch_p_to_nat_unscale(const uint32_t *in, uint32_t *out, size_t n, int m_div, const DevTables &t,
+                                   hipStream_t s) {
+  k_p_to_nat_unscale<<<grid_for(n * N, 256), 256, 0, s>>>(in, out, n, m_div, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
 hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s) {
   k_db_from_ref<<<grid_for((size_t)2 * R * C * N, 256), 256, 0, s>>>(in, out, R, C);
   return hipGetLastError();
@@ -1102,6 +1121,139 @@ __global__ __launch_bounds__(256) void k_fold_finish(uint32_t *__restrict__ accs
   }
   __syncthreads();
   fold_tail(tiles, Y, tw_base, mc0, mc1, wave, lane);
+}
+
+// ---- Regev -> GSW conversion ------------------------------------------------
+// Spiral's RegevToGSW (the reference declares the conversion key and the
+// external-product shape "2 by 2t_conv" x "2 by 1 -> 2t_conv by 1",
+// gpu-pir/structures.cuh:173-177, gadget.cuh:41-53, and leaves the function a
+// stub, cpu_version/conversion.py:3).  Block (k, query) converts GSW slot
+// k = l t_gsw + j: ct = v[2k + 1] (NTT form, layout P, canonical).
+//   column 2j + 1 of level l = ct
+//   column 2j     of level l = conv * NTT(g^-1(ct)), rdim 2, 4 digits of 14
+//                              bits, key column c = row + 2 d
+// Waves 0-3 take the four (row, residue) polynomials of ct out of the NTT
+// domain; wave c then owns digit polynomial c, transforms it in each residue
+// into its tile, and every thread multiply-accumulates the 8 tiles at its 4
+// positions against the key (layout [m][c][row][P], 2^32-scaled).
+// Digits are below 2^14 and 16 q + 2^14 < 2^32 for both moduli, so the first
+// 8 forward stages skip the reduction (outputs < 8q, reduced when stored).
+// MONT: outputs 2^32-scaled (the fold's key form), else standard form.
+// only (optional): per-query flags; queries whose flag is 0 are left alone.
+constexpr int CONV_COLS = 8;  // 2 t_conv, t_conv = 4
+
+template <bool MONT, int M>
+__device__ __forceinline__ void conv_residue(const uint32_t (&dp)[16], uint32_t (*tiles)[LDS_TILE],
+                                             const uint32_t *__restrict__ kc, uint32_t *__restrict__ out_col,
+                                             const uint32_t *__restrict__ tw_base, const ModConst &mc, int wave,
+                                             int lane, int tid) {
+  constexpr int PPT = N / 512;
+  uint32_t r[32];
+#pragma unroll
+  for (int i = 0; i < 32; i++) r[i] = (dp[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+  ntt_fwd_wave_t<8, true>(r, tw_base + (M * 2) * TW_WORDS, tiles[wave], mc.q, mc.mu1, lane);
+  store_tile_p_full(tiles[wave], r, lane, mc.q, mc.mu1);
+  __syncthreads();
+  uint64_t acc[2][PPT];
+#pragma unroll
+  for (int row = 0; row < 2; row++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) acc[row][e] = 0;
+#pragma unroll
+  for (int c = 0; c < CONV_COLS; c++) {
+    const uint32_t *k0 = kc + ((size_t)(M * CONV_COLS + c) * 2) * N + PPT * tid;
+    const uint32_t *k1 = k0 + N;
+    const uint32_t *tl = &tiles[c][PPT * tid];
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      const uint32_t d = tl[e];
+      acc[0][e] += (uint64_t)d * k0[e];
+      acc[1][e] += (uint64_t)d * k1[e];
+    }
+  }
+#pragma unroll
+  for (int row = 0; row < 2; row++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      uint32_t x = redc64(acc[row][e], mc.q, mc.qinv_neg);
+      if (MONT) x = canon(shoup_mul(x, mc.two32, mc.two32_p, mc.q), mc.q);
+      out_col[(size_t)row * N + PPT * tid + e] = x;
+    }
+  __syncthreads();
+}
+
+template <bool MONT>
+__global__ __launch_bounds__(512) void k_regev_to_gsw(const uint32_t *__restrict__ v, size_t v_q, int t_gsw,
+                                                      const uint32_t *__restrict__ conv, size_t conv_q,
+                                                      uint32_t *__restrict__ fk, size_t fk_q,
+                                                      const int *__restrict__ only,
+                                                      const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                      ModConst mc1) {
+  if (only && !only[blockIdx.y]) return;  // uniform over the block
+  __shared__ uint32_t tiles[CONV_COLS][LDS_TILE];
+  uint32_t *xm = &tiles[4][0];  // CRT-scaled residues [row][m][N] (tiles 4-7)
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int k = blockIdx.x, l = k / t_gsw, j = k % t_gsw;
+  const int mx = 2 * t_gsw;
+  const uint32_t *ct = v + (size_t)blockIdx.y * v_q + (size_t)(2 * k + 1) * 4 * N;
+  const uint32_t *kc = conv + (size_t)blockIdx.y * conv_q;
+  uint32_t *lv = fk + (size_t)blockIdx.y * fk_q + (size_t)l * 2 * mx * 2 * N;  // level l: [m][col][row][P]
+
+  for (int u = tid; u < N; u += 512) {  // column 2j + 1 = ct: 4 polys x N/4 uint4
+    const int poly = u / (N / 4), k4 = u % (N / 4), row = poly >> 1, m = poly & 1;
+    uint4 x = reinterpret_cast<const uint4 *>(ct + (size_t)poly * N)[k4];
+    if (MONT) {
+      const ModConst mc = m ? mc1 : mc0;
+      x.x = canon(shoup_mul(x.x, mc.two32, mc.two32_p, mc.q), mc.q);
+      x.y = canon(shoup_mul(x.y, mc.two32, mc.two32_p, mc.q), mc.q);
+      x.z = canon(shoup_mul(x.z, mc.two32, mc.two32_p, mc.q), mc.q);
+      x.w = canon(shoup_mul(x.w, mc.two32, mc.two32_p, mc.q), mc.q);
+    }
+    reinterpret_cast<uint4 *>(lv + ((size_t)(m * mx + 2 * j + 1) * 2 + row) * N)[k4] = x;
+  }
+  if (wave < 4) {
+    const int row = wave >> 1, m = wave & 1;
+    const ModConst mc = m ? mc1 : mc0;
+    uint32_t r[32];
+    load_p(r, ct + (size_t)(row * 2 + m) * N, lane);
+    ntt_inv_wave(r, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+#pragma unroll
+    for (int i = 0; i < 32; i++)
+      xm[(row * 2 + m) * N + 64 * i + lane] = canon(shoup_mul(r[i], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  }
+  __syncthreads();
+  // Digit polynomial c = wave: row c & 1, digit d = c >> 1 of the raw value
+  // x = CRT(x0, x1) in [0, Q); coefficient 64 i + lane, two digits per word.
+  const int drow = wave & 1, shift = 14 * (wave >> 1);
+  uint32_t dp[16];
+#pragma unroll
+  for (int i2 = 0; i2 < 16; i2++) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int z = 64 * (2 * i2 + h) + lane;
+      uint64_t x = (uint64_t)xm[(drow * 2) * N + z] * Q1 + (uint64_t)xm[(drow * 2 + 1) * N + z] * Q0;
+      x = (x >= QQ) ? x - QQ : x;
+      w |= (uint32_t)((x >> shift) & 0x3FFFu) << (16 * h);
+    }
+    dp[i2] = w;
+  }
+  __syncthreads();  // the residues in tiles 4-7 are dead: every wave now uses its own tile
+  uint32_t *col = lv + (size_t)(2 * j) * 2 * N;
+  conv_residue<MONT, 0>(dp, tiles, kc, col, tw_base, mc0, wave, lane, tid);
+  conv_residue<MONT, 1>(dp, tiles, kc, col + (size_t)mx * 2 * N, tw_base, mc1, wave, lane, tid);
+}
+
+hipError_t launch_regev_to_gsw(bool mont, const uint32_t *v, size_t v_q, int n_queries, int nu2, int t_gsw,
+                               const uint32_t *conv, size_t conv_q, uint32_t *fk, size_t fk_q, const int *only,
+                               const DevTables &t, hipStream_t s) {
+  const dim3 grid((unsigned)(nu2 * t_gsw), (unsigned)n_queries);
+  if (mont)
+    k_regev_to_gsw<true><<<grid, 512, 0, s>>>(v, v_q, t_gsw, conv, conv_q, fk, fk_q, only, t.tw, t.mc[0], t.mc[1]);
+  else
+    k_regev_to_gsw<false><<<grid, 512, 0, s>>>(v, v_q, t_gsw, conv, conv_q, fk, fk_q, only, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
 }
 
 // Expansion phases 1-2 for element v_i: waves 0/1 take v_i.a out of the NTT

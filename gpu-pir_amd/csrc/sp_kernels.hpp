@@ -26,6 +26,9 @@ hipError_t launch_extprod(const uint32_t *in, int rdim, int mx, int bits, const 
 hipError_t launch_nat_to_p(const uint32_t *in, uint32_t *out, size_t n, int m_div, int mont, const DevTables &t,
                            hipStream_t s);
 hipError_t launch_p_to_nat(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s);
+// P -> natural with the 2^32 scaling removed (poly k is residue (k / m_div) & 1).
+hipError_t launch_p_to_nat_unscale(const uint32_t *in, uint32_t *out, size_t n, int m_div, const DevTables &t,
+                                   hipStream_t s);
 hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s);
 hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int R, int C,
                             const DevTables &t, hipStream_t s);
@@ -45,6 +48,14 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
                                hipStream_t s);
 // rows of 4N words at src + r * src_stride -> contiguous dst (response gather for readout).
 hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s);
+
+// Regev -> GSW conversion (t_conv = 4): GSW slots k < nu2 t_gsw of every query,
+// v [query][>= 2 nu2 t_gsw][4N] layout P; conv [query][m][8][row][P] 2^32-scaled;
+// fk [query][nu2][m][2 t_gsw][row][P], 2^32-scaled when mont, else standard.
+// only: NULL, or per-query flags (device int[n_queries]); queries flagged 0 are skipped.
+hipError_t launch_regev_to_gsw(bool mont, const uint32_t *v, size_t v_q, int n_queries, int nu2, int t_gsw,
+                               const uint32_t *conv, size_t conv_q, uint32_t *fk, size_t fk_q, const int *only,
+                               const DevTables &t, hipStream_t s);
 
 // Gadget 2^7 x 8 (t_gsw = t_exp = 8) fold and expansion kernels with digit-owning waves.
 // variant: 0 = by grid size, 2 = 8-wave blocks, 3 = 16-wave blocks, 4 = split

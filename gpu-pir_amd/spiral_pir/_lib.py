@@ -50,12 +50,14 @@ SIGNATURES = {
     "sp_fold": [P, P, I, I, P, P, P],
     "sp_expansion_depth": [I, I, I, PI, PI],
     "sp_expand": [P, P, I, I, I, I, P, P, I, P],
+    "sp_regev_to_gsw": [P, P, SZ, I, I, I, P, P, P],
     "sp_db_create": [P, I, I, I, PP],
     "sp_db_load_bytes": [P, P],
     "sp_db_load_ntt": [P, P],
     "sp_db_destroy": [P],
     "sp_query_create": [P, I, I, PP],
     "sp_query_load": [P, P, P, P, P, P],
+    "sp_query_load_packed": [P, P, P],
     "sp_query_run": [P, I, P],
     "sp_query_read": [P, I, P],
     "sp_query_destroy": [P],
@@ -72,7 +74,9 @@ SIGNATURES = {
     "sp_batch_destroy": [P],
     "sp_keys_create": [P, I, I, P, P, PP],
     "sp_keys_destroy": [P],
+    "sp_keys_set_conversion": [P, I, P],
     "sp_batch_load_many": [P, I, I, P, P, P],
+    "sp_batch_load_packed": [P, I, I, P, P],
     "sp_group_create": [P, I, PP],
     "sp_group_launch": [P, PI, P],
     "sp_group_synchronize": [P],
@@ -83,6 +87,10 @@ SIGNATURES = {
     "sp_client_keygen": [P, P, P],
     "sp_client_query": [P, U64, P, P],
     "sp_client_decode": [P, P, P],
+    "sp_client_conversion_key": [P, I, P],
+    "sp_client_query_packed": [P, U64, P],
+    "sp_client_phase": [P, P, P],
+    "sp_client_secret": [P, P],
 }
 _RET = {
     "sp_last_error": (ctypes.c_char_p, []),

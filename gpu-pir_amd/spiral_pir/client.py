@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .params import Shape
+from .params import T_CONV, Shape
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
@@ -53,6 +53,33 @@ class Client:
         vf = np.zeros(s.fold_words, np.uint32)
         _lib.call("sp_client_query", self._h, ctypes.c_uint64(index), _ptr(q), _ptr(vf))
         return q.reshape(2, 2, -1), vf.reshape(s.nu2, 2, 2 * s.t_gsw, 2, -1)
+
+    def conversion_key(self) -> np.ndarray:
+        """Regev -> GSW conversion key CiphertextHalf[2][2 t_conv] = u32 [2][8][2][N]."""
+        s = self.shape
+        out = np.zeros(s.conv_words, np.uint32)
+        _lib.call("sp_client_conversion_key", self._h, T_CONV, _ptr(out))
+        return out.reshape(2, 2 * T_CONV, 2, -1)
+
+    def query_packed(self, index: int) -> np.ndarray:
+        """Packed query NTT Ciphertext [2][2][N] for record `index`: the server derives
+        the fold vectors from it (Regev -> GSW conversion)."""
+        q = np.zeros(self.shape.query_words, np.uint32)
+        _lib.call("sp_client_query_packed", self._h, ctypes.c_uint64(index), _ptr(q))
+        return q.reshape(2, 2, -1)
+
+    def phase(self, ct: np.ndarray) -> np.ndarray:
+        """b - a*s mod Q (u64 [N], coefficient order) of one NTT Ciphertext [2][2][N]."""
+        c = np.ascontiguousarray(ct, dtype=np.uint32)
+        assert c.size == 4 * 2048
+        out = np.zeros(2048, np.uint64)
+        _lib.call("sp_client_phase", self._h, _ptr(c), ctypes.c_void_p(out.ctypes.data))
+        return out
+
+    def secret(self) -> np.ndarray:
+        out = np.zeros(2048, np.int32)
+        _lib.call("sp_client_secret", self._h, ctypes.c_void_p(out.ctypes.data))
+        return out
 
     def decode(self, response: np.ndarray) -> np.ndarray:
         """n2 raw response ciphertexts -> record bytes [n2][N]."""

@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _lib
-from .params import T_GSW, T_EXP_LEFT, T_EXP_RIGHT
+from .params import T_CONV, T_GSW, T_EXP_LEFT, T_EXP_RIGHT
 from .server import Context
 
 
@@ -90,3 +90,13 @@ def coefficient_expansion(ctx: Context, v, left, right, nu1: int, nu2: int, t_gs
     assert T_EXP_LEFT == T_EXP_RIGHT
     _lib.call("sp_expand", ctx.handle, _dev(v, "v"), nu1, nu2, t_gsw, t_exp, _dev(left, "left"),
               _dev(right, "right"), rounds, _stream())
+
+
+def regev_to_gsw(ctx: Context, v, conversion, v_folding_out, nu2: int, t_gsw: int = T_GSW) -> None:
+    """Regev -> GSW conversion of the expanded query v (NTT Ciphertexts, as
+    coefficient_expansion leaves them) with the conversion key
+    CiphertextHalf[2][2 T_CONV] (PublicParameters.conversion, structures.cuh:176)
+    into v_folding_out CiphertextHalf[nu2][2][2 t_gsw]; the reference leaves
+    regev_to_gsw a stub (cpu_version/conversion.py:3)."""
+    _lib.call("sp_regev_to_gsw", ctx.handle, _dev(v, "v"), v.numel() // (4 * 2048), nu2, t_gsw, T_CONV,
+              _dev(conversion, "conversion"), _dev(v_folding_out, "v_folding_out"), _stream())

@@ -96,5 +96,10 @@ class Shape:
         return self.nu2 * 2 * (2 * self.t_gsw) * 2 * POLY_LEN
 
     @property
+    def conv_words(self) -> int:
+        """Conversion key CiphertextHalf[2][2 t_conv] (gpu-pir/structures.cuh:176)."""
+        return 2 * (2 * T_CONV) * 2 * POLY_LEN
+
+    @property
     def response_words(self) -> int:
         return self.n2 * 4 * POLY_LEN

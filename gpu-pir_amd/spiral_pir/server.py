@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .params import Shape
+from .params import T_CONV, Shape
 
 STAGE_EXPAND = 1
 STAGE_DB_MUL = 2
@@ -28,6 +28,7 @@ READ_RESPONSE = 0
 READ_EXPANDED = 1
 READ_DB_MUL = 2
 READ_RAW_ACCS = 3
+READ_V_FOLDING = 4
 
 
 def _ptr(a) -> ctypes.c_void_p:
@@ -123,6 +124,12 @@ class Query:
         neg = None if v_folding_neg is None else np.ascontiguousarray(v_folding_neg, dtype=np.uint32)
         _lib.call("sp_query_load", self._h, *[_ptr(a) for a in arrs], _ptr(neg))
 
+    def load_packed(self, query: np.ndarray, keys: "Keys") -> None:
+        """A packed query (client.query_packed) with resident keys that carry a
+        conversion key: the fold vectors are converted on the device."""
+        q = np.ascontiguousarray(query, dtype=np.uint32)
+        _lib.call("sp_query_load_packed", self._h, _ptr(q), keys.handle)
+
     def run(self, stages: int = STAGES_ALL, stream=None) -> None:
         _lib.call("sp_query_run", self._h, stages, ctypes.c_void_p(stream or 0))
 
@@ -136,6 +143,8 @@ class Query:
             out = np.zeros((s.n2, s.cols, 2, 2, 2048), np.uint32)
         elif what == READ_RAW_ACCS:
             out = np.zeros((s.n2, s.cols, 2, 2, 2048), np.uint32)
+        elif what == READ_V_FOLDING:
+            out = np.zeros((s.nu2, 2, 2 * s.t_gsw, 2, 2048), np.uint32)
         else:
             raise ValueError(what)
         _lib.call("sp_query_read", self._h, what, _ptr(out))
@@ -208,10 +217,27 @@ class Batch:
         ka = (P * n)(*[k.handle.value for k in keys])
         _lib.call("sp_batch_load_many", self._h, first, n, qa, ka, va)
 
+    def load_packed(self, first: int, queries, keys: "list[Keys]") -> None:
+        """Slots [first, first + len(keys)): packed queries (client.query_packed) with
+        resident keys that carry a conversion key; each slot's fold vectors are
+        converted from its expanded query on the device."""
+        n = len(keys)
+        s = self.db.shape
+        qs = [np.ascontiguousarray(q, dtype=np.uint32) for q in queries]
+        if len(qs) != n:
+            raise ValueError("load_packed: one query and key set per slot")
+        if any(q.size != s.query_words for q in qs):
+            raise ValueError("load_packed: query of the wrong size")
+        P = ctypes.c_void_p
+        qa = (P * n)(*[q.ctypes.data for q in qs])
+        ka = (P * n)(*[k.handle.value for k in keys])
+        _lib.call("sp_batch_load_packed", self._h, first, n, qa, ka)
+
     def read(self, slot: int, what: int = READ_RESPONSE) -> np.ndarray:
         s = self.db.shape
         shapes = {READ_RESPONSE: (s.n2, 2, 2, 2048), READ_EXPANDED: (1 << s.g, 2, 2, 2048),
-                  READ_DB_MUL: (s.n2, s.cols, 2, 2, 2048), READ_RAW_ACCS: (s.n2, s.cols, 2, 2, 2048)}
+                  READ_DB_MUL: (s.n2, s.cols, 2, 2, 2048), READ_RAW_ACCS: (s.n2, s.cols, 2, 2, 2048),
+                  READ_V_FOLDING: (s.nu2, 2, 2 * s.t_gsw, 2, 2048)}
         out = np.zeros(shapes[what], np.uint32)
         _lib.call("sp_batch_read", self._h, slot, what, _ptr(out))
         return out
@@ -233,9 +259,10 @@ class Batch:
 
 
 class Keys:
-    """One client's expansion keys, resident on the device for one database shape."""
+    """One client's expansion keys (and, for packed queries, its conversion key),
+    resident on the device for one database shape."""
 
-    def __init__(self, db: Database, left: np.ndarray, right: np.ndarray):
+    def __init__(self, db: Database, left: np.ndarray, right: np.ndarray, conv: np.ndarray | None = None):
         s = db.shape
         self.db = db
         h = ctypes.c_void_p()
@@ -243,6 +270,13 @@ class Keys:
         rgt = np.ascontiguousarray(right, dtype=np.uint32)
         _lib.call("sp_keys_create", db.handle, s.t_gsw, s.t_exp, _ptr(lft), _ptr(rgt), ctypes.byref(h))
         self._h = h
+        self.packed = conv is not None
+        if conv is not None:
+            cv = np.ascontiguousarray(conv, dtype=np.uint32)
+            if cv.size != s.conv_words:
+                self.close()
+                raise ValueError("Keys: conversion key of the wrong size")
+            _lib.call("sp_keys_set_conversion", h, T_CONV, _ptr(cv))
 
     @property
     def handle(self):

@@ -1,10 +1,12 @@
 """Continuous-batching PIR server over databases resident in HBM.
 
-Requests (a query ciphertext and its GSW fold vectors, from a client whose
-expansion keys are resident on the device) arrive at any time.  A scheduler
+Requests (a query ciphertext and its GSW fold vectors, or a packed query
+whose fold vectors the device converts from the query itself, from a client
+whose keys are resident on the device) arrive at any time.  A scheduler
 thread keeps, per database ("lane"), a few batch buffers of `capacity` slots,
 each with its own stream.  Whenever a buffer is free and requests are waiting,
-it takes up to `capacity` of them, loads them with one bulk call, and replays
+it takes up to `capacity` of them (of one form: packed or with fold vectors),
+loads them with one bulk call, and replays
 the captured stage chain of the smallest size class that holds them.  Requests
 complete when their batch finishes.  Buffers of one lane and of different lanes
 run on the device at the same time, so queries that arrive while others are
@@ -150,30 +152,39 @@ class Server:
         self._error: BaseException | None = None
 
     # ---- set-up ---------------------------------------------------------------
-    def register(self, lane: str, client, left: np.ndarray, right: np.ndarray) -> None:
-        """Make a client's expansion keys resident for `lane`."""
+    def register(self, lane: str, client, left: np.ndarray, right: np.ndarray, conv: np.ndarray | None = None) -> None:
+        """Make a client's expansion keys (and conversion key, for packed queries)
+        resident for `lane`."""
         ln = self.lanes[lane]
-        ln.keys[client] = Keys(ln.db, left, right)
+        ln.keys[client] = Keys(ln.db, left, right, conv)
 
-    def warmup(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray) -> None:
-        """Fill every buffer of `lane` with copies of one query and autotune and
-        capture every size class, so that serving starts without first-use costs."""
+    def warmup(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray | None = None) -> None:
+        """Fill every buffer of `lane` with copies of one query (packed when
+        v_folding is None) and autotune and capture every size class of that
+        form, so that serving starts without first-use costs."""
         ln = self.lanes[lane]
         keys = ln.keys[client]
         for buf, batch in enumerate(ln.batches):
             stream = ln.streams[buf].cuda_stream
-            batch.load_many(0, [query] * ln.capacity, [keys] * ln.capacity, [v_folding] * ln.capacity)
+            if v_folding is None:
+                batch.load_packed(0, [query] * ln.capacity, [keys] * ln.capacity)
+            else:
+                batch.load_many(0, [query] * ln.capacity, [keys] * ln.capacity, [v_folding] * ln.capacity)
             for c in sorted({size_class(n, ln.capacity) for n in range(1, ln.capacity + 1)}):
                 self._prepare(ln, buf, c)
                 batch.graph_launch(c, stream)
                 batch.synchronize()
 
     # ---- requests -------------------------------------------------------------
-    def submit(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray, tag=None) -> Request:
+    def submit(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray | None = None, tag=None) -> Request:
+        """Queue one query; v_folding None means a packed query (the client's
+        keys must carry a conversion key)."""
         if lane not in self.lanes:
             raise KeyError(f"no database lane {lane!r}")
         if client not in self.lanes[lane].keys:
             raise KeyError(f"client {client!r} has no resident keys for lane {lane!r}")
+        if v_folding is None and not getattr(self.lanes[lane].keys[client], "packed", False):
+            raise ValueError(f"client {client!r} has no conversion key for packed queries")
         r = Request(lane, client, query, v_folding, tag)
         with self._lock:
             self._queue.append(r)
@@ -248,12 +259,18 @@ class Server:
 
     def _launch(self, ln: _Lane) -> None:
         buf = ln.free.popleft()
-        n = min(ln.capacity, len(ln.pending))
-        reqs = [ln.pending.popleft() for _ in range(n)]
+        packed = ln.pending[0].v_folding is None
+        reqs = []
+        while ln.pending and len(reqs) < ln.capacity and (ln.pending[0].v_folding is None) == packed:
+            reqs.append(ln.pending.popleft())
+        n = len(reqs)
         batch = ln.batches[buf]
         try:
-            batch.load_many(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs],
-                            [r.v_folding for r in reqs])
+            if packed:
+                batch.load_packed(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs])
+            else:
+                batch.load_many(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs],
+                                [r.v_folding for r in reqs])
             c = size_class(n, ln.capacity)
             self._prepare(ln, buf, c)
             batch.graph_launch(c, ln.streams[buf].cuda_stream)

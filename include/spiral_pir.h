@@ -44,6 +44,7 @@ extern "C" {
 #define SP_ERR_STATE (-4)   /* object used in the wrong state             */
 
 #define SP_POLY_LEN 2048
+#define SP_T_CONV 4        /* conversion gadget digits (gpu-pir/params.cuh:65) */
 #define SP_Q1 268369921u
 #define SP_Q2 249561089u
 
@@ -141,6 +142,22 @@ int sp_expansion_depth(int nu1, int nu2, int t_gsw, int *g, int *stop_round);
 int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_gsw, int t_exp,
               const uint32_t *dev_left, const uint32_t *dev_right, int rounds, void *stream);
 
+/* ---- Regev -> GSW conversion ------------------------------------------------
+ * The reference declares the conversion key (PublicParameters.conversion,
+ * gpu-pir/structures.cuh:173-177; cpu_version/structures.py:44-45), its
+ * external-product shape "conversion: 2 by 2t_conv" x "2 by 1 -> 2t_conv by
+ * 1" (gpu-pir/gadget.cuh:41-53) and T_CONV = 4 (params.cuh:65), and leaves
+ * regev_to_gsw a stub (gpu-pir/cpu_version/conversion.py:3).  This is
+ * Spiral's RegevToGSW: GSW slot k = l*t_gsw + j takes the expanded Regev
+ * ciphertext v[2k+1]; column 2j+1 of fold level l is that ciphertext, column
+ * 2j is conv * NTT(g^-1(v[2k+1])) (rdim 2, t_conv digits of 14 bits).
+ *   dev_v          NTT Ciphertext[n_ct] as sp_expand leaves it (n_ct >= 2 nu2 t_gsw)
+ *   dev_conv       key matrix CiphertextHalf[2 crt][2 t_conv], standard form
+ *   dev_v_folding  out: CiphertextHalf[nu2][2 crt][2 t_gsw], as sp_fold takes it
+ * t_conv must be SP_T_CONV. */
+int sp_regev_to_gsw(sp_ctx *ctx, const uint32_t *dev_v, size_t n_ct, int nu2, int t_gsw, int t_conv,
+                    const uint32_t *dev_conv, uint32_t *dev_v_folding, void *stream);
+
 /* ---- resident database and answer path -------------------------------------
  * A database of 2^(nu1+nu2) records, each n2 polynomials of 2048 plaintext
  * bytes (n2 = n*n sub-databases; record index = i1 * 2^nu2 + i2).  It is
@@ -166,7 +183,9 @@ size_t sp_db_device_bytes(const sp_db *db);
  *   sp_query_read   synchronizes, then copies out: what = 0 the n2 response
  *                   ciphertexts (raw), 1 the expanded v (NTT, natural order),
  *                   2 the database-multiply accumulators (NTT, natural order,
- *                   between stages 2 and 4), 3 all n2*2^nu2 raw accumulators. */
+ *                   between stages 2 and 4), 3 all n2*2^nu2 raw accumulators,
+ *                   4 the fold vectors v_folding (CiphertextHalf[nu2][2][2 t_gsw],
+ *                   standard form, natural order: given or converted). */
 typedef struct sp_query sp_query;
 int sp_query_create(sp_db *db, int t_gsw, int t_exp, sp_query **out);
 int sp_query_load(sp_query *q, const uint32_t *host_query, const uint32_t *host_left,
@@ -215,6 +234,10 @@ typedef struct sp_keys sp_keys;
 int sp_keys_create(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_left, const uint32_t *host_right,
                    sp_keys **out);
 int sp_keys_destroy(sp_keys *k);
+/* Adds the client's conversion key (CiphertextHalf[2 crt][2 t_conv], standard
+ * form, host memory; t_conv = SP_T_CONV) for packed queries.  The reference
+ * keeps it in PublicParameters.conversion (gpu-pir/structures.cuh:176). */
+int sp_keys_set_conversion(sp_keys *k, int t_conv, const uint32_t *host_conv);
 /* Loads slots [first, first + n) of a batch in one go: host_queries[i] the
  * query ciphertext of slot first + i ([4N], NTT natural order), keys[i] its
  * resident keys (same database and gadget as the batch; copied device to
@@ -223,6 +246,16 @@ int sp_keys_destroy(sp_keys *k);
  * reused (pinned buffers make the transfers asynchronous DMA). */
 int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *host_queries, sp_keys *const *keys,
                        const uint32_t *const *host_v_folding);
+/* Packed queries: as sp_batch_load_many without fold vectors.  Each slot's
+ * v_folding is derived on the device from its own expanded query by the
+ * Regev -> GSW conversion (sp_regev_to_gsw), which then runs as part of the
+ * expansion stage; keys[i] must carry a conversion key.  A run may mix
+ * packed slots and slots loaded with fold vectors: only packed slots are
+ * converted.  No reference counterpart (the reference's conversion is a stub). */
+int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                         sp_keys *const *keys);
+/* A packed query into a single-query object (slot 0). */
+int sp_query_load_packed(sp_query *q, const uint32_t *host_query, sp_keys *keys);
 
 /* Several batches (each against its own database, e.g. of different shapes)
  * answered by one launch: sp_group_launch runs all stages of slots
@@ -263,6 +296,18 @@ int sp_client_keygen(sp_client *c, uint32_t *host_left, uint32_t *host_right);
 int sp_client_query(sp_client *c, uint64_t index, uint32_t *host_query,
                     uint32_t *host_v_folding);
 int sp_client_decode(sp_client *c, const uint32_t *host_response, uint8_t *host_record);
+/* Conversion key for packed queries: CiphertextHalf[2 crt][2 t_conv]; column
+ * 2k encrypts s^2 * 2^(bits k), column 2k+1 encrypts -s * 2^(bits k),
+ * bits = ceil(56 / t_conv). */
+int sp_client_conversion_key(sp_client *c, int t_conv, uint32_t *host_conv);
+/* Packed query: one NTT Ciphertext whose plaintext holds the first-dimension
+ * selector at coefficient 2 i1 (scaled by 2^-g) and bit l of i2 times
+ * 2^(bits j) at coefficient 2 (l t_gsw + j) + 1 (scaled by 2^-(stop_round+1)). */
+int sp_client_query_packed(sp_client *c, uint64_t index, uint32_t *host_query);
+/* Phase b - a*s mod Q (u64[2048], coefficient order) of one NTT Ciphertext,
+ * and the ternary secret (int32[2048]): for noise measurements in tests. */
+int sp_client_phase(const sp_client *c, const uint32_t *host_ct, uint64_t *host_phase);
+int sp_client_secret(const sp_client *c, int32_t *host_s);
 const char *sp_client_last_error(void);
 
 #ifdef __cplusplus

@@ -61,6 +61,7 @@ def lib():
             "or_v_neg1": ([I, P], None),
             "or_expand": ([P, I, I, I, I, P, P, I], None),
             "or_folding_neg": ([P, P, I, I], None),
+            "or_regev_to_gsw": ([P, I, I, I, P, P], None),
             "or_answer": ([P, I, I, I, I, I, P, P, P, P, P, P], None),
             "or_db_encode": ([P, P, I, I, I], None),
         }
@@ -197,6 +198,17 @@ def folding_neg(v_folding, nu2: int, t_gsw: int) -> np.ndarray:
     v_folding = _u32(v_folding)
     out = np.zeros_like(v_folding)
     lib().or_folding_neg(_p(v_folding), _p(out), nu2, t_gsw)
+    return out
+
+
+def regev_to_gsw(v, nu2: int, t_gsw: int, t_conv: int, conv) -> np.ndarray:
+    """Expanded NTT Ciphertexts v[>= 2 nu2 t_gsw] + conversion key
+    CiphertextHalf[2][2 t_conv] -> v_folding CiphertextHalf[nu2][2][2 t_gsw]."""
+    v = _u32(v)
+    conv = _u32(conv)
+    assert v.size >= 2 * nu2 * t_gsw * 4 * N and conv.size == 2 * 2 * t_conv * 2 * N
+    out = np.zeros((nu2, 2, 2 * t_gsw, 2, N), np.uint32)
+    lib().or_regev_to_gsw(_p(v), nu2, t_gsw, t_conv, _p(conv), _p(out))
     return out
 
 

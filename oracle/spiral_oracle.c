@@ -27,6 +27,12 @@
  *   or_db_mul      brute-force re-derivation per coefficient with Python ints;
  *                  linearity; decryption of the selected record
  *   or_answer      end to end: the client decodes the stored record
+ *   or_regev_to_gsw  no reference artifact (the reference's conversion is a
+ *                  stub): pinned by decryption of every converted column under
+ *                  the client's secret (column 2j+1 -> bit * 2^(7j), column 2j
+ *                  -> -s * bit * 2^(7j)) and by end-to-end decoding of packed
+ *                  queries; its from_ntt / external-product parts are the
+ *                  KAT-pinned functions above
  */
 #include "spiral_oracle.h"
 
@@ -411,6 +417,47 @@ void or_folding_neg(const uint32_t *v_folding, uint32_t *v_folding_neg,
                         v_folding_neg[base + z] =
                             (uint32_t)((gval + Q_[m] - v_folding[base + z]) % Q_[m]);
                 }
+}
+
+/* Regev -> GSW conversion.  The reference declares its pieces but leaves the
+ * function a stub (gpu-pir/cpu_version/conversion.py:3): the conversion key
+ * PublicParameters.conversion, 2 x 2*T_CONV (gpu-pir/structures.cuh:173-177,
+ * cpu_version/structures.py:44-45), T_CONV = 4 (gpu-pir/params.cuh:65), and the
+ * external-product shape "conversion: 2 by 1 -> 2t_conv by 1" with output
+ * "2 by 1" (gpu-pir/gadget.cuh:41-53).  The steps are Spiral's RegevToGSW
+ * (Menon-Wu 2022, section 3.3), as in spiral-rs's regev_to_gsw:
+ *   GSW slot k = l*t_gsw + j (level l, gadget digit j) takes the expanded
+ *   Regev ciphertext ct = v[2k + 1];
+ *   column 2j+1 of level l = ct;
+ *   column 2j   of level l = conv * NTT(g^-1(from_ntt(ct))), rdim 2 and
+ *                            t_conv digits of ceil(56 / t_conv) bits.
+ * v: NTT Ciphertext[>= 2 nu2 t_gsw]; conv: CiphertextHalf[2][2 t_conv];
+ * v_folding: CiphertextHalf[nu2][2][2 t_gsw]. */
+void or_regev_to_gsw(const uint32_t *v, int nu2, int t_gsw, int t_conv,
+                     const uint32_t *conv, uint32_t *v_folding) {
+    init();
+    const int mx = 2 * t_gsw;
+    const size_t ct_words = 4 * OR_N;
+    uint32_t *raw = (uint32_t *)malloc(ct_words * sizeof(uint32_t));
+    uint32_t *acc = (uint32_t *)malloc(ct_words * sizeof(uint32_t));
+    for (int l = 0; l < nu2; l++)
+        for (int j = 0; j < t_gsw; j++) {
+            const uint32_t *ct = v + (size_t)(2 * (l * t_gsw + j) + 1) * ct_words;
+            or_from_ntt(ct, raw);
+            or_from_ntt(ct + 2 * OR_N, raw + 2 * OR_N);
+            memset(acc, 0, ct_words * sizeof(uint32_t));
+            or_extprod_acc(raw, 2, 2 * t_conv, conv, acc);
+            for (int m = 0; m < 2; m++)
+                for (int row = 0; row < 2; row++) {
+                    uint32_t *lv = v_folding + (((size_t)l * 2 + m) * mx) * 2 * OR_N;
+                    memcpy(lv + ((size_t)(2 * j) * 2 + row) * OR_N, acc + ((size_t)row * 2 + m) * OR_N,
+                           OR_N * sizeof(uint32_t));
+                    memcpy(lv + ((size_t)(2 * j + 1) * 2 + row) * OR_N, ct + ((size_t)row * 2 + m) * OR_N,
+                           OR_N * sizeof(uint32_t));
+                }
+        }
+    free(raw);
+    free(acc);
 }
 
 void or_answer(const uint32_t *db, int n2, int nu1, int nu2, int t_gsw, int t_exp,

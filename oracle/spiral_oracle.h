@@ -107,6 +107,14 @@ void or_expand(uint32_t *v, int nu1, int nu2, int t_gsw, int t_exp,
 void or_folding_neg(const uint32_t *v_folding, uint32_t *v_folding_neg,
                     int nu2, int t_gsw);
 
+/* Regev -> GSW conversion (Spiral's RegevToGSW; the reference leaves it a
+ * stub, gpu-pir/cpu_version/conversion.py:3).  For GSW slot k = l*t_gsw + j:
+ * column 2j+1 of level l = v[2k+1], column 2j = conv * NTT(g^-1(v[2k+1]))
+ * with rdim 2 and t_conv digits.  v: NTT Ciphertext[>= 2 nu2 t_gsw];
+ * conv: CiphertextHalf[2 crt][2 t_conv]; v_folding: CiphertextHalf[nu2][2][2 t_gsw]. */
+void or_regev_to_gsw(const uint32_t *v, int nu2, int t_gsw, int t_conv,
+                     const uint32_t *conv, uint32_t *v_folding);
+
 /* Whole answer for one query against an n^2-trial database:
  *   db   [n2][2^nu2][2^nu1] NTT Poly;  query: NTT Ciphertext;
  *   out  [n2] raw Ciphertext.                                              */

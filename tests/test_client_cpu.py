@@ -74,3 +74,116 @@ def test_client_rejects_bad_index():
     c = Client(Shape(6, 2), 1)
     with pytest.raises(SpiralError):
         c.query(1 << 8)
+
+
+# ---- packed queries: Regev -> GSW conversion --------------------------------
+# The reference leaves the conversion a stub (gpu-pir/cpu_version/conversion.py:3),
+# so no reference artifact pins it: the checks below are decryption invariants
+# under the client's secret and end-to-end decoding.
+
+def _signed(phase: np.ndarray) -> np.ndarray:
+    """Phase in [0, Q) -> signed integers in (-Q/2, Q/2] (object array)."""
+    from oracle import Q
+
+    return np.array([int(x) - Q if int(x) > Q // 2 else int(x) for x in phase], dtype=object)
+
+
+def _negacyclic_const_times(s: np.ndarray, c: int) -> np.ndarray:
+    """c * s(x) as signed integers (c a constant polynomial)."""
+    return np.array([c * int(x) for x in s], dtype=object)
+
+
+def _packed_case(oracle, nu1, nu2, index, seed_client):
+    from spiral_pir import Shape
+    from spiral_pir.client import Client
+    from spiral_pir.params import T_CONV
+
+    s = Shape(nu1, nu2)
+    c = Client(s, seed_client)
+    left, right = c.keygen()
+    conv = c.conversion_key()
+    q = c.query_packed(index)
+    v = oracle.expand(q, nu1, nu2, s.t_gsw, s.t_exp, left, right)
+    vf = oracle.regev_to_gsw(v, nu2, s.t_gsw, T_CONV, conv)
+    return s, c, (q, left, right, conv), v, vf
+
+
+@pytest.mark.parametrize("index", [37, 255, 0])
+def test_packed_query_end_to_end(oracle, index):
+    """Packed query -> expansion -> conversion -> answer decodes the record."""
+    nu1, nu2 = 6, 2
+    s, c, (q, left, right, conv), v, vf = _packed_case(oracle, nu1, nu2, index, 2300 + index)
+    recs = rng(1000).integers(0, 256, size=(s.records, s.n2, 2048), dtype=np.uint8)
+    db = oracle.db_encode(recs, s.n2, nu1, nu2)
+    out = oracle.answer(db, s.n2, nu1, nu2, s.t_gsw, s.t_exp, q, left, right, vf,
+                        oracle.folding_neg(vf, nu2, s.t_gsw))
+    np.testing.assert_array_equal(c.decode(out), recs[index])
+
+
+def test_converted_columns_decrypt_to_gadget_times_bit(oracle):
+    """Column 2j+1 of level l decrypts to bit_l * 2^(7j), column 2j to
+    -s * bit_l * 2^(7j), both up to noise far below Q / (2p)."""
+    nu1, nu2, index = 6, 2, 37 * 4 + 2  # i2 = 2: bit 0 = 0, bit 1 = 1
+    s, c, _, v, vf = _packed_case(oracle, nu1, nu2, index, 2400)
+    sec = c.secret()
+    i2 = index & ((1 << nu2) - 1)
+    worst = 0
+    for lvl in range(nu2):
+        bit = (i2 >> lvl) & 1
+        for j in range(s.t_gsw):
+            g = 1 << (7 * j)
+            for col, want in ((2 * j + 1, None), (2 * j, "neg_s")):
+                ct = np.stack([np.stack([vf[lvl, m, col, row] for m in range(2)]) for row in range(2)])
+                ph = _signed(c.phase(ct))
+                if want is None:
+                    exp = np.zeros(2048, dtype=object)
+                    exp[0] = bit * g
+                else:
+                    exp = _negacyclic_const_times(sec, -bit * g)
+                noise = max(abs(int(x)) for x in ph - exp)
+                worst = max(worst, noise)
+    # Q / (2p) = 2^47; the conversion noise is about 2^25 here
+    assert worst < 2 ** 36, worst
+
+
+def test_packed_even_slots_are_the_selectors(oracle):
+    """The GSW bits in the odd coefficients leave the first-dimension selectors intact."""
+    nu1, nu2, index = 6, 2, 37
+    s, c, _, v, vf = _packed_case(oracle, nu1, nu2, index, 2500)
+    from oracle import Q
+
+    delta = Q // 256
+    for j in (0, index >> nu2, 63):
+        ph = _signed(c.phase(v[2 * j]))
+        exp = delta if j == index >> nu2 else 0
+        assert abs(int(ph[0]) - exp) < 2 ** 36
+        assert max(abs(int(x)) for x in ph[1:]) < 2 ** 36
+
+
+def test_conversion_detects_swapped_columns(oracle):
+    """A conversion with its two columns swapped (a planted mistake) fails the
+    decryption check: column 2j+1 then decrypts to -s * bit * 2^(7j), far from
+    the constant bit * 2^(7j)."""
+    nu1, nu2, index = 6, 2, 3  # i2 = 3: both bits set
+    s, c, _, v, vf = _packed_case(oracle, nu1, nu2, index, 2600)
+    bad = vf.copy()
+    bad[:, :, 0::2], bad[:, :, 1::2] = vf[:, :, 1::2], vf[:, :, 0::2]
+    j = 7
+    for arr, ok in ((vf, True), (bad, False)):
+        ct = np.stack([np.stack([arr[0, m, 2 * j + 1, row] for m in range(2)]) for row in range(2)])
+        exp = np.zeros(2048, dtype=object)
+        exp[0] = 1 << (7 * j)
+        noise = max(abs(int(x)) for x in _signed(c.phase(ct)) - exp)
+        assert (noise < 2 ** 36) == ok, noise
+
+
+def test_conversion_key_shape_and_determinism():
+    from spiral_pir import Shape
+    from spiral_pir.client import Client
+
+    s = Shape(6, 2)
+    a, b = Client(s, 9), Client(s, 9)
+    ka, kb = a.conversion_key(), b.conversion_key()
+    assert ka.shape == (2, 8, 2, 2048)
+    np.testing.assert_array_equal(ka, kb)
+    assert (ka[0] < 268369921).all() and (ka[1] < 249561089).all()

@@ -197,3 +197,55 @@ def test_serving_warmup_and_size_classes(gpu_ctx):
     srv.stop()
     srv.close()
     db.close()
+
+
+def test_serving_packed_and_vector_requests(gpu_ctx):
+    """Packed queries (fold vectors converted on the device) and queries with
+    fold vectors, interleaved in one lane: each response equals the query
+    answered alone and decodes to the record."""
+    from spiral_pir.client import Client
+    from spiral_pir.serving import Server
+
+    s, recs, db = _db(gpu_ctx, 6, 4, 1600)
+    srv = Server({"x": db}, max_batch=4, buffers=2)
+    clients = []
+    for k in range(2):
+        cl = Client(s, 5600 + k)
+        left, right = cl.keygen()
+        conv = cl.conversion_key()
+        srv.register("x", k, left, right, conv)
+        clients.append((cl, left, right, conv))
+    from spiral_pir.server import Keys, Query
+
+    expected = {}
+    reqs_in = []
+    g = rng(88)
+    for it in range(14):
+        k = int(g.integers(0, 2))
+        cl, left, right, conv = clients[k]
+        idx = int(g.integers(0, s.records))
+        one = Query(db)
+        if it % 3 == 0:
+            q, vf = cl.query(idx)
+            one.load(q, left, right, vf)
+        else:
+            q, vf = cl.query_packed(idx), None
+            kk = Keys(db, left, right, conv)
+            one.load_packed(q, kk)
+        one.run()
+        expected[it] = (cl, idx, one.read())
+        one.close()
+        if vf is None:
+            kk.close()
+        reqs_in.append((k, q, vf, it))
+    srv.start()
+    reqs = [srv.submit("x", k, q, vf, tag=it) for (k, q, vf, it) in reqs_in]
+    for r in reqs:
+        r.wait(timeout=60)
+    srv.stop()
+    for r in reqs:
+        cl, idx, ref = expected[r.tag]
+        np.testing.assert_array_equal(r.response, ref, err_msg=str(r.tag))
+        np.testing.assert_array_equal(cl.decode(r.response), recs[idx])
+    srv.close()
+    db.close()

@@ -23,12 +23,21 @@ class FakeBatch:
         self.loaded = []
         self.launches = []
         self.tuned = []
+        self.forms = []
 
     def load_many(self, first, queries, keys, v_folding):
         if self.fail_load or (self.fail_after is not None and len(self.launches) >= self.fail_after):
             raise RuntimeError("injected load failure")
         assert first == 0 and len(queries) == len(keys) == len(v_folding)
         self.loaded = [int(q[0]) for q in queries]
+        self.forms.append(("vectors", len(queries)))
+
+    def load_packed(self, first, queries, keys):
+        if self.fail_load:
+            raise RuntimeError("injected load failure")
+        assert first == 0 and len(queries) == len(keys)
+        self.loaded = [int(q[0]) for q in queries]
+        self.forms.append(("packed", len(queries)))
 
     def run(self, n, stages, stream=None):
         pass
@@ -58,7 +67,8 @@ def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
         ln.batches = batches
         ln.streams = [FakeStream() for _ in batches]
         ln.free = collections.deque(range(len(batches)))
-        ln.inflight, ln.pending, ln.keys, ln.ready = {}, collections.deque(), {"c": object()}, set()
+        ln.inflight, ln.pending, ln.ready = {}, collections.deque(), set()
+        ln.keys = {"c": object(), "p": type("PackedKeys", (), {"packed": True})()}
         ln.launched_batches = ln.launched_queries = 0
         srv.lanes[name] = ln
     srv.admission = {}
@@ -148,3 +158,23 @@ def test_stop_drains_pending_requests():
     reqs = [srv.submit("x", "c", _q(i), np.zeros(2, np.uint32), tag=i) for i in range(7)]
     srv.stop()
     assert all(r.done and int(r.response[0]) == r.tag for r in reqs)
+
+
+def test_packed_and_vector_requests_batch_separately():
+    """Consecutive requests of one form share a batch; a change of form starts a new one."""
+    b = FakeBatch()
+    srv = _server({"x": [b]}, capacity=8)
+    forms = ["v", "v", "p", "p", "p", "v"]
+    reqs = [srv.submit("x", "p" if f == "p" else "c", _q(i), None if f == "p" else np.zeros(2, np.uint32), tag=i)
+            for i, f in enumerate(forms)]
+    srv.start()
+    for r in reqs:
+        assert int(r.wait(5)[0]) == r.tag
+    srv.stop()
+    assert b.forms == [("vectors", 2), ("packed", 3), ("vectors", 1)]
+
+
+def test_packed_request_needs_conversion_key():
+    srv = _server({"x": [FakeBatch()]})
+    with pytest.raises(ValueError, match="conversion key"):
+        srv.submit("x", "c", _q(0), None)
