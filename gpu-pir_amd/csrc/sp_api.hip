@@ -369,6 +369,37 @@ extern "C" int sp_regev_to_gsw(sp_ctx *ctx, const uint32_t *dev_v, size_t n_ct, 
   return SP_OK;
 }
 
+static int check_packing(int n2, int q2_bits, const char *who) {
+  if (n2 != SP_PACK_N * SP_PACK_N) return fail(SP_ERR_ARG, "%s: packing needs n2 = %d (got %d)", who, SP_PACK_N * SP_PACK_N, n2);
+  if (q2_bits < SP_PACK_Q1_BITS + 1 || q2_bits > 32) return fail(SP_ERR_ARG, "%s: q2_bits %d out of [11, 32]", who, q2_bits);
+  return SP_OK;
+}
+
+extern "C" int sp_packed_response_bytes(int n2, int q2_bits, size_t *bytes) {
+  if (!bytes) return fail(SP_ERR_ARG, "sp_packed_response_bytes: NULL");
+  int rc = check_packing(n2, q2_bits, "sp_packed_response_bytes");
+  if (rc) return rc;
+  *bytes = packed_response_bytes(q2_bits);
+  return SP_OK;
+}
+
+extern "C" int sp_pack_response(sp_ctx *ctx, const uint32_t *dev_raw, int n2, const uint32_t *dev_w, int t_conv,
+                                int q2_bits, uint8_t *dev_out, void *stream) {
+  if (!ctx || !dev_raw || !dev_w || !dev_out) return fail(SP_ERR_ARG, "sp_pack_response: NULL");
+  int rc = check_packing(n2, q2_bits, "sp_pack_response");
+  if (rc) return rc;
+  if (t_conv != SP_T_CONV) return fail(SP_ERR_ARG, "sp_pack_response: t_conv %d unsupported (%d)", t_conv, SP_T_CONV);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const size_t w_polys = (size_t)SP_PACK_N * 2 * t_conv * (SP_PACK_N + 1);
+  DevBuf kw;
+  HIP_TRY(kw.alloc(w_polys * N * 4));
+  HIP_TRY(launch_nat_to_p(dev_w, kw.u32(), w_polys, t_conv * (SP_PACK_N + 1), 1, ctx->tables, s));
+  HIP_TRY(launch_pack_response(dev_raw, 0, 4 * N, 1, kw.u32(), 0, nullptr, dev_out, 0, q2_bits, ctx->tables, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SP_OK;
+}
+
 static int ceil_log2(uint64_t x) {
   int v = 0;
   uint64_t p = 1;
@@ -593,6 +624,13 @@ struct sp_batch {
   size_t conv_q = 0;
   std::vector<int> packed;        // per slot: 1 = packed query (v_folding converted on the device)
   int *d_packed = nullptr;        // device copy of `packed`
+  int pack_q2 = 0;                // response packing: 0 off, else bits of the rescaled row 0
+  uint32_t *pk = nullptr;         // packing keys per slot [cap][r][m][k][row][P], 2^32-scaled
+  size_t pk_q = 0;
+  std::vector<int> haspk;         // per slot: 1 = packing keys loaded
+  int *d_haspk = nullptr;
+  uint8_t *pout = nullptr;        // packed responses [cap][pout_q] bytes
+  size_t pout_q = 0;
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
@@ -615,6 +653,9 @@ static void batch_free(sp_batch *b) {
     if (p) (void)hipFree(p);
   if (b->mfma_a) (void)hipFree(b->mfma_a);
   if (b->d_packed) (void)hipFree(b->d_packed);
+  if (b->d_haspk) (void)hipFree(b->d_haspk);
+  if (b->pk) (void)hipFree(b->pk);
+  if (b->pout) (void)hipFree(b->pout);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->io) (void)hipStreamDestroy(b->io);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -646,6 +687,8 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   b->acc_q = (size_t)db->n2 * C * 4 * N;
   b->conv_q = (size_t)2 * (2 * SP_T_CONV) * 2 * N;
   b->packed.assign((size_t)capacity, 0);
+  b->haspk.assign((size_t)capacity, 0);
+  b->pk_q = (size_t)SP_PACK_N * 2 * SP_T_CONV * (SP_PACK_N + 1) * N;
   b->stage_words = std::max<size_t>({b->l_q, b->r_q, b->f_q, b->acc_q, b->v_q});
   const size_t cap = (size_t)capacity;
   hipError_t e = hipSuccess;
@@ -661,6 +704,8 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   A(&b->stage, b->stage_words);
   if (e == hipSuccess) e = hipMalloc(&b->d_packed, cap * sizeof(int));
   if (e == hipSuccess) e = hipMemset(b->d_packed, 0, cap * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_haspk, cap * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(b->d_haspk, 0, cap * sizeof(int));
   if (t_exp == 8) {  // digit scratch of the split expansion
     b->dig_q = (size_t)expansion_max_work(db->nu1, db->nu2, t_gsw) * 8 * 512;
     A(&b->dig, cap * b->dig_q);
@@ -705,6 +750,8 @@ static int set_slot_form(sp_batch *b, int first, int n, int packed) {
   return SP_OK;
 }
 
+static int load_slot_packing(sp_batch *b, int first, int n, sp_keys *const *keys);
+
 static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const uint32_t *host_left,
                       const uint32_t *host_right, const uint32_t *host_v_folding, const uint32_t *host_v_folding_neg) {
   if (!b || !host_query || !host_left || !host_right || !host_v_folding) return fail(SP_ERR_ARG, "load: NULL");
@@ -730,6 +777,7 @@ static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const u
       return rc;
   }
   if ((rc = set_slot_form(b, slot, 1, 0))) return rc;
+  if ((rc = load_slot_packing(b, slot, 1, nullptr))) return rc;
   b->fold_mode = mode;
   return SP_OK;
 }
@@ -746,7 +794,33 @@ struct sp_keys {
   size_t l_words = 0, r_words = 0;
   uint32_t *left = nullptr, *right = nullptr;  // layout P, 2^32-scaled: as a batch slot's keys
   uint32_t *conv = nullptr;                    // conversion key [m][2 t_conv][row][P], 2^32-scaled
+  uint32_t *pk = nullptr;                      // packing keys [r][m][k][row][P], 2^32-scaled
 };
+
+// Packing keys of slots [first, first + n): copied from keys[i] when it has
+// them (device to device, io stream), flags recorded on host and device.
+static int load_slot_packing(sp_batch *b, int first, int n, sp_keys *const *keys) {
+  bool any = false;
+  for (int i = 0; i < n; i++) any |= keys && keys[i]->pk;
+  if (any && !b->pk) {
+    hipError_t e = hipMalloc(&b->pk, (size_t)b->cap * b->pk_q * 4);
+    if (e != hipSuccess) {
+      b->pk = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "packing keys: %s", hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    const bool has = keys && keys[i]->pk;
+    b->haspk[(size_t)first + i] = has;
+    if (has)
+      HIP_TRY(hipMemcpyAsync(b->pk + ((size_t)first + i) * b->pk_q, keys[i]->pk, b->pk_q * 4,
+                             hipMemcpyDeviceToDevice, b->io));
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_haspk + first, b->haspk.data() + first, (size_t)n * sizeof(int),
+                         hipMemcpyHostToDevice, b->io));
+  HIP_TRY(hipStreamSynchronize(b->io));
+  return SP_OK;
+}
 
 extern "C" int sp_keys_create(sp_db *db, int t_gsw, int t_exp, const uint32_t *host_left, const uint32_t *host_right,
                               sp_keys **out) {
@@ -791,6 +865,7 @@ extern "C" int sp_keys_destroy(sp_keys *k) {
   if (k->left) (void)hipFree(k->left);
   if (k->right) (void)hipFree(k->right);
   if (k->conv) (void)hipFree(k->conv);
+  if (k->pk) (void)hipFree(k->pk);
   delete k;
   return SP_OK;
 }
@@ -858,8 +933,80 @@ extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t 
   }
   HIP_TRY(hipStreamSynchronize(b->io));
   int rc = set_slot_form(b, first, n, 0);
+  if (!rc) rc = load_slot_packing(b, first, n, keys);
   if (rc) return rc;
   b->fold_mode = 1;
+  return SP_OK;
+}
+
+extern "C" int sp_keys_set_packing(sp_keys *k, int t_conv, const uint32_t *host_w) {
+  if (!k || !host_w) return fail(SP_ERR_ARG, "sp_keys_set_packing: NULL");
+  if (k->db->n2 != SP_PACK_N * SP_PACK_N) return fail(SP_ERR_ARG, "sp_keys_set_packing: packing needs n2 = 4");
+  if (t_conv != SP_T_CONV) return fail(SP_ERR_ARG, "sp_keys_set_packing: t_conv %d unsupported (%d)", t_conv, SP_T_CONV);
+  sp_ctx *ctx = k->db->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t polys = (size_t)SP_PACK_N * 2 * t_conv * (SP_PACK_N + 1);
+  if (!k->pk) {
+    hipError_t e = hipMalloc(&k->pk, polys * N * 4);
+    if (e != hipSuccess) {
+      k->pk = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_keys_set_packing: %s", hipGetErrorString(e));
+    }
+  }
+  DevBuf stage;
+  HIP_TRY(stage.alloc(polys * N * 4));
+  HIP_TRY(hipMemcpyAsync(stage.p, host_w, polys * N * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(launch_nat_to_p(stage.u32(), k->pk, polys, t_conv * (SP_PACK_N + 1), 1, ctx->tables, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SP_OK;
+}
+
+extern "C" int sp_batch_set_packing(sp_batch *b, int q2_bits) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_set_packing: NULL");
+  if (q2_bits == 0) {
+    b->pack_q2 = 0;
+    return SP_OK;
+  }
+  int rc = check_packing(b->db->n2, q2_bits, "sp_batch_set_packing");
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));
+  const size_t bytes = packed_response_bytes(q2_bits);
+  if (bytes > b->pout_q) {
+    if (b->pout) (void)hipFree(b->pout);
+    b->pout = nullptr;
+    b->pout_q = 0;
+    hipError_t e = hipMalloc(&b->pout, (size_t)b->cap * bytes);
+    if (e != hipSuccess) {
+      b->pout = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_batch_set_packing: %s", hipGetErrorString(e));
+    }
+    b->pout_q = bytes;
+  }
+  if (!b->pk) {
+    hipError_t e = hipMalloc(&b->pk, (size_t)b->cap * b->pk_q * 4);
+    if (e != hipSuccess) {
+      b->pk = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_batch_set_packing: %s", hipGetErrorString(e));
+    }
+  }
+  b->pack_q2 = q2_bits;
+  return SP_OK;
+}
+
+extern "C" int sp_batch_read_packed(sp_batch *b, int n, uint8_t *host_out) {
+  if (!b || !host_out) return fail(SP_ERR_ARG, "sp_batch_read_packed: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_read_packed: n = %d", n);
+  if (!b->pack_q2) return fail(SP_ERR_STATE, "sp_batch_read_packed: packing is off");
+  for (int i = 0; i < n; i++)
+    if (!b->haspk[i]) return fail(SP_ERR_STATE, "sp_batch_read_packed: slot %d has no packing keys", i);
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  HIP_TRY(hipEventSynchronize(b->done));
+  const size_t bytes = packed_response_bytes(b->pack_q2);
+  for (int i = 0; i < n; i++)
+    HIP_TRY(hipMemcpyAsync(host_out + (size_t)i * bytes, b->pout + (size_t)i * b->pout_q, bytes,
+                           hipMemcpyDeviceToHost, b->io));
+  HIP_TRY(hipStreamSynchronize(b->io));
   return SP_OK;
 }
 
@@ -897,6 +1044,7 @@ extern "C" int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_
   }
   HIP_TRY(hipStreamSynchronize(b->io));
   int rc = set_slot_form(b, first, n, 1);
+  if (!rc) rc = load_slot_packing(b, first, n, keys);
   if (rc) return rc;
   b->fold_mode = 1;
   return SP_OK;
@@ -979,6 +1127,9 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
                                   fast ? nullptr : b->fkeys_neg + off, b->f_q, b->t_gsw, gadget_bits(b->t_gsw),
                                   ctx->tables, s));
     }
+    if (b->pack_q2)
+      HIP_TRY(launch_pack_response(b->acc, b->acc_q, (size_t)C * 4 * N, n, b->pk, b->pk_q, b->d_haspk, b->pout,
+                                   b->pout_q, b->pack_q2, ctx->tables, s));
   }
   return SP_OK;
 }
@@ -1001,7 +1152,7 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_graph_launch: n = %d, capacity %d", n, b->cap);
   if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_graph_launch: nothing loaded");
   const int packed = slots_packed(b, n);
-  const int gkey = n | (packed << 20);  // packed slots add the conversion to the graph
+  const int gkey = n | (packed << 20) | (b->pack_q2 << 21);  // conversion and packing change the graph
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   hipGraphExec_t exec = nullptr;
   for (auto &ge : b->graphs)
@@ -1105,8 +1256,8 @@ extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
     if (key[i] > 0 && g->b[i]->fold_mode < 0) return fail(SP_ERR_STATE, "sp_group_launch: batch %d not loaded", i);
     active += key[i] > 0;
   }
-  for (int i = 0; i < count; i++)  // the graph also depends on which batches convert packed queries
-    key.push_back(key[i] > 0 ? slots_packed(g->b[i], key[i]) : 0);
+  for (int i = 0; i < count; i++)  // the graph also depends on conversion and packing per batch
+    key.push_back(key[i] > 0 ? slots_packed(g->b[i], key[i]) | (g->b[i]->pack_q2 << 1) : 0);
   if (!active) return SP_OK;
   HIP_TRY(hipSetDevice(g->b[0]->db->ctx->device));
   hipGraphExec_t exec = nullptr;
@@ -1318,6 +1469,16 @@ extern "C" int sp_query_load(sp_query *q, const uint32_t *host_query, const uint
 extern "C" int sp_query_load_packed(sp_query *q, const uint32_t *host_query, sp_keys *keys) {
   if (!q || !host_query || !keys) return fail(SP_ERR_ARG, "sp_query_load_packed: NULL");
   return sp_batch_load_packed(q->b, 0, 1, &host_query, &keys);
+}
+
+extern "C" int sp_query_set_packing(sp_query *q, int q2_bits) {
+  if (!q) return fail(SP_ERR_ARG, "sp_query_set_packing: NULL");
+  return sp_batch_set_packing(q->b, q2_bits);
+}
+
+extern "C" int sp_query_read_packed(sp_query *q, uint8_t *host_out) {
+  if (!q) return fail(SP_ERR_ARG, "sp_query_read_packed: NULL");
+  return sp_batch_read_packed(q->b, 1, host_out);
 }
 
 extern "C" int sp_query_run(sp_query *q, int stages, void *stream) {

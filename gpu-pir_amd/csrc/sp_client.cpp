@@ -219,7 +219,22 @@ struct sp_client {
   std::vector<uint32_t> s_ntt;   // [m][N]
   std::vector<double> cdt;       // discrete Gaussian CDF over [-B, B]
   int gauss_bound = 0;
-  explicit sp_client(uint64_t seed) : rng(seed) {}
+  // Matrix secret of the packed response (n ternary polynomials), drawn on
+  // first use from a stream of its own so the other keys do not depend on it.
+  uint64_t seed;
+  int n_mat = 0;                   // n with n * n = n2, 0 if n2 is not a square
+  std::vector<int32_t> s_mat;      // [n][N]
+  std::vector<uint32_t> s_mat_ntt; // [n][m][N]
+  explicit sp_client(uint64_t sd) : rng(sd), seed(sd) {}
+
+  void make_matrix_secret() {
+    if (!s_mat.empty()) return;
+    ChaCha r2(seed ^ 0x6d61747269787365ull);
+    s_mat.resize((size_t)n_mat * N);
+    s_mat_ntt.resize((size_t)n_mat * 2 * N);
+    for (auto &x : s_mat) x = (int32_t)r2.below(3) - 1;
+    for (int i = 0; i < n_mat; i++) small_to_ntt(s_mat.data() + (size_t)i * N, s_mat_ntt.data() + (size_t)i * 2 * N);
+  }
 
   int32_t gauss() {
     const double u = rng.unit();
@@ -273,6 +288,8 @@ extern "C" int sp_client_create(int nu1, int nu2, int n2, int t_gsw, int t_exp, 
   if (g > LOGN) return cfail(SP_ERR_ARG, "sp_client_create: expansion depth exceeds log2 N");
   sp_client *c = new sp_client(seed);
   c->nu1 = nu1; c->nu2 = nu2; c->n2 = n2; c->t_gsw = t_gsw; c->t_exp = t_exp; c->g = g; c->stop = stop;
+  for (int k = 1; k * k <= n2; k++)
+    if (k * k == n2) c->n_mat = k;
   const double sigma = 6.4;
   c->gauss_bound = (int)std::ceil(12 * sigma);
   double total = 0;
@@ -447,6 +464,105 @@ extern "C" int sp_client_query_packed(sp_client *c, uint64_t index, uint32_t *ho
   std::vector<uint32_t> mu_ntt(2 * N);
   raw_to_ntt(mu.data(), mu_ntt.data());
   c->encrypt(host_query, mu_ntt.data());
+  return SP_OK;
+}
+
+// Packing keys of the packed response (Spiral's response packing; the
+// reference has none): for r < n, W_r is an (n+1) x t_conv matrix in NTT form,
+// u32 [r][m][k][row][N], standard form.  Column k: row 0 = a_k uniform,
+// row 1 + i = s~_i a_k + e_ik - [i == r] s 2^(bits k), with s~ the matrix
+// secret and bits = ceil(56 / t_conv).  Then for a scalar ciphertext (a, b)
+// of mu under s, W_r g^-1(a) + b e_{1+r} decrypts under S = [-s~ | I] to
+// mu on row r.
+extern "C" int sp_client_packing_keys(sp_client *c, int t_conv, uint32_t *host_w) {
+  if (!c || !host_w) return cfail(SP_ERR_ARG, "sp_client_packing_keys: NULL");
+  if (!c->n_mat) return cfail(SP_ERR_ARG, "sp_client_packing_keys: n2 is not a square");
+  if (t_conv < 2 || t_conv > 28) return cfail(SP_ERR_ARG, "sp_client_packing_keys: t_conv out of range");
+  c->make_matrix_secret();
+  const int n = c->n_mat, bits = gadget_bits(t_conv);
+  std::vector<int32_t> e(N);
+  std::vector<uint32_t> e_ntt(2 * N);
+  for (int r = 0; r < n; r++)
+    for (int k = 0; k < t_conv; k++) {
+      std::vector<uint32_t> a(2 * N);
+      for (int m = 0; m < 2; m++)
+        for (int z = 0; z < N; z++) a[m * N + z] = c->rng.below(QM[m]);
+      for (int row = 0; row <= n; row++) {
+        if (row > 0) {
+          for (int z = 0; z < N; z++) e[z] = c->gauss();
+          small_to_ntt(e.data(), e_ntt.data());
+        }
+        for (int m = 0; m < 2; m++) {
+          const uint64_t q = QM[m];
+          uint32_t *o = host_w + ((((size_t)r * 2 + m) * t_conv + k) * (n + 1) + row) * N;
+          if (row == 0) {
+            memcpy(o, a.data() + m * N, N * 4);
+            continue;
+          }
+          const int i = row - 1;
+          const uint64_t gk = powmod(2, (uint64_t)bits * k, q);
+          for (int z = 0; z < N; z++) {
+            uint64_t v = (uint64_t)c->s_mat_ntt[((size_t)i * 2 + m) * N + z] * a[m * N + z] % q;
+            v = (v + e_ntt[m * N + z]) % q;
+            if (i == r) v = (v + q - (uint64_t)c->s_ntt[m * N + z] * gk % q) % q;
+            o[z] = (uint32_t)v;
+          }
+        }
+      }
+    }
+  return SP_OK;
+}
+
+// Decodes a packed, modulus-switched response (layout: the n row-0 polys at
+// q2_bits per coefficient, then poly (i, c) of rows 1..n at index (i-1) n + c
+// at 10 bits = log2(4 p); coefficient k at bits [k b, (k+1) b), LSB first)
+// into the record's n2 * 2048 bytes (trial t = i n + c):
+//   u = round(2^10 (s~_i * C0_c mod 2^q2_bits) / 2^q2_bits),
+//   record[i n + c] = round(((C_{1+i, c} - u) mod 2^10) / 4) mod 256.
+extern "C" int sp_client_decode_packed(sp_client *c, const uint8_t *host_response, size_t bytes, int q2_bits,
+                                       uint8_t *host_record) {
+  if (!c || !host_response || !host_record) return cfail(SP_ERR_ARG, "sp_client_decode_packed: NULL");
+  if (!c->n_mat) return cfail(SP_ERR_ARG, "sp_client_decode_packed: n2 is not a square");
+  if (q2_bits < 11 || q2_bits > 32) return cfail(SP_ERR_ARG, "sp_client_decode_packed: q2_bits out of range");
+  const int n = c->n_mat, q1_bits = 10;
+  const size_t want = (size_t)n * N * q2_bits / 8 + (size_t)n * n * N * q1_bits / 8;
+  if (bytes != want) return cfail(SP_ERR_ARG, "sp_client_decode_packed: response size");
+  c->make_matrix_secret();
+  auto unpack = [&](const uint8_t *src, int b, std::vector<uint64_t> &out) {
+    out.assign(N, 0);
+    for (int k = 0; k < N; k++) {
+      uint64_t v = 0;
+      for (int j = 0; j < b; j++) {
+        const size_t pos = (size_t)k * b + j;
+        v |= (uint64_t)((src[pos >> 3] >> (pos & 7)) & 1) << j;
+      }
+      out[k] = v;
+    }
+  };
+  const uint8_t *rows = host_response + (size_t)n * N * q2_bits / 8;
+  const uint64_t mask2 = (q2_bits == 64) ? ~0ull : ((1ull << q2_bits) - 1);
+  std::vector<uint64_t> c0, ci;
+  std::vector<uint32_t> c0_ntt(2 * N), prod(2 * N);
+  for (int col = 0; col < n; col++) {
+    unpack(host_response + (size_t)col * N * q2_bits / 8, q2_bits, c0);
+    raw_to_ntt(c0.data(), c0_ntt.data());
+    for (int i = 0; i < n; i++) {
+      // s~_i * C0 as an exact integer polynomial (|coefficients| < N 2^32 < Q/2), via the NTT mod Q
+      for (int m = 0; m < 2; m++)
+        for (int z = 0; z < N; z++)
+          prod[m * N + z] = (uint32_t)((uint64_t)c->s_mat_ntt[((size_t)i * 2 + m) * N + z] * c0_ntt[m * N + z] % QM[m]);
+      ntt().inv(prod.data(), 0);
+      ntt().inv(prod.data() + N, 1);
+      unpack(rows + (size_t)(i * n + col) * N * q1_bits / 8, q1_bits, ci);
+      for (int z = 0; z < N; z++) {
+        const uint64_t x = crt(prod[z], prod[N + z]);
+        const uint64_t t = (x > QQ / 2 ? (uint64_t)(x - QQ) : x) & mask2;  // mod 2^q2_bits (two's complement)
+        const uint64_t u = ((t + (1ull << (q2_bits - q1_bits - 1))) >> (q2_bits - q1_bits)) & 1023u;
+        const uint64_t v = (ci[z] + 1024 - u) & 1023u;
+        host_record[(size_t)(i * n + col) * N + z] = (uint8_t)(((v + 2) >> 2) & 255u);
+      }
+    }
+  }
   return SP_OK;
 }
 

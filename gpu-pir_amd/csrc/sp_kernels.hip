@@ -4,6 +4,8 @@
 // canonical residue (or a canonical raw value in [0, Q]) so results compare
 // bit for bit with the reference semantics.  See sp_device.hpp for the NTT
 // schedule and the internal NTT-domain layout P.
+#include <cmath>
+
 #include "sp_kernels.hpp"
 
 namespace sp {
@@ -1253,6 +1255,170 @@ hipError_t launch_regev_to_gsw(bool mont, const uint32_t *v, size_t v_q, int n_q
     k_regev_to_gsw<true><<<grid, 512, 0, s>>>(v, v_q, t_gsw, conv, conv_q, fk, fk_q, only, t.tw, t.mc[0], t.mc[1]);
   else
     k_regev_to_gsw<false><<<grid, 512, 0, s>>>(v, v_q, t_gsw, conv, conv_q, fk, fk_q, only, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+// ---- response packing + modulus switching ---------------------------------
+// Spiral's response path (not in the reference): the n^2 = 4 scalar Regev
+// responses (trial t = r n + c, raw) of a query become one 3 x 2 matrix
+// ciphertext; column c = from_ntt(sum_r W_r NTT(g^-1(a_{2r+c}))) + (0, b_c,
+// b_{2+c}) with 4 digits of 14 bits (rdim 1), W_r the packing keys
+// [r][m][k][row][P] (2^32-scaled).  Row 0 is rescaled to 2^q2_bits, rows 1-2
+// to 2^10 = 4 p, and all three polynomials of the column are bit-packed into
+// the compact response.  Block (c, query): wave 4 r + d owns digit d of
+// a_{2r+c}; after the per-residue transforms and multiply-accumulates, waves
+// 0-5 take the six (row, residue) sums out of the NTT domain.
+constexpr int PACK_N = 2;      // plaintext matrix dimension n (n^2 = 4 sub-databases)
+constexpr int PACK_ROWS = PACK_N + 1;
+constexpr int PACK_Q1_BITS = 10;
+
+// floor((x 2^bits + floor(Q/2)) / Q) mod 2^bits for x < Q, bits <= 32, exactly:
+// a double-precision estimate (off by at most 2) corrected with 128-bit products.
+__device__ __forceinline__ uint32_t switch_modulus(uint64_t x, int bits, double scale) {
+  typedef unsigned __int128 u128;
+  const u128 num = ((u128)x << bits) + (QQ >> 1);
+  uint64_t est = (uint64_t)((double)x * scale);
+  u128 prod = (u128)est * QQ;
+  for (int i = 0; i < 3 && prod > num; i++) {
+    est--;
+    prod -= QQ;
+  }
+  for (int i = 0; i < 3 && prod + QQ <= num; i++) {
+    est++;
+    prod += QQ;
+  }
+  return (uint32_t)(est & ((1ull << bits) - 1));
+}
+
+template <int M>
+__device__ __forceinline__ void pack_residue(const uint32_t (&dp)[16], uint32_t (*tiles)[LDS_TILE],
+                                             const uint32_t *__restrict__ wk, uint32_t (&res)[PACK_ROWS][N / 512],
+                                             const uint32_t *__restrict__ tw_base, const ModConst &mc, int wave,
+                                             int lane, int tid) {
+  constexpr int PPT = N / 512;
+  uint32_t r[32];
+#pragma unroll
+  for (int i = 0; i < 32; i++) r[i] = (dp[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+  ntt_fwd_wave_t<8, true>(r, tw_base + (M * 2) * TW_WORDS, tiles[wave], mc.q, mc.mu1, lane);
+  store_tile_p_full(tiles[wave], r, lane, mc.q, mc.mu1);
+  __syncthreads();
+  uint64_t acc[PACK_ROWS][PPT];
+#pragma unroll
+  for (int row = 0; row < PACK_ROWS; row++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) acc[row][e] = 0;
+#pragma unroll
+  for (int w2 = 0; w2 < 8; w2++) {
+    const int r2 = w2 >> 2, d2 = w2 & 3;
+    const uint32_t *tl = &tiles[w2][PPT * tid];
+#pragma unroll
+    for (int row = 0; row < PACK_ROWS; row++) {
+      const uint32_t *kk = wk + ((((size_t)r2 * 2 + M) * 4 + d2) * PACK_ROWS + row) * N + PPT * tid;
+#pragma unroll
+      for (int e = 0; e < PPT; e++) acc[row][e] += (uint64_t)tl[e] * kk[e];
+    }
+  }
+#pragma unroll
+  for (int row = 0; row < PACK_ROWS; row++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) res[row][e] = redc64(acc[row][e], mc.q, mc.qinv_neg);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(512) void k_pack_response(const uint32_t *__restrict__ raw, size_t raw_q,
+                                                       size_t trial_stride, const uint32_t *__restrict__ pk,
+                                                       size_t pk_q, const int *__restrict__ only,
+                                                       uint8_t *__restrict__ out, size_t out_q, int q2_bits,
+                                                       double scale2, double scale1,
+                                                       const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                       ModConst mc1) {
+  constexpr int PPT = N / 512;
+  const int c = blockIdx.x, b = blockIdx.y;
+  if (only && !only[b]) return;  // uniform over the block
+  __shared__ uint32_t tiles[8][LDS_TILE];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const uint32_t *base = raw + (size_t)b * raw_q;
+  const uint32_t *wk = pk + (size_t)b * pk_q;
+  // digit d = wave & 3 of a_{r n + c}, r = wave >> 2; coefficient 64 i + lane, two per word
+  const uint32_t *a = base + (size_t)((wave >> 2) * PACK_N + c) * trial_stride;
+  const int shift = 14 * (wave & 3);
+  uint32_t dp[16];
+#pragma unroll
+  for (int i2 = 0; i2 < 16; i2++) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int z = 64 * (2 * i2 + h) + lane;
+      const uint64_t x = ((uint64_t)a[z] << 32) | a[N + z];
+      w |= (uint32_t)((x >> shift) & 0x3FFFu) << (16 * h);
+    }
+    dp[i2] = w;
+  }
+  uint32_t res[2][PACK_ROWS][PPT];
+  pack_residue<0>(dp, tiles, wk, res[0], tw_base, mc0, wave, lane, tid);
+  pack_residue<1>(dp, tiles, wk, res[1], tw_base, mc1, wave, lane, tid);
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int row = 0; row < PACK_ROWS; row++)
+#pragma unroll
+      for (int e = 0; e < PPT; e++) tiles[row * 2 + m][PPT * tid + e] = res[m][row][e];
+  __syncthreads();
+  if (wave < 2 * PACK_ROWS) {  // poly (row, m) = tile wave: out of the NTT domain, CRT-scaled, coefficient order
+    const int m = wave & 1;
+    const ModConst mc = m ? mc1 : mc0;
+    uint32_t sv[32];
+    load_p(sv, tiles[wave], lane);
+    wave_lds_sync();
+    ntt_inv_wave(sv, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+#pragma unroll
+    for (int ii = 0; ii < 32; ii++) tiles[wave][64 * ii + lane] = canon(shoup_mul(sv[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  }
+  __syncthreads();
+  // Three polys x 256 groups of 8 coefficients: CRT, + b, rescale, bit-pack (8 * bits bits = bits bytes).
+  uint8_t *ob = out + (size_t)b * out_q;
+  for (int task = tid; task < PACK_ROWS * 256; task += 512) {
+    const int row = task >> 8, grp = task & 255;
+    const int bits = row ? PACK_Q1_BITS : q2_bits;
+    uint8_t *dst = ob + (row == 0 ? (size_t)c * N * q2_bits / 8
+                                  : (size_t)PACK_N * N * q2_bits / 8 +
+                                        (size_t)((row - 1) * PACK_N + c) * N * PACK_Q1_BITS / 8) +
+                   (size_t)grp * bits;
+    const uint32_t *bb = base + (size_t)((row > 0 ? row - 1 : 0) * PACK_N + c) * trial_stride + 2 * N;
+    uint64_t bitbuf = 0;
+    int nbits = 0, nb = 0;
+    for (int k = 0; k < 8; k++) {
+      const int z = grp * 8 + k;
+      uint64_t x = (uint64_t)tiles[2 * row][z] * Q1 + (uint64_t)tiles[2 * row + 1][z] * Q0;
+      x = (x >= QQ) ? x - QQ : x;
+      if (row) {
+        x += ((uint64_t)bb[z] << 32) | bb[N + z];
+        x = (x >= QQ) ? x - QQ : x;
+      }
+      bitbuf |= (uint64_t)switch_modulus(x, bits, row ? scale1 : scale2) << nbits;
+      nbits += bits;
+      while (nbits >= 8) {
+        dst[nb++] = (uint8_t)bitbuf;
+        bitbuf >>= 8;
+        nbits -= 8;
+      }
+    }
+  }
+}
+
+size_t packed_response_bytes(int q2_bits) {
+  return (size_t)PACK_N * N * q2_bits / 8 + (size_t)PACK_N * PACK_N * N * PACK_Q1_BITS / 8;
+}
+
+hipError_t launch_pack_response(const uint32_t *raw, size_t raw_q, size_t trial_stride, int n_queries,
+                                const uint32_t *pk, size_t pk_q, const int *only, uint8_t *out, size_t out_q,
+                                int q2_bits, const DevTables &t, hipStream_t s) {
+  const double scale2 = std::ldexp(1.0, q2_bits) / (double)QQ;
+  const double scale1 = std::ldexp(1.0, PACK_Q1_BITS) / (double)QQ;
+  k_pack_response<<<dim3(PACK_N, (unsigned)n_queries), 512, 0, s>>>(raw, raw_q, trial_stride, pk, pk_q, only, out,
+                                                                     out_q, q2_bits, scale2, scale1, t.tw, t.mc[0],
+                                                                     t.mc[1]);
   return hipGetLastError();
 }
 

@@ -49,6 +49,14 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
 // rows of 4N words at src + r * src_stride -> contiguous dst (response gather for readout).
 hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s);
 
+// Response packing (n = 2, t_conv = 4) + modulus switching: per query, raw
+// responses of the 4 trials at raw + trial * trial_stride, packing keys
+// [r][m][k][row][P] (2^32-scaled); out: packed_response_bytes(q2_bits) bytes.
+size_t packed_response_bytes(int q2_bits);
+hipError_t launch_pack_response(const uint32_t *raw, size_t raw_q, size_t trial_stride, int n_queries,
+                                const uint32_t *pk, size_t pk_q, const int *only, uint8_t *out, size_t out_q,
+                                int q2_bits, const DevTables &t, hipStream_t s);
+
 // Regev -> GSW conversion (t_conv = 4): GSW slots k < nu2 t_gsw of every query,
 // v [query][>= 2 nu2 t_gsw][4N] layout P; conv [query][m][8][row][P] 2^32-scaled;
 // fk [query][nu2][m][2 t_gsw][row][P], 2^32-scaled when mont, else standard.

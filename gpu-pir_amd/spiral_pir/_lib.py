@@ -51,6 +51,8 @@ SIGNATURES = {
     "sp_expansion_depth": [I, I, I, PI, PI],
     "sp_expand": [P, P, I, I, I, I, P, P, I, P],
     "sp_regev_to_gsw": [P, P, SZ, I, I, I, P, P, P],
+    "sp_packed_response_bytes": [I, I, PSZ],
+    "sp_pack_response": [P, P, I, P, I, I, P, P],
     "sp_db_create": [P, I, I, I, PP],
     "sp_db_load_bytes": [P, P],
     "sp_db_load_ntt": [P, P],
@@ -58,6 +60,8 @@ SIGNATURES = {
     "sp_query_create": [P, I, I, PP],
     "sp_query_load": [P, P, P, P, P, P],
     "sp_query_load_packed": [P, P, P],
+    "sp_query_set_packing": [P, I],
+    "sp_query_read_packed": [P, P],
     "sp_query_run": [P, I, P],
     "sp_query_read": [P, I, P],
     "sp_query_destroy": [P],
@@ -75,6 +79,9 @@ SIGNATURES = {
     "sp_keys_create": [P, I, I, P, P, PP],
     "sp_keys_destroy": [P],
     "sp_keys_set_conversion": [P, I, P],
+    "sp_keys_set_packing": [P, I, P],
+    "sp_batch_set_packing": [P, I],
+    "sp_batch_read_packed": [P, I, P],
     "sp_batch_load_many": [P, I, I, P, P, P],
     "sp_batch_load_packed": [P, I, I, P, P],
     "sp_group_create": [P, I, PP],
@@ -91,6 +98,8 @@ SIGNATURES = {
     "sp_client_query_packed": [P, U64, P],
     "sp_client_phase": [P, P, P],
     "sp_client_secret": [P, P],
+    "sp_client_packing_keys": [P, I, P],
+    "sp_client_decode_packed": [P, P, SZ, I, P],
 }
 _RET = {
     "sp_last_error": (ctypes.c_char_p, []),

@@ -68,6 +68,21 @@ class Client:
         _lib.call("sp_client_query_packed", self._h, ctypes.c_uint64(index), _ptr(q))
         return q.reshape(2, 2, -1)
 
+    def packing_keys(self) -> np.ndarray:
+        """Response packing keys u32 [n][2][t_conv][n+1][N] (n * n = n2)."""
+        s = self.shape
+        out = np.zeros(s.packing_words, np.uint32)
+        _lib.call("sp_client_packing_keys", self._h, T_CONV, _ptr(out))
+        return out.reshape(s.pack_n, 2, T_CONV, s.pack_n + 1, -1)
+
+    def decode_packed(self, response: np.ndarray, q2_bits: int) -> np.ndarray:
+        """Compact (packed, modulus-switched) response bytes -> record bytes [n2][N]."""
+        s = self.shape
+        resp = np.ascontiguousarray(response, dtype=np.uint8)
+        out = np.zeros((s.n2, 2048), np.uint8)
+        _lib.call("sp_client_decode_packed", self._h, _ptr(resp), ctypes.c_size_t(resp.size), q2_bits, _ptr(out))
+        return out
+
     def phase(self, ct: np.ndarray) -> np.ndarray:
         """b - a*s mod Q (u64 [N], coefficient order) of one NTT Ciphertext [2][2][N]."""
         c = np.ascontiguousarray(ct, dtype=np.uint32)

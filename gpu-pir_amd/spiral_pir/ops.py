@@ -100,3 +100,13 @@ def regev_to_gsw(ctx: Context, v, conversion, v_folding_out, nu2: int, t_gsw: in
     regev_to_gsw a stub (cpu_version/conversion.py:3)."""
     _lib.call("sp_regev_to_gsw", ctx.handle, _dev(v, "v"), v.numel() // (4 * 2048), nu2, t_gsw, T_CONV,
               _dev(conversion, "conversion"), _dev(v_folding_out, "v_folding_out"), _stream())
+
+
+def pack_response(ctx: Context, raw_cts, packing_keys, out, q2_bits: int) -> None:
+    """Response packing + modulus switching of n^2 = 4 raw Ciphertexts (contiguous)
+    with packing keys u32 [n][2][T_CONV][n+1][N] into the compact response `out`
+    (uint8 tensor of sp_packed_response_bytes bytes).  Not in the reference."""
+    if not out.is_cuda or not out.is_contiguous():
+        raise ValueError("out must be a contiguous GPU tensor")
+    _lib.call("sp_pack_response", ctx.handle, _dev(raw_cts, "raw_cts"), raw_cts.numel() // (4 * 2048),
+              _dev(packing_keys, "packing_keys"), T_CONV, q2_bits, ctypes.c_void_p(out.data_ptr()), _stream())

@@ -21,6 +21,8 @@ MODULUS = Q0 * Q1                   # params.cuh:40
 MOD_BITS = 56                       # params.cuh:41
 PT_MODULUS = 256                    # params.cuh:42
 T_CONV = 4                          # params.cuh:65
+PACK_Q1_BITS = 10                   # rows 1..n of a packed response: modulus 4 p = 2^10
+PACK_Q2_BITS = 20                   # row 0 of a packed response (default)
 T_GSW = 8                           # params.cuh:66
 T_EXP_LEFT = 8                      # params.cuh:67
 T_EXP_RIGHT = 8                     # params.cuh:68
@@ -99,6 +101,23 @@ class Shape:
     def conv_words(self) -> int:
         """Conversion key CiphertextHalf[2][2 t_conv] (gpu-pir/structures.cuh:176)."""
         return 2 * (2 * T_CONV) * 2 * POLY_LEN
+
+    @property
+    def pack_n(self) -> int:
+        """Plaintext matrix dimension n (n * n = n2) of a packed response."""
+        n = int(round(self.n2 ** 0.5))
+        return n if n * n == self.n2 else 0
+
+    @property
+    def packing_words(self) -> int:
+        """Response packing keys u32 [n][2][t_conv][n+1][N]."""
+        n = self.pack_n
+        return n * 2 * T_CONV * (n + 1) * POLY_LEN
+
+    def packed_response_bytes(self, q2_bits: int) -> int:
+        """Compact response: n polys at q2_bits, n^2 polys at log2(4 p) bits per coefficient."""
+        n = self.pack_n
+        return n * POLY_LEN * q2_bits // 8 + n * n * POLY_LEN * PACK_Q1_BITS // 8
 
     @property
     def response_words(self) -> int:

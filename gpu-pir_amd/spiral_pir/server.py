@@ -130,6 +130,17 @@ class Query:
         q = np.ascontiguousarray(query, dtype=np.uint32)
         _lib.call("sp_query_load_packed", self._h, _ptr(q), keys.handle)
 
+    def set_packing(self, q2_bits: int) -> None:
+        """Response packing + modulus switching for later runs (0: off)."""
+        _lib.call("sp_query_set_packing", self._h, q2_bits)
+        self.q2_bits = q2_bits
+
+    def read_packed(self) -> np.ndarray:
+        """The compact response (needs set_packing and keys with packing keys)."""
+        out = np.zeros(self.db.shape.packed_response_bytes(self.q2_bits), np.uint8)
+        _lib.call("sp_query_read_packed", self._h, _ptr(out))
+        return out
+
     def run(self, stages: int = STAGES_ALL, stream=None) -> None:
         _lib.call("sp_query_run", self._h, stages, ctypes.c_void_p(stream or 0))
 
@@ -233,6 +244,17 @@ class Batch:
         ka = (P * n)(*[k.handle.value for k in keys])
         _lib.call("sp_batch_load_packed", self._h, first, n, qa, ka)
 
+    def set_packing(self, q2_bits: int) -> None:
+        """Response packing + modulus switching for later runs (0: off)."""
+        _lib.call("sp_batch_set_packing", self._h, q2_bits)
+        self.q2_bits = q2_bits
+
+    def packed_responses(self, n: int) -> np.ndarray:
+        """Compact responses of slots [0, n): uint8 [n][bytes]."""
+        out = np.zeros((n, self.db.shape.packed_response_bytes(self.q2_bits)), np.uint8)
+        _lib.call("sp_batch_read_packed", self._h, n, _ptr(out))
+        return out
+
     def read(self, slot: int, what: int = READ_RESPONSE) -> np.ndarray:
         s = self.db.shape
         shapes = {READ_RESPONSE: (s.n2, 2, 2, 2048), READ_EXPANDED: (1 << s.g, 2, 2, 2048),
@@ -262,7 +284,8 @@ class Keys:
     """One client's expansion keys (and, for packed queries, its conversion key),
     resident on the device for one database shape."""
 
-    def __init__(self, db: Database, left: np.ndarray, right: np.ndarray, conv: np.ndarray | None = None):
+    def __init__(self, db: Database, left: np.ndarray, right: np.ndarray, conv: np.ndarray | None = None,
+                 packing: np.ndarray | None = None):
         s = db.shape
         self.db = db
         h = ctypes.c_void_p()
@@ -277,6 +300,12 @@ class Keys:
                 self.close()
                 raise ValueError("Keys: conversion key of the wrong size")
             _lib.call("sp_keys_set_conversion", h, T_CONV, _ptr(cv))
+        if packing is not None:
+            pw = np.ascontiguousarray(packing, dtype=np.uint32)
+            if pw.size != s.packing_words:
+                self.close()
+                raise ValueError("Keys: packing keys of the wrong size")
+            _lib.call("sp_keys_set_packing", h, T_CONV, _ptr(pw))
 
     @property
     def handle(self):

@@ -13,6 +13,9 @@ run on the device at the same time, so queries that arrive while others are
 being answered are admitted as soon as a buffer frees up, and their work
 overlaps on the GPU.
 
+With `pack_q2_bits` set, every response is packed and modulus-switched on the
+device (Spiral's compact response) and requests complete with its bytes.
+
 Admission is by free device memory: the slot capacity of each lane is sized at
 start-up from the free HBM (hipMemGetInfo) and the device bytes one slot needs;
 requests beyond the free slots wait in the lane's queue.
@@ -93,13 +96,16 @@ class Request:
 
 
 class _Lane:
-    def __init__(self, name: str, db: Database, capacity: int, buffers: int):
+    def __init__(self, name: str, db: Database, capacity: int, buffers: int, pack_q2_bits: int = 0):
         import torch
 
         self.name = name
         self.db = db
         self.capacity = capacity
         self.batches = [Batch(db, capacity) for _ in range(buffers)]
+        if pack_q2_bits:
+            for b in self.batches:
+                b.set_packing(pack_q2_bits)
         self.streams = [torch.cuda.Stream() for _ in range(buffers)]
         self.free = collections.deque(range(buffers))
         self.inflight: dict[int, list[Request]] = {}
@@ -125,7 +131,7 @@ class Server:
 
     def __init__(self, dbs: dict[str, Database], max_batch: int = 128, buffers: int = 2,
                  hbm_fraction: float = 0.8, weights: dict[str, float] | None = None,
-                 poll_interval: float = 1e-4):
+                 poll_interval: float = 1e-4, pack_q2_bits: int = 0):
         import torch
 
         free, total = torch.cuda.mem_get_info()
@@ -141,9 +147,10 @@ class Server:
             if cap < 1:
                 raise MemoryError(f"lane {name}: {share / 2**30:.1f} GiB of free HBM does not hold one slot "
                                   f"({per_slot / 2**20:.0f} MiB) per buffer")
-            self.lanes[name] = _Lane(name, db, cap, buffers)
+            self.lanes[name] = _Lane(name, db, cap, buffers, pack_q2_bits)
             self.admission["lanes"][name] = {"capacity": cap, "buffers": buffers, "slot_bytes": per_slot}
         self.poll_interval = poll_interval
+        self.pack_q2_bits = pack_q2_bits
         self._queue: collections.deque[Request] = collections.deque()
         self._lock = threading.Lock()
         self._wake = threading.Event()
@@ -152,11 +159,14 @@ class Server:
         self._error: BaseException | None = None
 
     # ---- set-up ---------------------------------------------------------------
-    def register(self, lane: str, client, left: np.ndarray, right: np.ndarray, conv: np.ndarray | None = None) -> None:
-        """Make a client's expansion keys (and conversion key, for packed queries)
-        resident for `lane`."""
+    def register(self, lane: str, client, left: np.ndarray, right: np.ndarray, conv: np.ndarray | None = None,
+                 packing: np.ndarray | None = None) -> None:
+        """Make a client's expansion keys (and conversion key, for packed queries,
+        and packing keys, for compact responses) resident for `lane`."""
         ln = self.lanes[lane]
-        ln.keys[client] = Keys(ln.db, left, right, conv)
+        if self.pack_q2_bits and packing is None:
+            raise ValueError("this server packs responses: register the client's packing keys")
+        ln.keys[client] = Keys(ln.db, left, right, conv, packing)
 
     def warmup(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray | None = None) -> None:
         """Fill every buffer of `lane` with copies of one query (packed when
@@ -294,7 +304,7 @@ class Server:
             if not batch.poll():
                 continue
             reqs = ln.inflight.pop(buf)
-            resp = batch.responses(len(reqs))
+            resp = batch.packed_responses(len(reqs)) if self.pack_q2_bits else batch.responses(len(reqs))
             t = time.perf_counter()
             for i, r in enumerate(reqs):
                 r.response = resp[i]

@@ -45,6 +45,8 @@ extern "C" {
 
 #define SP_POLY_LEN 2048
 #define SP_T_CONV 4        /* conversion gadget digits (gpu-pir/params.cuh:65) */
+#define SP_PACK_N 2        /* plaintext matrix dimension of response packing (n^2 = 4 sub-databases) */
+#define SP_PACK_Q1_BITS 10 /* rows 1..n of a packed response are rescaled to 4 p = 2^10 */
 #define SP_Q1 268369921u
 #define SP_Q2 249561089u
 
@@ -158,6 +160,24 @@ int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_gsw, int t_e
 int sp_regev_to_gsw(sp_ctx *ctx, const uint32_t *dev_v, size_t n_ct, int nu2, int t_gsw, int t_conv,
                     const uint32_t *dev_conv, uint32_t *dev_v_folding, void *stream);
 
+/* ---- response packing + modulus switching -----------------------------------
+ * Spiral's response path; the reference has none (it stops at the folded
+ * ciphertexts).  The n^2 = 4 raw response ciphertexts of one query (trial
+ * t = r n + c) become one (n+1) x n matrix ciphertext: column c is
+ * from_ntt(sum_r W_r * NTT(g^-1(a_{rn+c}))) (t_conv digits of 14 bits) plus
+ * b_{rn+c} on row 1 + r, W_r the client's packing keys.  Row 0 is then
+ * rescaled to 2^q2_bits and rows 1..n to 2^10 (x -> floor((x 2^bits +
+ * floor(Q/2)) / Q) mod 2^bits) and bit-packed: the n row-0 polynomials, then
+ * poly (i, c) of rows 1..n at index (i-1) n + c; coefficient k of a poly at
+ * bits [k bits, (k+1) bits), LSB first.  Size: sp_packed_response_bytes.
+ *   dev_raw  n2 raw Ciphertexts (contiguous)
+ *   dev_w    packing keys u32 [n][2 crt][t_conv][n+1][2048], standard form
+ *   dev_out  the compact response (device bytes)
+ * Needs n2 = 4, t_conv = SP_T_CONV, 11 <= q2_bits <= 32. */
+int sp_packed_response_bytes(int n2, int q2_bits, size_t *bytes);
+int sp_pack_response(sp_ctx *ctx, const uint32_t *dev_raw, int n2, const uint32_t *dev_w, int t_conv,
+                     int q2_bits, uint8_t *dev_out, void *stream);
+
 /* ---- resident database and answer path -------------------------------------
  * A database of 2^(nu1+nu2) records, each n2 polynomials of 2048 plaintext
  * bytes (n2 = n*n sub-databases; record index = i1 * 2^nu2 + i2).  It is
@@ -238,6 +258,16 @@ int sp_keys_destroy(sp_keys *k);
  * form, host memory; t_conv = SP_T_CONV) for packed queries.  The reference
  * keeps it in PublicParameters.conversion (gpu-pir/structures.cuh:176). */
 int sp_keys_set_conversion(sp_keys *k, int t_conv, const uint32_t *host_conv);
+/* Adds the client's response packing keys (u32 [n][2][t_conv][n+1][2048],
+ * standard form, host memory).  No reference counterpart. */
+int sp_keys_set_packing(sp_keys *k, int t_conv, const uint32_t *host_w);
+/* Response packing for later runs of a batch (q2_bits 0: off).  With packing
+ * on, the fold stage ends with sp_pack_response for every slot whose keys
+ * carry packing keys (loaded with sp_batch_load_many / _packed), and
+ * sp_batch_read_packed copies the compact responses of slots [0, n)
+ * ([n][sp_packed_response_bytes] bytes).  No reference counterpart. */
+int sp_batch_set_packing(sp_batch *b, int q2_bits);
+int sp_batch_read_packed(sp_batch *b, int n, uint8_t *host_out);
 /* Loads slots [first, first + n) of a batch in one go: host_queries[i] the
  * query ciphertext of slot first + i ([4N], NTT natural order), keys[i] its
  * resident keys (same database and gadget as the batch; copied device to
@@ -254,8 +284,11 @@ int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *hos
  * converted.  No reference counterpart (the reference's conversion is a stub). */
 int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
                          sp_keys *const *keys);
-/* A packed query into a single-query object (slot 0). */
+/* A packed query into a single-query object (slot 0); response packing and
+ * the compact response of a single query, as for batches. */
 int sp_query_load_packed(sp_query *q, const uint32_t *host_query, sp_keys *keys);
+int sp_query_set_packing(sp_query *q, int q2_bits);
+int sp_query_read_packed(sp_query *q, uint8_t *host_out);
 
 /* Several batches (each against its own database, e.g. of different shapes)
  * answered by one launch: sp_group_launch runs all stages of slots
@@ -307,6 +340,14 @@ int sp_client_query_packed(sp_client *c, uint64_t index, uint32_t *host_query);
 /* Phase b - a*s mod Q (u64[2048], coefficient order) of one NTT Ciphertext,
  * and the ternary secret (int32[2048]): for noise measurements in tests. */
 int sp_client_phase(const sp_client *c, const uint32_t *host_ct, uint64_t *host_phase);
+/* Response packing keys u32 [n][2 crt][t_conv][n+1][2048] (n * n = n2): for
+ * r < n, column k of W_r is (a_k, s~_i a_k + e_ik - [i == r] s 2^(bits k))
+ * over rows 0, 1 + i, with s~ the client's matrix secret (n ternary polys). */
+int sp_client_packing_keys(sp_client *c, int t_conv, uint32_t *host_w);
+/* Decodes a compact response (sp_pack_response layout) into the record's
+ * n2 * 2048 bytes. */
+int sp_client_decode_packed(sp_client *c, const uint8_t *host_response, size_t bytes, int q2_bits,
+                            uint8_t *host_record);
 int sp_client_secret(const sp_client *c, int32_t *host_s);
 const char *sp_client_last_error(void);
 

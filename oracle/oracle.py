@@ -62,6 +62,9 @@ def lib():
             "or_expand": ([P, I, I, I, I, P, P, I], None),
             "or_folding_neg": ([P, P, I, I], None),
             "or_regev_to_gsw": ([P, I, I, I, P, P], None),
+            "or_switch_modulus": ([ctypes.c_uint64, I], U),
+            "or_packed_response_bytes": ([I, I, I], ctypes.c_size_t),
+            "or_pack_response": ([P, I, P, I, I, I, P], None),
             "or_answer": ([P, I, I, I, I, I, P, P, P, P, P, P], None),
             "or_db_encode": ([P, P, I, I, I], None),
         }
@@ -209,6 +212,20 @@ def regev_to_gsw(v, nu2: int, t_gsw: int, t_conv: int, conv) -> np.ndarray:
     assert v.size >= 2 * nu2 * t_gsw * 4 * N and conv.size == 2 * 2 * t_conv * 2 * N
     out = np.zeros((nu2, 2, 2 * t_gsw, 2, N), np.uint32)
     lib().or_regev_to_gsw(_p(v), nu2, t_gsw, t_conv, _p(conv), _p(out))
+    return out
+
+
+def switch_modulus(x: int, bits: int) -> int:
+    return lib().or_switch_modulus(x, bits)
+
+
+def pack_response(raw_cts, n: int, w, t_conv: int, q2_bits: int, q1_bits: int = 10) -> np.ndarray:
+    """n^2 raw Ciphertexts + packing keys [n][2][t_conv][n+1][N] -> compact response bytes."""
+    raw_cts = _u32(raw_cts)
+    w = _u32(w)
+    assert raw_cts.size == n * n * 4 * N and w.size == n * 2 * t_conv * (n + 1) * N
+    out = np.zeros(lib().or_packed_response_bytes(n, q2_bits, q1_bits), np.uint8)
+    lib().or_pack_response(_p(raw_cts), n, _p(w), t_conv, q2_bits, q1_bits, _p(out))
     return out
 
 

@@ -27,6 +27,11 @@
  *   or_db_mul      brute-force re-derivation per coefficient with Python ints;
  *                  linearity; decryption of the selected record
  *   or_answer      end to end: the client decodes the stored record
+ *   or_pack_response  no reference artifact (packing and modulus switching are
+ *                  spiral-rs code outside the reference): pinned by decryption
+ *                  of the packed, rescaled response under the client's matrix
+ *                  secret, an exact brute-force re-derivation of the rounding,
+ *                  and a planted-mistake check
  *   or_regev_to_gsw  no reference artifact (the reference's conversion is a
  *                  stub): pinned by decryption of every converted column under
  *                  the client's secret (column 2j+1 -> bit * 2^(7j), column 2j
@@ -458,6 +463,86 @@ void or_regev_to_gsw(const uint32_t *v, int nu2, int t_gsw, int t_conv,
         }
     free(raw);
     free(acc);
+}
+
+/* x -> floor((x * 2^bits + floor(Q/2)) / Q) mod 2^bits: x * 2^bits / Q rounded
+ * to the nearest integer (ties up), reduced mod 2^bits. */
+uint32_t or_switch_modulus(uint64_t x, int bits) {
+    const u128 Qm = (u128)Q_[0] * Q_[1];
+    const u128 num = ((u128)x << bits) + Qm / 2;
+    const uint64_t y = (uint64_t)(num / Qm);
+    return (uint32_t)(y & ((1ull << bits) - 1));
+}
+
+size_t or_packed_response_bytes(int n, int q2_bits, int q1_bits) {
+    return (size_t)n * OR_N * q2_bits / 8 + (size_t)n * n * OR_N * q1_bits / 8;
+}
+
+static void put_bits(uint8_t *dst, const uint32_t *vals, int bits) {
+    memset(dst, 0, (size_t)OR_N * bits / 8);
+    for (int k = 0; k < OR_N; k++)
+        for (int b = 0; b < bits; b++)
+            if ((vals[k] >> b) & 1u) {
+                const size_t pos = (size_t)k * bits + b;
+                dst[pos >> 3] |= (uint8_t)(1u << (pos & 7));
+            }
+}
+
+/* Response packing + modulus switching (Spiral's response path; not in the
+ * reference).  n^2 scalar Regev ciphertexts (trial t = r n + c, raw) become
+ * one (n+1) x n matrix ciphertext whose column c is
+ *     sum_r W_r * NTT(g^-1(a_{rn+c}))   (t_conv digits of ceil(56/t_conv) bits)
+ * taken out of the NTT domain, plus b_{rn+c} on row 1 + r; W_r is the
+ * (n+1) x t_conv packing key [2 crt][t_conv][n+1][N] (NTT form, standard).
+ * Row 0 is rescaled to 2^q2_bits, rows 1..n to 2^q1_bits (or_switch_modulus)
+ * and bit-packed: row-0 polys c = 0..n-1, then poly (i, c) of rows 1..n at
+ * index (i-1) n + c; coefficient k at bits [k bits, (k+1) bits), LSB first. */
+void or_pack_response(const uint32_t *raw_cts, int n, const uint32_t *w, int t_conv,
+                      int q2_bits, int q1_bits, uint8_t *out) {
+    init();
+    const size_t ct_words = 4 * OR_N;
+    uint32_t *digits = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)t_conv * OR_N);
+    uint32_t *col = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1) * 2 * OR_N); /* [row][m][N] */
+    uint32_t raw[2 * OR_N], d[OR_N], vals[OR_N];
+    uint8_t *row1 = out + (size_t)n * OR_N * q2_bits / 8;
+    for (int c = 0; c < n; c++) {
+        memset(col, 0, sizeof(uint32_t) * (size_t)(n + 1) * 2 * OR_N);
+        for (int r = 0; r < n; r++) {
+            const uint32_t *ct = raw_cts + (size_t)(r * n + c) * ct_words;
+            const uint32_t *wr = w + (size_t)r * 2 * t_conv * (n + 1) * OR_N;
+            or_gadget_inv(ct, 1, t_conv, digits);
+            for (int k = 0; k < t_conv; k++)
+                for (int m = 0; m < 2; m++) {
+                    const uint64_t q = Q_[m];
+                    for (int z = 0; z < OR_N; z++) d[z] = (uint32_t)(digits[(size_t)k * OR_N + z] % q);
+                    or_ntt_fwd(d, m);
+                    for (int row = 0; row <= n; row++) {
+                        const uint32_t *kk = wr + (((size_t)m * t_conv + k) * (n + 1) + row) * OR_N;
+                        uint32_t *o = col + ((size_t)row * 2 + m) * OR_N;
+                        for (int z = 0; z < OR_N; z++)
+                            o[z] = (uint32_t)(((uint64_t)o[z] + (uint64_t)d[z] * kk[z]) % q);
+                    }
+                }
+        }
+        for (int row = 0; row <= n; row++) {
+            or_from_ntt(col + (size_t)row * 2 * OR_N, raw);
+            for (int z = 0; z < OR_N; z++) {
+                uint64_t x = raw_get(raw, z);
+                if (row > 0) {
+                    const uint32_t *b = raw_cts + (size_t)((row - 1) * n + c) * ct_words + 2 * OR_N;
+                    const u128 sum = (u128)x + raw_get(b, z);
+                    x = (uint64_t)(sum % ((u128)Q_[0] * Q_[1]));
+                }
+                vals[z] = or_switch_modulus(x, row ? q1_bits : q2_bits);
+            }
+            if (row == 0)
+                put_bits(out + (size_t)c * OR_N * q2_bits / 8, vals, q2_bits);
+            else
+                put_bits(row1 + (size_t)((row - 1) * n + c) * OR_N * q1_bits / 8, vals, q1_bits);
+        }
+    }
+    free(digits);
+    free(col);
 }
 
 void or_answer(const uint32_t *db, int n2, int nu1, int nu2, int t_gsw, int t_exp,

@@ -33,6 +33,7 @@
  */
 #ifndef SPIRAL_ORACLE_H
 #define SPIRAL_ORACLE_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -114,6 +115,16 @@ void or_folding_neg(const uint32_t *v_folding, uint32_t *v_folding_neg,
  * conv: CiphertextHalf[2 crt][2 t_conv]; v_folding: CiphertextHalf[nu2][2][2 t_gsw]. */
 void or_regev_to_gsw(const uint32_t *v, int nu2, int t_gsw, int t_conv,
                      const uint32_t *conv, uint32_t *v_folding);
+
+/* Response packing + modulus switching (Spiral's response path, not in the
+ * reference): n^2 raw Ciphertexts (trial t = r n + c) and packing keys
+ * W [n][2 crt][t_conv][n+1][N] -> compact bytes (or_packed_response_bytes):
+ * column c = from_ntt(sum_r W_r NTT(g^-1(a_{rn+c}))) + (0, b_{c}, b_{n+c}, ...),
+ * row 0 rescaled to 2^q2_bits, rows 1..n to 2^q1_bits, bit-packed LSB first. */
+uint32_t or_switch_modulus(uint64_t x, int bits);
+size_t or_packed_response_bytes(int n, int q2_bits, int q1_bits);
+void or_pack_response(const uint32_t *raw_cts, int n, const uint32_t *w, int t_conv,
+                      int q2_bits, int q1_bits, uint8_t *out);
 
 /* Whole answer for one query against an n^2-trial database:
  *   db   [n2][2^nu2][2^nu1] NTT Poly;  query: NTT Ciphertext;

@@ -187,3 +187,53 @@ def test_conversion_key_shape_and_determinism():
     assert ka.shape == (2, 8, 2, 2048)
     np.testing.assert_array_equal(ka, kb)
     assert (ka[0] < 268369921).all() and (ka[1] < 249561089).all()
+
+
+# ---- packed responses: packing + modulus switching ----------------------------
+# Not in the reference (spiral-rs code outside it): pinned by decryption of the
+# compact response under the client's matrix secret, by an exact re-derivation
+# of the rounding with Python integers, and by planted mistakes.
+
+def test_switch_modulus_is_exact_rounding(oracle):
+    from oracle import Q
+
+    g = rng(41)
+    xs = [0, 1, Q - 1, Q // 2, Q // 2 + 1] + [int(v) for v in g.integers(0, Q, size=300, dtype=np.uint64)]
+    for bits in (10, 11, 20, 32):
+        for x in xs:
+            want = ((x << bits) + Q // 2) // Q % (1 << bits)
+            assert oracle.switch_modulus(x, bits) == want, (x, bits)
+
+
+@pytest.mark.parametrize("q2_bits", [16, 20, 32])
+def test_packed_response_decodes(oracle, q2_bits):
+    """(6, 2), n = 2: the four folded responses packed into one 3 x 2 matrix
+    ciphertext, modulus-switched and bit-packed, decode to the record."""
+    c, recs, out, _ = _case(oracle, 6, 2, 37, seed_client=2700)
+    w = c.packing_keys()
+    resp = oracle.pack_response(out, 2, w, 4, q2_bits)
+    assert resp.size == c.shape.packed_response_bytes(q2_bits) == 2 * 2048 * q2_bits // 8 + 4 * 2048 * 10 // 8
+    np.testing.assert_array_equal(c.decode_packed(resp, q2_bits), recs[37])
+
+
+def test_packed_response_planted_mistakes_fail(oracle):
+    """Keys of another client, and a response with its two columns swapped,
+    do not decode to the record."""
+    from spiral_pir import Shape
+    from spiral_pir.client import Client
+
+    c, recs, out, _ = _case(oracle, 6, 2, 37, seed_client=2800)
+    other = Client(Shape(6, 2), 2801).packing_keys()
+    bad = oracle.pack_response(out, 2, other, 4, 20)
+    assert (c.decode_packed(bad, 20) != recs[37]).mean() > 0.9
+    swapped = out[[1, 0, 3, 2]]  # trials (r, c) -> (r, 1 - c)
+    dec = c.decode_packed(oracle.pack_response(swapped, 2, c.packing_keys(), 4, 20), 20)
+    np.testing.assert_array_equal(dec, recs[37][[1, 0, 3, 2]])
+    assert not np.array_equal(dec, recs[37])
+
+
+def test_packed_response_is_smaller():
+    from spiral_pir import Shape
+
+    s = Shape(9, 6)
+    assert s.packed_response_bytes(20) * 6 < s.response_words * 4

@@ -73,6 +73,7 @@ def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
         srv.lanes[name] = ln
     srv.admission = {}
     srv.poll_interval = 1e-4
+    srv.pack_q2_bits = 0
     srv._queue = collections.deque()
     srv._lock = threading.Lock()
     srv._wake = threading.Event()
