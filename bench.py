@@ -122,6 +122,69 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
             "db_mul_autotune": tuned, "decoded_all_correct": ok, "slots_decoded_wrong": bad}
 
 
+def run_full_path(torch, mods, db, shape, records, B: int, steps: int, stream, q2_bits: int = 20) -> dict:
+    """Spiral's whole server path: packed queries (fold vectors converted on the
+    device from the expanded query), expansion, database multiply, fold, response
+    packing and modulus switching to the compact response.  One query answered
+    K times (latency form), then B queries per hipGraph launch."""
+    Client, Keys, Query, Batch = mods["Client"], mods["Keys"], mods["Query"], mods["Batch"]
+    sptr = stream.cuda_stream
+    n_clients = min(8, max(B, 1))
+    clients = [Client(shape, 2300 + k) for k in range(n_clients)]
+    keys = []
+    for c in clients:
+        left, right = c.keygen()
+        keys.append(Keys(db, left, right, c.conversion_key(), c.packing_keys()))
+    idx = [int(x) for x in np.random.default_rng(2400).integers(0, shape.records, size=max(B, 1))]
+    qs = [clients[j % n_clients].query_packed(idx[j]) for j in range(max(B, 1))]
+    out = {"q2_bits": q2_bits, "response_bytes": shape.packed_response_bytes(q2_bits),
+           "raw_response_bytes": shape.response_words * 4}
+    one = Query(db)
+    one.set_packing(q2_bits)
+    one.load_packed(qs[0], keys[0])
+    for _ in range(2):
+        one.run(7, sptr)
+    torch.cuda.synchronize()
+    ok = bool(np.array_equal(clients[0].decode_packed(one.read_packed(), q2_bits), records[idx[0]]))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for k in range(steps):
+        one.run(7, sptr)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(steps)])
+    out["single"] = {"ms_per_query": float(lat.mean()), "queries_per_s": 1000.0 / float(lat.mean()),
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+    one.close()
+    if B > 0:
+        batch = Batch(db, B)
+        batch.set_packing(q2_bits)
+        batch.load_packed(0, qs[:B], [keys[j % n_clients] for j in range(B)])
+        batch.run(B, 1, sptr)
+        batch.tune(B)
+        batch.graph_launch(B, sptr)
+        batch.graph_launch(B, sptr)
+        torch.cuda.synchronize()
+        resp = batch.packed_responses(B)
+        bad = [j for j in range(B)
+               if not np.array_equal(clients[j % n_clients].decode_packed(resp[j], q2_bits), records[idx[j]])]
+        ok = ok and not bad
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            batch.graph_launch(B, sptr)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        out["batched"] = {"B": B, "ms_per_batch": ms, "queries_per_s": B * 1000.0 / ms,
+                          "slots_decoded_wrong": bad}
+        batch.close()
+    for k in keys:
+        k.close()
+    out["decoded_all_correct"] = ok
+    return out
+
+
 def _make_db(ctx, Database, Shape, nu1: int, nu2: int, seed: int):
     shape = Shape(nu1, nu2)
     records = np.random.default_rng(seed).integers(0, 256, size=(shape.records, shape.n2, 2048), dtype=np.uint8)
@@ -274,6 +337,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-rates", default="200,1000,2000", help="offered queries/s, comma separated")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--no-full-path", action="store_true",
+                    help="skip the packed-query / compact-response leg (conversion and response packing)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -286,8 +351,8 @@ def main() -> int:
     import torch
     from spiral_pir import Shape, version
     from spiral_pir.client import Client
-    from spiral_pir.server import (Batch, Context, Database, Group, Query, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD,
-                                   STAGES_ALL)
+    from spiral_pir.server import (Batch, Context, Database, Group, Keys, Query, STAGE_DB_MUL, STAGE_EXPAND,
+                                   STAGE_FOLD, STAGES_ALL)
 
     torch.cuda.init()
     if args.copy_mib > 0:
@@ -367,7 +432,11 @@ def main() -> int:
                               eager=args.batch_eager)
         correct = correct and batched["decoded_all_correct"]
     mods = {"Batch": Batch, "Client": Client, "Database": Database, "Group": Group, "Shape": Shape, "Query": Query,
-            "STAGE_EXPAND": STAGE_EXPAND}
+            "Keys": Keys, "STAGE_EXPAND": STAGE_EXPAND}
+    full_path = None
+    if not args.no_full_path:
+        full_path = run_full_path(torch, mods, db, shape, records, args.batch, args.batch_steps, stream)
+        correct = correct and full_path["decoded_all_correct"]
     group = serving = None
     if not args.no_group or not args.no_serve:
         group, dbs = run_group(torch, mods, ctx, (shape, records, db), args.group_steps, stream)
@@ -424,6 +493,7 @@ def main() -> int:
         "batched": None if batched is None else {k: v for k, v in batched.items() if k != "decoded_all_correct"},
         "group": group,
         "serving": serving,
+        "full_path": full_path,
         "ingest_s": t_ingest,
         "library": version(),
     }
