@@ -1105,7 +1105,7 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
       rc = mfma_workspace(b);
       if (rc) return rc;
       HIP_TRY(launch_dbmul_mfma(b->v, b->v_q, n, db->d_limbs, db->n2, R, C, mfma_rows(n), b->mfma_a, b->mfma_out,
-                                b->acc, b->acc_q, ctx->d_pw, ctx->tables, s));
+                                b->acc, b->acc_q, ctx->d_pw, ctx->tables, mfma_rows(n) > 64 ? 4 : 0, s));
     } else {
       HIP_TRY(launch_db_mul(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, ctx->tables, s));
     }

@@ -1791,42 +1791,57 @@ namespace sp {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// Selector limbs from the expanded ciphertexts of a batch.  Block: 4 rows i
-// x 32 j x 64 positions through LDS; writes 128-byte runs.
+// Selector limbs from the expanded ciphertexts of a batch.  Block: 4 rows i x
+// 32 j x 64 positions; 16-byte loads of 4 positions into an LDS tile (rows of
+// 68 words, 16-byte aligned), then each thread packs the 128-byte chunk of one
+// (position, limb) and the chunks leave through the same LDS (rows of 33
+// words) so that every store instruction writes whole 128-byte lines (8 lanes
+// per chunk): 2.1x faster than each lane storing its own chunk.
 __global__ __launch_bounds__(256) void k_sel_to_limbs(const uint32_t *__restrict__ v, size_t v_q, int nq, int R,
                                                       int I, uint8_t *__restrict__ A) {
-  __shared__ uint32_t tile[4][32][65];
+  constexpr int PS = 68;
+  __shared__ __align__(16) uint32_t tile[4 * 32 * PS];
   const int pb = blockIdx.x;           // position block of 64
   const int jb = blockIdx.y;           // j block of 32
   const int m = blockIdx.z & 1;
   const int i0 = (blockIdx.z >> 1) * 4;
   const int JB = R / 32;
   const int tid = threadIdx.x;
-  for (int k = tid; k < 4 * 32 * 64; k += 256) {
-    const int p = k & 63, j = (k >> 6) & 31, ii = k >> 11;
+#pragma unroll
+  for (int k = tid; k < 4 * 32 * 16; k += 256) {  // (row, j, 4 positions)
+    const int p4 = k & 15, j = (k >> 4) & 31, ii = k >> 9;
     const int i = i0 + ii;
-    uint32_t x = 0;
+    uint4 x = make_uint4(0, 0, 0, 0);
     if ((i >> 1) < nq)
-      x = v[(size_t)(i >> 1) * v_q + (size_t)(2 * (jb * 32 + j)) * 4 * N + (size_t)((i & 1) * 2 + m) * N + pb * 64 + p];
-    tile[ii][j][p] = x;
+      x = *reinterpret_cast<const uint4 *>(v + (size_t)(i >> 1) * v_q + (size_t)(2 * (jb * 32 + j)) * 4 * N +
+                                           (size_t)((i & 1) * 2 + m) * N + pb * 64 + 4 * p4);
+    *reinterpret_cast<uint4 *>(&tile[(ii * 32 + j) * PS + 4 * p4]) = x;
   }
   __syncthreads();
   // 64 positions x 4 limbs x 4 rows x 32 bytes: thread -> (p, limb), 128 bytes.
-  {
-    const int p = tid & 63, limb = tid >> 6;
-    uint32_t words[32];
+  const int p = tid & 63, limb = tid >> 6;
+  uint32_t words[32];
 #pragma unroll
-    for (int ii = 0; ii < 4; ii++)
+  for (int ii = 0; ii < 4; ii++)
 #pragma unroll
-      for (int w = 0; w < 8; w++) {
-        uint32_t packed = 0;
+    for (int w = 0; w < 8; w++) {
+      uint32_t packed = 0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) packed |= ((tile[ii][w * 4 + b][p] >> (7 * limb)) & 127u) << (8 * b);
-        words[ii * 8 + w] = packed;
-      }
-    uint4 *dst = reinterpret_cast<uint4 *>(A + (((((size_t)m * N + pb * 64 + p) * JB + jb) * 4 + limb) * I + i0) * 32);
+      for (int b = 0; b < 4; b++) packed |= ((tile[(ii * 32 + w * 4 + b) * PS + p] >> (7 * limb)) & 127u) << (8 * b);
+      words[ii * 8 + w] = packed;
+    }
+  __syncthreads();  // the tile is dead: it stages the chunks, [chunk = tid][33 words]
 #pragma unroll
-    for (int u = 0; u < 8; u++) dst[u] = make_uint4(words[4 * u], words[4 * u + 1], words[4 * u + 2], words[4 * u + 3]);
+  for (int w = 0; w < 32; w++) tile[tid * 33 + w] = words[w];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const int g = u * 256 + tid, c = g >> 3, piece = g & 7;
+    const int cp = c & 63, cl = c >> 6;
+    const uint32_t *src = &tile[c * 33 + 4 * piece];
+    uint4 *dst = reinterpret_cast<uint4 *>(A + (((((size_t)m * N + pb * 64 + cp) * JB + jb) * 4 + cl) * I + i0) * 32) +
+                 piece;
+    *dst = make_uint4(src[0], src[1], src[2], src[3]);
   }
 }
 
@@ -1926,29 +1941,148 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma(const uint8_t *__restrict__ 
   }
 }
 
+// Same tile and block mapping as k_dbmul_mfma, with the operands staged through
+// LDS and loaded PD j blocks ahead.  The database limbs stream from HBM once
+// (no reuse), so their latency has to be covered by loads in flight: with one
+// block per CU and loads one j block ahead the multiply ran at the latency-bound
+// ~1.4 TB/s of database limbs.  Here each thread loads three 16-byte pieces per
+// j block (selector limbs 0-1 and 2-3 of one row, database limbs of one
+// column) into a ring of PD register slots; the block's 8 waves read their
+// fragments from a double-buffered LDS stage.  Rows past I and columns past C
+// load row / column 0 and are zeroed.
+struct MfmaPiece {
+  uint4 a0, a1, d;
+};
+
+template <int PD>
+__global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
+                                                         uint32_t *__restrict__ outp, int T, int R, int C, int I,
+                                                         int n_ig, int n_cg, ModConst mc0, ModConst mc1,
+                                                         const uint32_t *__restrict__ pw) {
+  __shared__ uint4 sA[2][4 * 128 * 2];
+  __shared__ uint4 sD[2][4 * 64 * 2];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  int bid = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+  const int t = bid % T; bid /= T;
+  const int cg = bid % n_cg; bid /= n_cg;
+  const int ig = bid % n_ig; bid /= n_ig;
+  const int p = bid % N;
+  const int m = bid / N;
+  const int JB = R / 32;
+  const int ar = (tid >> 1) & 127, al = tid >> 8, hh = tid & 1;
+  const int dl = tid >> 7, dc = (tid >> 1) & 63;
+  const bool a_ok = ig * 128 + ar < I, d_ok = cg * 64 + dc < C;
+  const uint32_t ma = a_ok ? ~0u : 0u, md = d_ok ? ~0u : 0u;
+  const size_t a_jb = (size_t)4 * I * 32, d_jb = (size_t)4 * C * 32;
+  const uint8_t *a0 = A + (((size_t)m * N + p) * JB * 4 * I + (size_t)al * I + (a_ok ? ig * 128 + ar : 0)) * 32 + 16 * hh;
+  const uint8_t *a1 = a0 + (size_t)2 * I * 32;
+  const uint8_t *d0 = D + ((((size_t)t * 2 + m) * N + p) * JB * 4 * C + (size_t)dl * C + (d_ok ? cg * 64 + dc : 0)) * 32 +
+                      16 * hh;
+  MfmaPiece ring[PD];
+#pragma unroll
+  for (int k = 0; k < PD; k++) {
+    const int jb = min(k, JB - 1);
+    ring[k].a0 = *reinterpret_cast<const uint4 *>(a0 + jb * a_jb);
+    ring[k].a1 = *reinterpret_cast<const uint4 *>(a1 + jb * a_jb);
+    ring[k].d = *reinterpret_cast<const uint4 *>(d0 + jb * d_jb);
+  }
+  auto masked = [](uint4 x, uint32_t mk) { return make_uint4(x.x & mk, x.y & mk, x.z & mk, x.w & mk); };
+  sA[0][tid] = masked(ring[0].a0, ma);
+  sA[0][tid + 512] = masked(ring[0].a1, ma);
+  sD[0][tid] = masked(ring[0].d, md);
+  v16i acc[7];
+#pragma unroll
+  for (int w = 0; w < 7; w++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[w][v] = 0;
+  const int rw = wave >> 1, cw = wave & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  for (int jb0 = 0; jb0 < JB; jb0 += PD) {
+#pragma unroll
+    for (int k = 0; k < PD; k++) {
+      const int jb = jb0 + k;
+      if (jb < JB) {
+        // slot k held block jb (already in LDS): refill it with block jb + PD
+        const int jn = min(jb + PD, JB - 1);
+        ring[k].a0 = *reinterpret_cast<const uint4 *>(a0 + jn * a_jb);
+        ring[k].a1 = *reinterpret_cast<const uint4 *>(a1 + jn * a_jb);
+        ring[k].d = *reinterpret_cast<const uint4 *>(d0 + jn * d_jb);
+        __syncthreads();
+        const uint4 *la = sA[jb & 1], *ld = sD[jb & 1];
+        v4i fa[4], fd[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          const uint4 x = la[(a * 128 + rw * 32 + l32) * 2 + h];
+          const uint4 y = ld[(a * 64 + cw * 32 + l32) * 2 + h];
+          fa[a] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+          fd[a] = v4i{(int)y.x, (int)y.y, (int)y.z, (int)y.w};
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fd[b], acc[a + b], 0, 0, 0);
+        // block jb + 1 (slot k + 1) into the other stage; never read after the last block
+        const MfmaPiece &nx = ring[(k + 1) % PD];
+        const int st = (jb + 1) & 1;
+        sA[st][tid] = masked(nx.a0, ma);
+        sA[st][tid + 512] = masked(nx.a1, ma);
+        sD[st][tid] = masked(nx.d, md);
+      }
+    }
+  }
+  const int row0 = ig * 128 + rw * 32, col0 = cg * 64 + cw * 32;
+  if (row0 >= I || col0 >= C) return;
+  const ModConst &mc = m ? mc1 : mc0;
+  const uint32_t *pwm = pw + m * 8;
+  const int c = col0 + l32;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
+    uint64_t sum = 0;
+#pragma unroll
+    for (int w = 0; w < 7; w++) sum += (uint64_t)(uint32_t)acc[w][v] * pwm[w];
+    outp[((((size_t)m * N + p) * T + t) * I + i) * C + c] = mod_u64(sum, mc);
+  }
+}
+
 // p-major products [m][p][t][I][C] -> per-query accumulators [b][t][c][row][m][P].
 __global__ __launch_bounds__(256) void k_outp_to_acc(const uint32_t *__restrict__ outp, int T, int I, int C, int nq,
                                                      uint32_t *__restrict__ acc, size_t acc_q) {
   __shared__ uint32_t tile[64][65];
+  __shared__ long long dst_of[64];                 // accumulator word of combination x (-1: none)
   const int pb = blockIdx.x;                       // 64 positions
   const int cb = blockIdx.y;                       // 64 (t, i, c) combinations
   const int m = blockIdx.z;
   const int tid = threadIdx.x;
   const int TIC = T * I * C;
-  for (int k = tid; k < 64 * 64; k += 256) {
-    const int x = k & 63, pp = k >> 6;
-    const int e = cb * 64 + x;
-    tile[pp][x] = e < TIC ? outp[((size_t)m * N + pb * 64 + pp) * TIC + e] : 0u;
+  if (tid < 64) {  // one division per combination, not per element
+    const int e = cb * 64 + tid;
+    long long o = -1;
+    if (e < TIC) {
+      const int c = e % C, i = (e / C) % I, t = e / (C * I);
+      const int b = i >> 1, row = i & 1;
+      if (b < nq) o = (long long)((size_t)b * acc_q + (((size_t)t * C + c) * 2 + row) * 2 * N + (size_t)m * N);
+    }
+    dst_of[tid] = o;
+  }
+#pragma unroll
+  for (int k = tid; k < 64 * 16; k += 256) {  // 16-byte loads: 4 combinations of one position
+    const int q = k & 15, pp = k >> 4;
+    const int e = cb * 64 + 4 * q;            // TIC is a multiple of 4
+    uint4 x = make_uint4(0, 0, 0, 0);
+    if (e < TIC) x = *reinterpret_cast<const uint4 *>(outp + ((size_t)m * N + pb * 64 + pp) * TIC + e);
+    tile[pp][4 * q] = x.x;
+    tile[pp][4 * q + 1] = x.y;
+    tile[pp][4 * q + 2] = x.z;
+    tile[pp][4 * q + 3] = x.w;
   }
   __syncthreads();
   for (int k = tid; k < 64 * 64; k += 256) {
     const int pp = k & 63, x = k >> 6;
-    const int e = cb * 64 + x;
-    if (e >= TIC) continue;
-    const int c = e % C, i = (e / C) % I, t = e / (C * I);
-    const int b = i >> 1, row = i & 1;
-    if (b >= nq) continue;
-    acc[(size_t)b * acc_q + (((size_t)t * C + c) * 2 + row) * 2 * N + (size_t)m * N + pb * 64 + pp] = tile[pp][x];
+    const long long o = dst_of[x];
+    if (o >= 0) acc[(size_t)o + pb * 64 + pp] = tile[pp][x];
   }
 }
 
@@ -1960,13 +2094,16 @@ hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *
 
 hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
                              uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
-                             const DevTables &t, hipStream_t s) {
+                             const DevTables &t, int ring, hipStream_t s) {
   k_sel_to_limbs<<<dim3(N / 64, R / 32, 2 * (I / 4)), 256, 0, s>>>(v, v_q, nq, R, I, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int n_ig = (I + 127) / 128, n_cg = (C + 63) / 64;
   const unsigned blocks = (unsigned)(2 * N * n_ig * n_cg * T);  // 2N is a multiple of 8
-  k_dbmul_mfma<<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
+  if (ring == 0)
+    k_dbmul_mfma<<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
+  else
+    k_dbmul_mfma_ring<4><<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int TIC = T * I * C;

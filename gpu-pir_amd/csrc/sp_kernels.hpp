@@ -90,8 +90,10 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
 //   A: workspace for selector limbs, 2 * N * R * 4 * I bytes (I = 2*nq padded to 32)
 //   outp: workspace, 2 * N * T * I * C words;  pw: [m][8] = 2^(7w) mod q_m
 hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *D, hipStream_t s);
+// ring: 0 = operands straight from global memory (small I: no padded rows are
+// multiplied), 4 = staged through LDS with a 4-deep prefetch ring (I > 64).
 hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
                              uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
-                             const DevTables &t, hipStream_t s);
+                             const DevTables &t, int ring, hipStream_t s);
 
 }  // namespace sp

@@ -189,9 +189,12 @@ def test_dbmul_mfma_matches_oracle(gpu_ctx, oracle, force_dbmul, nu1, nu2, B):
     db.close()
 
 
-def test_dbmul_mfma_extreme_residues(gpu_ctx, oracle, force_dbmul):
+@pytest.mark.parametrize("B", [2, 40])
+def test_dbmul_mfma_extreme_residues(gpu_ctx, oracle, force_dbmul, B):
     """A database of all-(q-1) residues (every limb at its maximum, 512 rows) through the
-    MFMA multiply: the int32 limb sums stay exact, against the oracle."""
+    MFMA multiply (B = 2: operands from global memory; B = 40, 80 selector rows: the
+    LDS-staged kernel with a partial row tile): the int32 limb sums stay exact, against
+    the oracle."""
     from spiral_pir import Shape
     from spiral_pir.server import Batch, Database, READ_DB_MUL, READ_EXPANDED, STAGE_DB_MUL, STAGE_EXPAND
 
@@ -203,16 +206,16 @@ def test_dbmul_mfma_extreme_residues(gpu_ctx, oracle, force_dbmul):
     db_ntt[:, :, :, 1] = Q1 - 1
     db = Database(gpu_ctx, s)
     db.load_ntt(db_ntt)
-    qs = _queries(s, 2, 3600)
-    b = Batch(db, 2)
+    qs = _clients_queries(s, B, 2, 3600)
+    b = Batch(db, B)
     for slot, (c, idx, q, left, right, vf) in enumerate(qs):
         b.load(slot, q, left, right, vf)
     force_dbmul(2)
-    b.run(2, STAGE_EXPAND)
-    b.run(2, STAGE_DB_MUL)
-    v = b.read(1, READ_EXPANDED)
+    b.run(B, STAGE_EXPAND)
+    b.run(B, STAGE_DB_MUL)
+    v = b.read(B - 1, READ_EXPANDED)
     sel = np.ascontiguousarray(v[0:2 * R:2])
-    acc = b.read(1, READ_DB_MUL)
+    acc = b.read(B - 1, READ_DB_MUL)
     np.testing.assert_array_equal(acc[0], oracle.db_mul(db_ntt[0], sel, s.nu1, s.nu2, nthreads=0))
     b.close()
     db.close()
