@@ -1032,36 +1032,53 @@ void k_fold8(uint32_t *__restrict__ accs, size_t set_stride, int n_trials, int n
 }
 
 // ---- split fold level: partial products + finish ----------------------------
-// k_fold_part: one 8-wave block per (pair i, trial, query, residue m, column
-// half h): digits 4h .. 4h + 3 of both polys (key columns c = 8h .. 8h + 7,
-// c = j + 2k), one transform per wave, multiply-accumulate of the 8 tiles,
-// and the Montgomery-reduced partial sums of both rows (NTT domain, layout P,
-// < q) to part[set][i][m][h][row][P] (set = query * n_trials + trial).
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+// k_fold_part<NG>: one block of NW = 16 / NG waves per (pair i, trial, query,
+// residue m, key-column group g < NG): key columns c = NW g .. NW g + NW - 1
+// (c = j + 2k: digits NW g / 2 .. of both polys), one transform per wave, a
+// multiply-accumulate of the NW tiles, and the Montgomery-reduced partial sums
+// of both rows (NTT domain, layout P, < q) to part[set][i][m][g][row][P]
+// (set = query * n_trials + trial).  NG = 2 (8-wave blocks) for grids that fill
+// the chip; NG = 4 (4-wave blocks: one transform per SIMD, twice the CUs) for
+// small grids, where the blocks' latency is the cost.
+template <int NG>
+__device__ __forceinline__ void fold_part_digits(const uint32_t *__restrict__ X, const uint32_t *__restrict__ Y,
+                                                 uint32_t *__restrict__ dig, int g, int tid) {
+  constexpr int NK = 8 / NG, NT = 16 / NG * 64;
+  if (NG == 2) {
+    if (g == 0) fold_digits<NT, 0, NK>(X, Y, dig, tid);
+    else fold_digits<NT, NK, NK>(X, Y, dig, tid);
+  } else {
+    if (g == 0) fold_digits<NT, 0, NK>(X, Y, dig, tid);
+    else if (g == 1) fold_digits<NT, NK, NK>(X, Y, dig, tid);
+    else if (g == 2) fold_digits<NT, 2 * NK, NK>(X, Y, dig, tid);
+    else fold_digits<NT, 3 * NK, NK>(X, Y, dig, tid);
+  }
+}
+
+template <int NG>
+__global__ __launch_bounds__(16 / NG * 64) __attribute__((amdgpu_waves_per_eu(4)))
 void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_trials, int num_per,
                  const uint32_t *__restrict__ cpos, size_t key_qstride, uint32_t *__restrict__ part,
                  size_t part_set_stride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
-  constexpr int PPT = 4;
-  __shared__ uint32_t tiles[8][LDS_TILE];
-  uint32_t *dig = &tiles[0][0];  // 8 digit polynomials, 4096 words over tiles 0-1
+  constexpr int NW = 16 / NG, NK = 8 / NG;
+  constexpr int PPT = N / (NW * 64);
+  __shared__ uint32_t tiles[NW][LDS_TILE];
+  uint32_t *dig = &tiles[0][0];  // 2 polys x NK digit polynomials, 1024 * NK words
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int i = blockIdx.x;
-  const int trial = blockIdx.y >> 2, m = (blockIdx.y >> 1) & 1, h = blockIdx.y & 1;
+  const int g = blockIdx.y % NG, m = (blockIdx.y / NG) & 1, trial = blockIdx.y / (2 * NG);
   const int set = blockIdx.z * n_trials + trial;
   const uint32_t *base = accs + (size_t)set * set_stride;
   const uint32_t *X = base + (size_t)(num_per + i) * 4 * N;
   const uint32_t *Y = base + (size_t)i * 4 * N;
   const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
-  if (h == 0)
-    fold_digits<512, 0, 4>(X, Y, dig, tid);
-  else
-    fold_digits<512, 4, 4>(X, Y, dig, tid);
+  fold_part_digits<NG>(X, Y, dig, g, tid);
   __syncthreads();
-  const int c = 8 * h + wave, j = c & 1, kl = (c >> 1) - 4 * h;
+  const int c = NW * g + wave, j = c & 1, kl = (c >> 1) - NK * g;
   uint32_t dw[8];
 #pragma unroll
-  for (int i4 = 0; i4 < 8; i4++) dw[i4] = dig[(j * 4 + kl) * DIG_WORDS + i4 * 64 + lane];
+  for (int i4 = 0; i4 < 8; i4++) dw[i4] = dig[(j * NK + kl) * DIG_WORDS + i4 * 64 + lane];
   __syncthreads();  // the digit bytes are dead from here on
   const uint32_t q = m ? mc1.q : mc0.q, mu1 = m ? mc1.mu1 : mc0.mu1, qinv = m ? mc1.qinv_neg : mc0.qinv_neg;
   {
@@ -1081,8 +1098,8 @@ void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_tri
 #pragma unroll
     for (int e = 0; e < PPT; e++) acc[row][e] = 0;
 #pragma unroll
-  for (int w2 = 0; w2 < 8; w2++) {
-    const uint32_t *k0 = kp + ((size_t)(m * 16 + 8 * h + w2) * 2) * N + PPT * tid;
+  for (int w2 = 0; w2 < NW; w2++) {
+    const uint32_t *k0 = kp + ((size_t)(m * 16 + NW * g + w2) * 2) * N + PPT * tid;
     const uint32_t *k1 = k0 + N;
     const uint32_t *tl = &tiles[w2][PPT * tid];
 #pragma unroll
@@ -1092,15 +1109,16 @@ void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_tri
       acc[1][e] += (uint64_t)d * k1[e];
     }
   }
-  uint32_t *dst = part + (size_t)set * part_set_stride + ((((size_t)i * 2 + m) * 2 + h) * 2) * N + PPT * tid;
+  uint32_t *dst = part + (size_t)set * part_set_stride + ((((size_t)i * 2 + m) * NG + g) * 2) * N + PPT * tid;
 #pragma unroll
   for (int row = 0; row < 2; row++)
 #pragma unroll
     for (int e = 0; e < PPT; e++) dst[(size_t)row * N + e] = redc64(acc[row][e], q, qinv);
 }
 
-// k_fold_finish: one 4-wave block per (pair i, trial, query): tile 2m + row =
-// part[m][0][row] + part[m][1][row] mod q_m, then the fold epilogue on Y.
+// k_fold_finish<NG>: one 4-wave block per (pair i, trial, query): tile 2m + row =
+// sum over g of part[m][g][row] mod q_m, then the fold epilogue on Y.
+template <int NG>
 __global__ __launch_bounds__(256) void k_fold_finish(uint32_t *__restrict__ accs, size_t set_stride, int n_trials,
                                                      const uint32_t *__restrict__ part, size_t part_set_stride,
                                                      const uint32_t *__restrict__ tw_base, ModConst mc0,
@@ -1111,15 +1129,18 @@ __global__ __launch_bounds__(256) void k_fold_finish(uint32_t *__restrict__ accs
   const int i = blockIdx.x;
   const int set = blockIdx.z * n_trials + blockIdx.y;
   uint32_t *Y = accs + (size_t)set * set_stride + (size_t)i * 4 * N;
-  const uint32_t *pp = part + (size_t)set * part_set_stride + (size_t)i * 8 * N;  // [m][h][row][P]
+  const uint32_t *pp = part + (size_t)set * part_set_stride + (size_t)i * 2 * NG * 2 * N;  // [m][g][row][P]
   for (int u = tid; u < N; u += 256) {  // 4 tiles x N/4 uint4
     const int w = u / (N / 4), k4 = u % (N / 4);
     const int m = w >> 1, row = w & 1;
     const uint32_t q = m ? mc1.q : mc0.q;
-    const uint4 a = reinterpret_cast<const uint4 *>(pp + (size_t)((m * 2 + 0) * 2 + row) * N)[k4];
-    const uint4 b = reinterpret_cast<const uint4 *>(pp + (size_t)((m * 2 + 1) * 2 + row) * N)[k4];
-    reinterpret_cast<uint4 *>(tiles[w])[k4] =
-        make_uint4(canon(a.x + b.x, q), canon(a.y + b.y, q), canon(a.z + b.z, q), canon(a.w + b.w, q));
+    uint4 a = reinterpret_cast<const uint4 *>(pp + (size_t)((m * NG + 0) * 2 + row) * N)[k4];
+#pragma unroll
+    for (int g = 1; g < NG; g++) {  // partial sums are < q: add and reduce once per term
+      const uint4 b = reinterpret_cast<const uint4 *>(pp + (size_t)((m * NG + g) * 2 + row) * N)[k4];
+      a = make_uint4(canon(a.x + b.x, q), canon(a.y + b.y, q), canon(a.z + b.z, q), canon(a.w + b.w, q));
+    }
+    reinterpret_cast<uint4 *>(tiles[w])[k4] = a;
   }
   __syncthreads();
   fold_tail(tiles, Y, tw_base, mc0, mc1, wave, lane);
@@ -1733,12 +1754,27 @@ hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_q
   // vs 0.26 ms for one query, 2.75 vs 3.17 ms for 64 queries against the
   // 16-wave blocks).
   if (part && (variant == 4 || variant == 0)) {
-    k_fold_part<<<dim3(num_per, n_trials * 4, n_queries), 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos,
-                                                                        key_qstride, part, part_set_stride, t.tw,
-                                                                        t.mc[0], t.mc[1]);
+    // Quarter blocks when even they give each CU at most one block (one
+    // transform per SIMD) and the partial sums of four groups fit the scratch
+    // (part_set_stride holds 8N words for each of the first level's pairs:
+    // four groups need 16N per pair).  At (9,6), one query: levels 2-5,
+    // 8.6-9.1 us per partial launch against 10.2-11.2 us for halves.
+    const bool quarters = 8L * pairs <= (long)t.n_cu && (size_t)num_per * 16 * N <= part_set_stride;
+    if (quarters) {
+      k_fold_part<4><<<dim3(num_per, n_trials * 8, n_queries), 256, 0, s>>>(
+          accs, set_stride, n_trials, num_per, cpos, key_qstride, part, part_set_stride, t.tw, t.mc[0], t.mc[1]);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      k_fold_finish<4><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, part, part_set_stride, t.tw, t.mc[0],
+                                            t.mc[1]);
+      return hipGetLastError();
+    }
+    k_fold_part<2><<<dim3(num_per, n_trials * 4, n_queries), 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos,
+                                                                           key_qstride, part, part_set_stride, t.tw,
+                                                                           t.mc[0], t.mc[1]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_fold_finish<<<grid, 256, 0, s>>>(accs, set_stride, n_trials, part, part_set_stride, t.tw, t.mc[0], t.mc[1]);
+    k_fold_finish<2><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, part, part_set_stride, t.tw, t.mc[0], t.mc[1]);
     return hipGetLastError();
   }
   if (fold_expand_waves(pairs, t, variant, true) == 16)
