@@ -439,9 +439,17 @@ static int expansion_family(const sp_ctx *ctx, int t_exp, const uint32_t *dig) {
   return dig ? 4 : 2;
 }
 
+// Snapshot sets of the latency-form rounds: room for `queries` queries per set.
+struct ExpandSnap {
+  uint32_t *p = nullptr;
+  size_t q_stride = 0, set_stride = 0;
+  int queries = 0;
+};
+
 static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_queries, int nu1, int nu2, int t_gsw,
                          int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t left_qstride,
-                         size_t right_qstride, int rounds, uint32_t *dig, size_t dig_qstride, hipStream_t s) {
+                         size_t right_qstride, int rounds, uint32_t *dig, size_t dig_qstride, const ExpandSnap &snap,
+                         const uint32_t *query, size_t query_qstride, hipStream_t s) {
   int g = 0, stop = 0;
   int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
   if (rc) return rc;
@@ -449,7 +457,24 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
   const size_t key_level = (size_t)2 * t_exp * 2 * N;
   const int family = expansion_family(ctx, t_exp, dig);
   if (family == 4 && !dig) return fail(SP_ERR_STATE, "split expansion without digit scratch");
-  for (int r = 0; r < rounds; r++) {
+  // Latency form for the leading rounds whose one-launch grid gives each CU at
+  // most one block: at (9,6), one query, rounds 0-6 take 16.4-18.5 us each
+  // against 21-22 us for the two split launches; round 7 (384 blocks) is
+  // slower in one launch (26 vs 25 us).
+  int fused = 0;
+  if (family == 4 && snap.p && n_queries <= snap.queries && ctx->variant[SP_STAGE_EXPAND] == 0)
+    while (fused < rounds && 2L * expand_fused_blocks(fused, stop, t_gsw * nu2) * n_queries <= (long)ctx->tables.n_cu)
+      fused++;
+  // The queries (if given) go to v[0]; with the latency form one launch also readies round 0.
+  if (fused > 0)
+    HIP_TRY(launch_expand_prep(query, query_qstride, v, q_stride, n_queries, snap.p, snap.q_stride, ctx->tables, s));
+  else if (query)
+    HIP_TRY(launch_copy_queries(query, query_qstride, v, q_stride, n_queries, s));
+  for (int r = 0; r < fused; r++)
+    HIP_TRY(launch_expand_fused(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2,
+                                left_p + (size_t)r * key_level, right_p + (size_t)std::min(r, stop) * key_level,
+                                left_qstride, right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s));
+  for (int r = fused; r < rounds; r++) {
     const int num_in = 1 << r;
     const uint32_t *neg1_r = ctx->tables.neg1 + (size_t)r * 2 * 2 * N;
     const uint32_t *lr = left_p + (size_t)r * key_level;
@@ -489,10 +514,17 @@ extern "C" int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_g
   HIP_TRY(launch_nat_to_p(dev_v, v.u32(), n_ct * 4, 1, 0, ctx->tables, s));
   HIP_TRY(launch_nat_to_p(dev_left, kl.u32(), (size_t)g * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
   HIP_TRY(launch_nat_to_p(dev_right, kr.u32(), (size_t)(stop + 1) * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
-  DevBuf dig;
-  if (t_exp == 8 && (ctx->variant[SP_STAGE_EXPAND] == 0 || ctx->variant[SP_STAGE_EXPAND] == 4))
+  DevBuf dig, snapbuf;
+  ExpandSnap snap;
+  if (t_exp == 8 && (ctx->variant[SP_STAGE_EXPAND] == 0 || ctx->variant[SP_STAGE_EXPAND] == 4)) {
     HIP_TRY(dig.alloc((size_t)expansion_max_work(nu1, nu2, t_gsw) * 8 * 512 * 4));
-  rc = run_expansion(ctx, v.u32(), 0, 1, nu1, nu2, t_gsw, t_exp, kl.u32(), kr.u32(), 0, 0, rounds, dig.u32(), 0, s);
+    snap.q_stride = snap.set_stride = n_ct * 2 * N;
+    snap.queries = 1;
+    HIP_TRY(snapbuf.alloc(2 * snap.set_stride * 4));
+    snap.p = snapbuf.u32();
+  }
+  rc = run_expansion(ctx, v.u32(), 0, 1, nu1, nu2, t_gsw, t_exp, kl.u32(), kr.u32(), 0, 0, rounds, dig.u32(), 0, snap,
+                     nullptr, 0, s);
   if (rc) return rc;
   HIP_TRY(launch_p_to_nat(v.u32(), dev_v, n_ct * 4, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -618,6 +650,7 @@ struct sp_batch {
   std::vector<hipGraph_t> graph_defs;
   uint32_t *bulk = nullptr;       // bulk-load staging: [cap][4N] queries then [cap][f_q] v_folding
   uint32_t *dig = nullptr;        // split-expansion digit scratch [cap][dig_q]
+  uint32_t *snap = nullptr;       // latency-form expansion snapshots, 2 sets x snap_q x SNAP_QUERIES
   size_t dig_q = 0;
   uint32_t *part = nullptr;       // split-fold partial sums [cap][n2][C/2][m][h][row][N]
   uint32_t *conv = nullptr;       // conversion keys of packed slots [cap][m][8][row][P], 2^32-scaled
@@ -637,6 +670,10 @@ struct sp_batch {
   std::vector<std::pair<int, float>> dbmul_ms[3];  // measurements per variant, for reporting
 };
 
+// Queries a batch keeps expansion snapshots for (the latency form runs only
+// for runs of at most this many queries; 2 x 16 MiB per query at (9,6)).
+constexpr int SNAP_QUERIES = 2;
+
 static int mfma_rows(int n) { return ((2 * n + 31) / 32) * 32; }
 
 struct sp_query {
@@ -654,6 +691,7 @@ static void batch_free(sp_batch *b) {
   if (b->mfma_a) (void)hipFree(b->mfma_a);
   if (b->d_packed) (void)hipFree(b->d_packed);
   if (b->d_haspk) (void)hipFree(b->d_haspk);
+  if (b->snap) (void)hipFree(b->snap);
   if (b->pk) (void)hipFree(b->pk);
   if (b->pout) (void)hipFree(b->pout);
   if (b->done) (void)hipEventDestroy(b->done);
@@ -709,6 +747,7 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   if (t_exp == 8) {  // digit scratch of the split expansion
     b->dig_q = (size_t)expansion_max_work(db->nu1, db->nu2, t_gsw) * 8 * 512;
     A(&b->dig, cap * b->dig_q);
+    A(&b->snap, 2 * (size_t)std::min(capacity, SNAP_QUERIES) * ((size_t)1 << g) * 2 * N);
   }
   if (t_gsw == 8) A(&b->part, cap * db->n2 * (size_t)(C / 2) * 8 * N);  // split-fold partial sums
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->io, hipStreamNonBlocking);
@@ -1089,9 +1128,13 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
   const int R = 1 << db->nu1, C = 1 << db->nu2;
   const int packed = slots_packed(b, n);
   if (stages & 1) {
-    HIP_TRY(launch_copy_queries(b->query_ct, 4 * N, b->v, b->v_q, n, s));
+    ExpandSnap snap;
+    snap.p = b->snap;
+    snap.queries = std::min(b->cap, SNAP_QUERIES);
+    snap.q_stride = ((size_t)1 << b->g) * 2 * N;
+    snap.set_stride = (size_t)snap.queries * snap.q_stride;
     int rc = run_expansion(ctx, b->v, b->v_q, n, db->nu1, db->nu2, b->t_gsw, b->t_exp, b->left, b->right, b->l_q,
-                           b->r_q, -1, b->dig, b->dig_q, s);
+                           b->r_q, -1, b->dig, b->dig_q, snap, b->query_ct, 4 * N, s);
     if (rc) return rc;
     if (packed)
       HIP_TRY(launch_regev_to_gsw(true, b->v, b->v_q, n, db->nu2, b->t_gsw, b->conv, b->conv_q, b->fkeys, b->f_q,

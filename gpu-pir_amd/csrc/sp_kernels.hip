@@ -1693,6 +1693,126 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   expand_update<4>(vi, W, tiles, 0, m, 4 * tid, t, q, qinv);
 }
 
+// ---- latency form of an expansion round (few steps) -------------------------
+// The split kernels above need two launches per round, because the v_neg1
+// product that creates element i >= 2^r reads v[i - 2^r] while that element's
+// own blocks update it.  For rounds with few steps the whole round runs in one
+// launch, one 8-wave block per (step, residue m):
+//   * v_i.a of both residues comes from a snapshot S_r of the round's inputs
+//     (row 0, layout [m][P] per element), which no block of the round writes;
+//     each block writes only residue m of v_i;
+//   * the epilogue prepares round r + 1: S_{r+1}[i] (the new v_i.a, residue m)
+//     and, when element i works in round r + 1, its child
+//     v[i + 2^(r+1)] = v_neg1[r+1] (.) v_i (residue m) with its snapshot, which
+//     is what the reference computes at the start of round r + 1;
+//   * blocks past the working steps form the v_neg1 children of skipped
+//     elements (odd, past stop_round), which no block of the round writes.
+// Round 0's snapshot and child come from k_expand_prep.  A run uses this form
+// for a prefix of the rounds, then the split kernels (which recompute the
+// children themselves).
+__device__ __forceinline__ bool expand_skipped(int i, int r, int stop, int max_bits) {
+  return stop > 0 && (i & 1) && (r > stop || (r == stop && (i >> 1) >= max_bits));
+}
+
+// dst (residue m, rows 0-1) = v_neg1 (.) src at 4 consecutive words from p0; the
+// row-0 result also goes to snap (if given).
+__device__ __forceinline__ void neg1_rows4(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                           uint32_t *__restrict__ snap, const uint32_t *__restrict__ neg1,
+                                           int m, int p0, const ModConst &mc) {
+  const uint4 w = *reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + p0);
+  const uint4 wp = *reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + N + p0);
+#pragma unroll
+  for (int row = 0; row < 2; row++) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(src + (size_t)(row * 2 + m) * N + p0);
+    uint4 y;
+    y.x = canon(shoup_mul(x.x, w.x, wp.x, mc.q), mc.q);
+    y.y = canon(shoup_mul(x.y, w.y, wp.y, mc.q), mc.q);
+    y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
+    y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
+    *reinterpret_cast<uint4 *>(dst + (size_t)(row * 2 + m) * N + p0) = y;
+    if (row == 0 && snap) *reinterpret_cast<uint4 *>(snap + (size_t)m * N + p0) = y;
+  }
+}
+
+// Round 0 of the latency form: v[0] = the query (when `query` is given; else
+// v[0] is in place), S_0[0] = v[0].a, v[1] = v_neg1[0] (.) v[0], S_0[1] = v[1].a.
+// Block (residue m, query); each thread owns 4 consecutive words of each poly.
+__global__ __launch_bounds__(256) void k_expand_prep(const uint32_t *__restrict__ query, size_t query_qstride,
+                                                     uint32_t *__restrict__ v, size_t q_stride,
+                                                     uint32_t *__restrict__ snap, size_t s_qstride,
+                                                     const uint32_t *__restrict__ neg1_0, ModConst mc0,
+                                                     ModConst mc1) {
+  uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
+  uint32_t *sq = snap + (size_t)blockIdx.y * s_qstride;
+  const int m = blockIdx.x;
+  const ModConst mc = m ? mc1 : mc0;
+  for (int p0 = 4 * threadIdx.x; p0 < N; p0 += 4 * 256) {
+    if (query) {
+      const uint32_t *qq = query + (size_t)blockIdx.y * query_qstride;
+#pragma unroll
+      for (int row = 0; row < 2; row++)
+        *reinterpret_cast<uint4 *>(vq + (size_t)(row * 2 + m) * N + p0) =
+            *reinterpret_cast<const uint4 *>(qq + (size_t)(row * 2 + m) * N + p0);
+    }
+    *reinterpret_cast<uint4 *>(sq + (size_t)m * N + p0) = *reinterpret_cast<const uint4 *>(vq + (size_t)m * N + p0);
+    neg1_rows4(vq, vq + 4 * N, sq + 2 * N, neg1_0, m, p0, mc);
+  }
+}
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void k_expand_fused(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work, int last,
+                    const uint32_t *__restrict__ neg1_r, const uint32_t *__restrict__ neg1_next,
+                    const uint32_t *__restrict__ s_in, uint32_t *__restrict__ s_out, size_t s_qstride,
+                    const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
+                    size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t tiles[8][LDS_TILE];
+  const int b = blockIdx.x, half = 1 << r, m = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const ModConst &mcm = m ? mc1 : mc0;
+  uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
+  if (b >= work) {  // v_neg1 child of a skipped (odd) element: i = half + 2 (b - work) + 1
+    const int i = half + 2 * (b - work) + 1;
+    for (int p0 = 4 * tid; p0 < N; p0 += 4 * 512)
+      neg1_rows4(vq + (size_t)(i - half) * 4 * N, vq + (size_t)i * 4 * N, nullptr, neg1_r, m, p0, mcm);
+    return;
+  }
+  const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  uint32_t *vi = vq + (size_t)i * 4 * N;
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  const bool use_left = (r != 0) && ((i & 1) == 0);
+  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
+  const uint32_t *si = s_in + (size_t)blockIdx.y * s_qstride + (size_t)i * 2 * N;
+  expand_digits<512>(si, tiles, t, tw_base, mc0, mc1, wave, lane, tid);
+  uint32_t dw[8];  // digit `wave`: byte b of word i4 is coefficient 64 (4 i4 + b) + lane
+  const uint32_t *dig = tiles[4];
+#pragma unroll
+  for (int i4 = 0; i4 < 8; i4++) dw[i4] = dig[wave * DIG_WORDS + i4 * 64 + lane];
+  __syncthreads();  // the digit bytes are dead from here on
+  {
+    uint32_t rr[32];
+#pragma unroll
+    for (int i4 = 0; i4 < 8; i4++)
+#pragma unroll
+      for (int bb = 0; bb < 4; bb++) rr[4 * i4 + bb] = (dw[i4] >> (8 * bb)) & 0xFFu;
+    // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
+    ntt_fwd_wave_t<8, true>(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], mcm.q, mcm.mu1, lane);
+    store_tile_p_full(tiles[wave], rr, lane, mcm.q, mcm.mu1);
+  }
+  __syncthreads();
+  const int p0 = 4 * tid;
+  expand_update<4>(vi, W, tiles, 0, m, p0, t, mcm.q, mcm.qinv_neg);
+  if (last) return;
+  // Round r + 1: this thread's 4 positions of residue m, as just written.
+  uint32_t *so = s_out + (size_t)blockIdx.y * s_qstride;
+  *reinterpret_cast<uint4 *>(so + (size_t)i * 2 * N + (size_t)m * N + p0) =
+      *reinterpret_cast<const uint4 *>(vi + (size_t)m * N + p0);
+  if (!expand_skipped(i, r + 1, stop, max_bits)) {
+    const int c = i + 2 * half;
+    neg1_rows4(vi, vq + (size_t)c * 4 * N, so + (size_t)c * 2 * N, neg1_next, m, p0, mcm);
+  }
+}
+
 // Working steps of round r (grid of k_expand8 / k_expand_update), and the
 // skipped odd elements >= 2^r that still take the v_neg1 product.
 static void round_plan(int r, int stop, int max_bits, int &work, int &odd_start, int &extra) {
@@ -1715,6 +1835,37 @@ int expand_split_work(int r, int stop, int max_bits) {
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
   return work;
+}
+
+hipError_t launch_expand_prep(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t q_stride,
+                              int n_queries, uint32_t *snap, size_t s_qstride, const DevTables &t, hipStream_t s) {
+  k_expand_prep<<<dim3(2, n_queries), 256, 0, s>>>(query, query_qstride, v, q_stride, snap, s_qstride, t.neg1,
+                                                   t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+int expand_fused_blocks(int r, int stop, int max_bits) {
+  int work, odd_start, extra;
+  round_plan(r, stop, max_bits, work, odd_start, extra);
+  return work + (stop > 0 && r > stop ? (1 << r) / 2 : 0);
+}
+
+hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
+                               const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
+                               uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s) {
+  int work, odd_start, extra;
+  round_plan(r, stop, max_bits, work, odd_start, extra);
+  // Children of skipped elements only: at stop_round the round plan's extra
+  // elements are children of working elements, made by the previous epilogue.
+  const int neg1_only = (stop > 0 && r > stop) ? (1 << r) / 2 : 0;
+  const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
+  const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
+  const uint32_t *s_in = snap + (size_t)(r & 1) * s_round;
+  uint32_t *s_out = snap + (size_t)((r + 1) & 1) * s_round;
+  k_expand_fused<<<dim3(work + neg1_only, n_queries, 2), 512, 0, s>>>(
+      v, q_stride, r, stop, max_bits, work, r + 1 >= end, neg1_r, neg1_next, s_in, s_out, s_qstride, left, right,
+      left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
 }
 
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int stop, int max_bits,

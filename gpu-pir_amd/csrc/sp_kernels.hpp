@@ -80,6 +80,19 @@ hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, in
 // update); also forms v[2^r + i] = v_neg1[r] * v[i] (no separate neg1 launch).
 //   dig: scratch, per query dig_qstride >= expand_split_work(r, ...) * 8 * 512 words
 int expand_split_work(int r, int stop, int max_bits);
+// Latency form of the expansion (one launch per round, for rounds with few
+// steps; a prefix of the rounds).  snap: two snapshot sets, set k at
+// snap + k * s_round, per query s_qstride words = 2^g * 2N (row 0 of every
+// element, layout [m][P]).  launch_expand_prep readies round 0, first copying
+// the queries into v[0] when `query` is given; expand_fused_blocks gives the
+// blocks per query and residue.
+hipError_t launch_expand_prep(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t q_stride,
+                              int n_queries, uint32_t *snap, size_t s_qstride, const DevTables &t, hipStream_t s);
+int expand_fused_blocks(int r, int stop, int max_bits);
+// end: the round after the last one this run performs (no epilogue for round end - 1).
+hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
+                               const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
+                               uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s);
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                const uint32_t *neg1_r, uint32_t *dig, size_t dig_qstride, const DevTables &t,
