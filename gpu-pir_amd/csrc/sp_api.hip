@@ -300,11 +300,13 @@ extern "C" int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *de
   HIP_TRY(hipSetDevice(ctx->device));
   hipStream_t s = pick(ctx, stream);
   const int R = 1 << nu1, C = 1 << nu2;
-  DevBuf db, sel, out;
-  HIP_TRY(db.alloc((size_t)2 * R * C * N * 4));
+  DevBuf stage, db, sel, out;
+  HIP_TRY(stage.alloc((size_t)2 * R * C * N * 4));
+  HIP_TRY(db.alloc(db_trial_words(R, C) * 4));
   HIP_TRY(sel.alloc((size_t)R * 4 * N * 4));
   HIP_TRY(out.alloc((size_t)C * 4 * N * 4));
-  HIP_TRY(launch_db_from_ref(dev_db, db.u32(), R, C, s));
+  HIP_TRY(launch_db_from_ref(dev_db, stage.u32(), R, C, s));
+  HIP_TRY(launch_db_pack(stage.u32(), db.u32(), 1, R, 0, R, C, s));
   HIP_TRY(launch_nat_to_p(dev_first_dim, sel.u32(), (size_t)R * 4, 1, 0, ctx->tables, s));
   HIP_TRY(launch_db_mul(db.u32(), sel.u32(), 4 * N, 0, out.u32(), 0, 1, 1, R, C, ctx->tables, s));
   HIP_TRY(launch_p_to_nat(out.u32(), dev_out, (size_t)C * 4, s));
@@ -537,9 +539,10 @@ extern "C" int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_g
 struct sp_db {
   sp_ctx *ctx = nullptr;
   int nu1 = 0, nu2 = 0, n2 = 0;
-  uint32_t *d_db = nullptr;  // [n2][2][R][C][P]
+  uint32_t *d_db = nullptr;  // [n2][2][R][C/CG][W][P], 28-bit residues packed (db_group_cols)
   size_t bytes = 0;
   uint8_t *d_limbs = nullptr;  // int8-limb copy for the MFMA multiply, built on first use
+  size_t limb_bytes = 0;
   bool limbs_valid = false;
 };
 
@@ -549,7 +552,7 @@ static bool mfma_shape_ok(const sp_db *db) { return db->nu2 >= 5 && db->nu1 >= 5
 static int db_limbs(sp_db *db, hipStream_t s) {
   if (db->limbs_valid) return SP_OK;
   if (!db->d_limbs) {
-    hipError_t e = hipMalloc(&db->d_limbs, db->bytes);
+    hipError_t e = hipMalloc(&db->d_limbs, db->limb_bytes);
     if (e != hipSuccess)
       return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "database limbs: %s", hipGetErrorString(e));
   }
@@ -566,7 +569,8 @@ extern "C" int sp_db_create(sp_ctx *ctx, int nu1, int nu2, int n2, sp_db **out) 
   sp_db *d = new sp_db();
   d->ctx = ctx;
   d->nu1 = nu1; d->nu2 = nu2; d->n2 = n2;
-  d->bytes = (size_t)n2 * 2 * ((size_t)1 << (nu1 + nu2)) * N * 4;
+  d->bytes = (size_t)n2 * db_trial_words(1 << nu1, 1 << nu2) * 4;
+  d->limb_bytes = (size_t)n2 * 2 * ((size_t)1 << (nu1 + nu2)) * N * 4;  // 4 limbs per residue
   hipError_t e = hipMalloc(&d->d_db, d->bytes);
   if (e != hipSuccess) {
     delete d;
@@ -594,16 +598,22 @@ extern "C" int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes) {
   db->limbs_valid = false;
   sp_ctx *ctx = db->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
-  const size_t nrec = (size_t)1 << (db->nu1 + db->nu2);
+  const int R = 1 << db->nu1, C = 1 << db->nu2;
+  const size_t nrec = (size_t)R * C;
   const size_t rec_bytes = (size_t)db->n2 * N;
-  const size_t chunk = std::max<size_t>(1, ((size_t)64 << 20) / rec_bytes);
-  DevBuf stage;
-  HIP_TRY(stage.alloc(std::min(chunk, nrec) * rec_bytes));
+  // Chunks of whole rows (C records): bytes -> u32 NTT residues -> packed rows.
+  size_t chunk = std::max<size_t>(1, ((size_t)64 << 20) / rec_bytes);
+  chunk = std::max<size_t>(C, chunk - chunk % C);
+  chunk = std::min(chunk, nrec);
+  DevBuf stage, wide;
+  HIP_TRY(stage.alloc(chunk * rec_bytes));
+  HIP_TRY(wide.alloc(chunk * rec_bytes * 2 * 4));
   for (size_t r0 = 0; r0 < nrec; r0 += chunk) {
     const size_t cnt = std::min(chunk, nrec - r0);
     HIP_TRY(hipMemcpyAsync(stage.p, host_bytes + r0 * rec_bytes, cnt * rec_bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_db_encode(static_cast<const uint8_t *>(stage.p), r0, cnt, db->d_db, db->n2, 1 << db->nu1,
-                             1 << db->nu2, ctx->tables, ctx->stream));
+    HIP_TRY(launch_db_encode(static_cast<const uint8_t *>(stage.p), r0, cnt, wide.u32(), db->n2, C, ctx->tables,
+                             ctx->stream));
+    HIP_TRY(launch_db_pack(wide.u32(), db->d_db, db->n2, (int)(cnt / C), (int)(r0 / C), R, C, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
   }
   return SP_OK;
@@ -616,11 +626,13 @@ extern "C" int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt) {
   HIP_TRY(hipSetDevice(ctx->device));
   const int R = 1 << db->nu1, C = 1 << db->nu2;
   const size_t trial_words = (size_t)2 * R * C * N;
-  DevBuf stage;
+  DevBuf stage, wide;
   HIP_TRY(stage.alloc(trial_words * 4));
+  HIP_TRY(wide.alloc(trial_words * 4));
   for (int t = 0; t < db->n2; t++) {
     HIP_TRY(hipMemcpyAsync(stage.p, host_ntt + t * trial_words, trial_words * 4, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_db_from_ref(stage.u32(), db->d_db + t * trial_words, R, C, ctx->stream));
+    HIP_TRY(launch_db_from_ref(stage.u32(), wide.u32(), R, C, ctx->stream));
+    HIP_TRY(launch_db_pack(wide.u32(), db->d_db + t * db_trial_words(R, C), 1, R, 0, R, C, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
   }
   return SP_OK;

@@ -5,6 +5,7 @@
 // bit for bit with the reference semantics.  See sp_device.hpp for the NTT
 // schedule and the internal NTT-domain layout P.
 #include <cmath>
+#include <utility>
 
 #include "sp_kernels.hpp"
 
@@ -259,11 +260,12 @@ __global__ __launch_bounds__(256) void k_db_from_ref(const uint32_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Database ingest: plaintext bytes -> NTT residues, internal layout
-//   db[t][m][j][c][p], record index = j * C + c.
+// Database ingest: plaintext bytes -> NTT residues, u32 staging layout
+//   out[t][m][j - j0][c][p], record index = j * C + c, j0 = rec0 / C (rec0 is a
+// multiple of C, so a chunk holds whole rows).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ bytes, size_t rec0, size_t nrec,
-                                                   uint32_t *__restrict__ db, int n2, int R, int C,
+                                                   uint32_t *__restrict__ db, int n2, int rows, int C,
                                                    const uint32_t *__restrict__ tw_base, ModConst mc0,
                                                    ModConst mc1) {
   __shared__ uint32_t lds_all[4 * LDS_TILE];
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ b
     const int t = (int)(pt % n2);
     const size_t rl = pt / n2;
     const size_t rec = rec0 + rl;
-    const int j = (int)(rec / C), c = (int)(rec % C);
+    const int j = (int)(rec / C - rec0 / C), c = (int)(rec % C);
     const uint8_t *src = bytes + pt * N;
     const ModConst mc = m ? mc1 : mc0;
     uint32_t r[32];
@@ -285,16 +287,77 @@ __global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ b
     ntt_fwd_wave(r, tw_base + (m * 2) * TW_WORDS, lds, mc.q, lane);
 #pragma unroll
     for (int i = 0; i < 32; i++) r[i] = canon4(r[i], mc.q);
-    store_p(db + ((((size_t)t * 2 + m) * R + j) * C + c) * N, r, lane);
+    store_p(db + ((((size_t)t * 2 + m) * rows + j) * C + c) * N, r, lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Resident database format.  Residues are below 2^28, so the CG = min(C, 8)
+// columns of one group (trial t, residue m, row j, position p) are packed into
+// W = ceil(28 CG / 32) words: bits 28 c .. 28 c + 27 of the group hold column
+// CG g + c.  Word k of all positions forms one plane:
+//   db[t][m][j][g][k][P],   g < C / CG,  k < W.
+// At C >= 8 the multiply streams 7/8 of the bytes of the plain u32 form.
+// ---------------------------------------------------------------------------
+template <int... I, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F &&f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, n - 1>).
+template <int n, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(std::make_integer_sequence<int, n>{}, f);
+}
+
+template <int c, int W>
+__device__ __forceinline__ uint32_t unpack28(const uint32_t (&w)[W]) {
+  constexpr int pos = 28 * c, k = pos >> 5, sh = pos & 31;
+  static_assert(k < W && (sh + 28 <= 32 || k + 1 < W), "column outside its group");
+  if constexpr (sh + 28 <= 32)
+    return (w[k] >> sh) & 0x0FFFFFFFu;
+  else
+    return __builtin_amdgcn_alignbit(w[k + 1], w[k], sh) & 0x0FFFFFFFu;
+}
+
+// Column c of the group whose word 0 is at w0 (planes `plane` words apart).
+__device__ __forceinline__ uint32_t unpack28_at(const uint32_t *__restrict__ w0, size_t plane, int c) {
+  const int pos = 28 * c, k = pos >> 5, sh = pos & 31;
+  const uint32_t lo = w0[(size_t)k * plane];
+  const uint32_t hi = sh + 28 > 32 ? w0[(size_t)(k + 1) * plane] : 0u;
+  return __builtin_amdgcn_alignbit(hi, lo, sh) & 0x0FFFFFFFu;
+}
+
+// u32 staging rows [t][m][rows][C][P] -> rows j0 .. j0 + rows - 1 of the
+// resident format (R rows per residue).
+__global__ __launch_bounds__(256) void k_db_pack(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, int n2,
+                                                 int rows, int j0, int R, int C) {
+  const int CG = db_group_cols(C), W = db_group_words(C), G = C / CG;
+  const size_t total = (size_t)n2 * 2 * rows * G * N;
+  for (size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x; gid < total; gid += (size_t)gridDim.x * 256) {
+    const int p = (int)(gid % N);
+    size_t rest = gid / N;
+    const int g = (int)(rest % G); rest /= G;
+    const int jl = (int)(rest % rows);
+    const size_t tm = rest / rows;  // t * 2 + m
+    uint32_t w[7] = {0, 0, 0, 0, 0, 0, 0};
+    const uint32_t *src = in + ((tm * rows + jl) * C + (size_t)g * CG) * N + p;
+    for (int c = 0; c < CG; c++) {
+      const uint32_t x = src[(size_t)c * N];
+      const int pos = 28 * c, k = pos >> 5, sh = pos & 31;
+      w[k] |= x << sh;
+      if (sh + 28 > 32) w[k + 1] |= x >> (32 - sh);
+    }
+    uint32_t *o = out + ((tm * R + j0 + jl) * G + g) * W * N + p;
+    for (int k = 0; k < W; k++) o[(size_t)k * N] = w[k];
   }
 }
 
 // ---------------------------------------------------------------------------
 // First-dimension DB multiply (single query), internal layouts:
-//   db  [T][m][R][C][P]      sel j at sel + j*sel_stride: [row][m][P]
+//   db  [T][m][R][C/CG][W][P] (resident format)   sel j at sel + j*sel_stride: [row][m][P]
 //   out [T][C][row][m][P]
-// Block = WPB waves splitting the R rows; a lane owns 2 positions and CG
-// columns, accumulating sel*db lazily in u64 (folded every 128 rows).  The
+// Block = WPB waves splitting the R rows; a lane owns 2 positions and the CG
+// columns of one group, accumulating sel*db lazily in u64 (folded every 128 rows).  The
 // database is streamed once with non-temporal loads (no reuse to cache);
 // the selectors are reused by every column group and trial and stay cached.
 // Blocks are numbered so that the 8 XCDs each own a quarter of the
@@ -314,6 +377,7 @@ __global__ __launch_bounds__(WPB * 64) void k_db_mul(const uint32_t *__restrict_
                                                      size_t sel_qstride, uint32_t *__restrict__ out_all,
                                                      size_t out_qstride, int nq, int T, int R, int C, ModConst mc0,
                                                      ModConst mc1) {
+  constexpr int W = (28 * CG + 31) / 32;
   __shared__ uint32_t red[WPB][CG][2][128];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ncg = C / CG;
@@ -343,7 +407,7 @@ __global__ __launch_bounds__(WPB * 64) void k_db_mul(const uint32_t *__restrict_
 
   // Products are < q^2 < 2^56; after each 128-row block the accumulators are
   // reduced mod q, so a block starts below q and ends below q + 128 q^2 < 2^63.
-  const uint32_t *dbase = db + ((((size_t)tt * 2 + m) * R) * C + (size_t)cg * CG) * N + p0;
+  const uint32_t *dbase = db + ((((size_t)tt * 2 + m) * R) * ncg + cg) * W * N + p0;
   for (int jb = 0; jb < rows_per_wave; jb += 128) {
     const int jend = min(rows_per_wave, jb + 128);
 #pragma unroll 2
@@ -352,15 +416,22 @@ __global__ __launch_bounds__(WPB * 64) void k_db_mul(const uint32_t *__restrict_
       const uint32_t *sp = sel + (size_t)j * sel_stride + (size_t)m * N + p0;
       const uint2 s0 = *reinterpret_cast<const uint2 *>(sp);
       const uint2 s1 = *reinterpret_cast<const uint2 *>(sp + 2 * N);
-      const uint32_t *dp = dbase + (size_t)j * C * N;
+      const uint32_t *dp = dbase + (size_t)j * ncg * W * N;
+      uint32_t wx[W], wy[W];  // the packed groups of positions p0 and p0 + 1
 #pragma unroll
-      for (int c = 0; c < CG; c++) {
-        const v2u32 d = __builtin_nontemporal_load(reinterpret_cast<const v2u32 *>(dp + (size_t)c * N));
-        acc[c][0][0] += (uint64_t)s0.x * d.x;
-        acc[c][1][0] += (uint64_t)s0.y * d.y;
-        acc[c][0][1] += (uint64_t)s1.x * d.x;
-        acc[c][1][1] += (uint64_t)s1.y * d.y;
+      for (int k = 0; k < W; k++) {
+        const v2u32 d = __builtin_nontemporal_load(reinterpret_cast<const v2u32 *>(dp + (size_t)k * N));
+        wx[k] = d.x;
+        wy[k] = d.y;
       }
+      static_for<CG>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        const uint32_t dx = unpack28<c, W>(wx), dy = unpack28<c, W>(wy);
+        acc[c][0][0] += (uint64_t)s0.x * dx;
+        acc[c][1][0] += (uint64_t)s0.y * dy;
+        acc[c][0][1] += (uint64_t)s1.x * dx;
+        acc[c][1][1] += (uint64_t)s1.y * dy;
+      });
     }
     if (jend < rows_per_wave) {
 #pragma unroll
@@ -734,16 +805,24 @@ hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, h
   return hipGetLastError();
 }
 
-hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int R, int C,
+hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int C,
                             const DevTables &t, hipStream_t s) {
-  k_db_encode<<<grid_for(nrec * n2 * 2, 4), 256, 0, s>>>(bytes, rec0, nrec, db, n2, R, C, t.tw, t.mc[0], t.mc[1]);
+  const int rows = (int)(nrec / C);
+  k_db_encode<<<grid_for(nrec * n2 * 2, 4), 256, 0, s>>>(bytes, rec0, nrec, db, n2, rows, C, t.tw, t.mc[0],
+                                                         t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_db_pack(const uint32_t *in, uint32_t *out, int n2, int rows, int j0, int R, int C, hipStream_t s) {
+  const size_t items = (size_t)n2 * 2 * rows * (C / db_group_cols(C)) * N;
+  k_db_pack<<<grid_for(items, 256), 256, 0, s>>>(in, out, n2, rows, j0, R, C);
   return hipGetLastError();
 }
 
 hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
                          uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,
                          hipStream_t s) {
-  const int cg = C >= 8 ? 8 : C;
+  const int cg = db_group_cols(C);
   const unsigned blocks = (unsigned)(32 * T * (C / cg) * nq);
   // 8 waves split the rows (R >= 8); the smallest databases (R = 4) use 4.
 #define SP_DBMUL(CGV, W)                                                                                      \
@@ -2032,7 +2111,7 @@ __global__ __launch_bounds__(256) void k_sel_to_limbs(const uint32_t *__restrict
   }
 }
 
-// Database limbs from the resident u32 database [t][m][j][c][P] (one-time).
+// Database limbs from the resident database [t][m][j][g][k][P] (one-time).
 __global__ __launch_bounds__(256) void k_db_to_limbs(const uint32_t *__restrict__ db, int R, int C,
                                                      uint8_t *__restrict__ D) {
   __shared__ uint32_t tile[4][32][65];
@@ -2041,10 +2120,13 @@ __global__ __launch_bounds__(256) void k_db_to_limbs(const uint32_t *__restrict_
   const int c0 = (blockIdx.z % cblocks) * 4;
   const int tm = blockIdx.z / cblocks;  // t * 2 + m
   const int JB = R / 32;
+  const int CG = db_group_cols(C), W = db_group_words(C), G = C / CG;
   const int tid = threadIdx.x;
   for (int k = tid; k < 4 * 32 * 64; k += 256) {
     const int p = k & 63, j = (k >> 6) & 31, cc = k >> 11;
-    tile[cc][j][p] = db[(((size_t)tm * R + jb * 32 + j) * C + c0 + cc) * N + pb * 64 + p];
+    const int c = c0 + cc;
+    const uint32_t *w0 = db + (((size_t)tm * R + jb * 32 + j) * G + c / CG) * W * N + pb * 64 + p;
+    tile[cc][j][p] = unpack28_at(w0, N, c % CG);
   }
   __syncthreads();
   const int p = tid & 63, limb = tid >> 6;

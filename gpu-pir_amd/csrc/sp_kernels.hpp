@@ -29,9 +29,22 @@ hipError_t launch_p_to_nat(const uint32_t *in, uint32_t *out, size_t n, hipStrea
 // P -> natural with the 2^32 scaling removed (poly k is residue (k / m_div) & 1).
 hipError_t launch_p_to_nat_unscale(const uint32_t *in, uint32_t *out, size_t n, int m_div, const DevTables &t,
                                    hipStream_t s);
+// Resident database format: groups of CG = min(C, 8) columns, 28 bits per
+// residue, packed into W = ceil(28 CG / 32) word planes: [t][m][R][C/CG][W][P].
+__host__ __device__ inline int db_group_cols(int C) { return C < 8 ? C : 8; }
+__host__ __device__ inline int db_group_words(int C) { return (28 * db_group_cols(C) + 31) / 32; }
+inline size_t db_trial_words(int R, int C) {
+  return (size_t)2 * R * (C / db_group_cols(C)) * db_group_words(C) * N;
+}
+
+// Reference Database layout [c][j][m][z] (one trial) -> u32 staging [m][j][c][P].
 hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s);
-hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int R, int C,
+// Plaintext records rec0 .. rec0 + nrec - 1 (rec0 and nrec multiples of C) ->
+// u32 staging [t][m][nrec / C][C][P].
+hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int C,
                             const DevTables &t, hipStream_t s);
+// u32 staging [t][m][rows][C][P] -> rows j0 .. j0 + rows - 1 of the resident format.
+hipError_t launch_db_pack(const uint32_t *in, uint32_t *out, int n2, int rows, int j0, int R, int C, hipStream_t s);
 // Batched over nq queries: selectors of query b at sel + b*sel_qstride, outputs at out + b*out_qstride.
 hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
                          uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,

@@ -76,9 +76,18 @@ class Shape:
         return self.n2 * POLY_LEN
 
     @property
-    def db_device_bytes(self) -> int:
-        """Resident NTT form: 2 residues x 4 bytes per plaintext coefficient."""
+    def db_ntt_bytes(self) -> int:
+        """NTT form as u32 words (the reference's Database): 2 residues x 4 bytes
+        per plaintext coefficient."""
         return self.records * self.n2 * 2 * POLY_LEN * 4
+
+    @property
+    def db_device_bytes(self) -> int:
+        """Resident form in HBM: the 28-bit residues of groups of min(2^nu2, 8)
+        columns packed into ceil(28 x group / 32) words (7/8 of the u32 form)."""
+        group = min(self.cols, 8)
+        words = (28 * group + 31) // 32
+        return self.db_ntt_bytes // (4 * group) * 4 * words
 
     # words (u32) of the client/server buffers
     @property

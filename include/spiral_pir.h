@@ -181,13 +181,16 @@ int sp_pack_response(sp_ctx *ctx, const uint32_t *dev_raw, int n2, const uint32_
 /* ---- resident database and answer path -------------------------------------
  * A database of 2^(nu1+nu2) records, each n2 polynomials of 2048 plaintext
  * bytes (n2 = n*n sub-databases; record index = i1 * 2^nu2 + i2).  It is
- * converted to NTT form on the GPU once and stays resident in HBM. */
+ * converted to NTT form on the GPU once and stays resident in HBM, its 28-bit
+ * residues packed 8 to 7 words (2^nu2 >= 8; narrower databases pack 2^nu2
+ * residues into ceil(28 * 2^nu2 / 32) words). */
 int sp_db_create(sp_ctx *ctx, int nu1, int nu2, int n2, sp_db **out);
 /* host_bytes: [records][n2][2048] u8, uploaded in chunks and NTT'd on device. */
 int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes);
 /* Reference layout (NTT Polys, [n2][2^nu2 cols][2^nu1 rows]), host memory. */
 int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt);
 int sp_db_destroy(sp_db *db);
+/* HBM bytes of the resident (packed) NTT form. */
 size_t sp_db_device_bytes(const sp_db *db);
 
 /* Per-query device state: the query ciphertext, its expansion and folding

@@ -80,5 +80,7 @@ def test_shape_helpers():
 
     s = Shape(9, 6)
     assert (s.g, s.stop_round) == (10, 6)
-    assert s.db_device_bytes == 2 ** 31
+    assert s.db_ntt_bytes == 2 ** 31
+    assert s.db_device_bytes == 7 * 2 ** 28
+    assert Shape(6, 2).db_device_bytes == Shape(6, 2).db_ntt_bytes
     assert s.records * s.record_bytes == 2 ** 28
