@@ -5,7 +5,8 @@ A step is one full server answer for one query against a resident database:
 coefficient expansion -> first-dimension database multiply (all n^2
 sub-databases) -> from_ntt -> GSW fold, with the database, the query and its
 keys already in HBM.  Default workload: (nu1, nu2) = (9, 6), n = 2 (2^15
-records of 8 KiB, 2 GiB resident in NTT form).
+records of 8 KiB; 2 GiB in NTT form as u32 residues, 1.75 GiB resident with
+the 28-bit residues packed).
 
 Prints one JSON line (queries/s) with the database-multiply kernel's HBM
 roofline and the CPU oracle's time for the same work on the host cores.
@@ -247,7 +248,8 @@ def run_group(torch, mods, ctx, main_db, steps: int, stream) -> tuple[dict, dict
     ms = start.elapsed_time(stop) / steps
     total = sum(ns)
     out = {"workload": "96 queries split 48/32/16 over resident (9,6)/(10,7)/(8,4) databases "
-                       "(2 GiB / 8 GiB / 128 MiB), one hipGraph launch with one branch per database",
+                       "(NTT form 2 GiB / 8 GiB / 128 MiB, resident packed 1.75 GiB / 7 GiB / 112 MiB), "
+                       "one hipGraph launch with one branch per database",
            "ms_per_launch": ms, "queries_per_s": total * 1000.0 / ms, "steps": steps,
            "db_mul_autotune": {ln[0]: ln[6] for ln in lanes}, "decoded_all_correct": bool(ok),
            "extra_ingest_s": t_ingest}
