@@ -665,6 +665,8 @@ struct sp_batch {
   uint32_t *snap = nullptr;       // latency-form expansion snapshots, 2 sets x snap_q x SNAP_QUERIES
   size_t dig_q = 0;
   uint32_t *part = nullptr;       // split-fold partial sums [cap][n2][C/2][m][h][row][N]
+  uint32_t *fold_alt = nullptr;   // latency-form fold: [FOLD_QUERIES][n2][max(C/4,1)] raw ciphertexts
+  uint32_t *part_alt = nullptr;   //   and [FOLD_QUERIES][n2][max(C/4,1)][m][group 4][row][N] partial sums
   uint32_t *conv = nullptr;       // conversion keys of packed slots [cap][m][8][row][P], 2^32-scaled
   size_t conv_q = 0;
   std::vector<int> packed;        // per slot: 1 = packed query (v_folding converted on the device)
@@ -685,6 +687,9 @@ struct sp_batch {
 // Queries a batch keeps expansion snapshots for (the latency form runs only
 // for runs of at most this many queries; 2 x 16 MiB per query at (9,6)).
 constexpr int SNAP_QUERIES = 2;
+// Queries a batch keeps latency-form fold buffers for (runs of at most this
+// many queries fold with k_fold_fused; 10 MiB per query at (9,6)).
+constexpr int FOLD_QUERIES = 2;
 
 static int mfma_rows(int n) { return ((2 * n + 31) / 32) * 32; }
 
@@ -697,7 +702,8 @@ static void batch_free(sp_batch *b) {
   for (auto &ge : b->graphs) (void)hipGraphExecDestroy(ge.second);
   for (auto &gd : b->graph_defs) (void)hipGraphDestroy(gd);
   uint32_t *bufs[] = {b->query_ct, b->v,        b->left, b->right, b->fkeys, b->fkeys_neg, b->acc,
-                      b->stage,    b->mfma_out, b->bulk, b->dig,   b->part,  b->conv};
+                      b->stage,    b->mfma_out, b->bulk, b->dig,   b->part,  b->conv,
+                      b->fold_alt, b->part_alt};
   for (uint32_t *p : bufs)
     if (p) (void)hipFree(p);
   if (b->mfma_a) (void)hipFree(b->mfma_a);
@@ -761,7 +767,12 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
     A(&b->dig, cap * b->dig_q);
     A(&b->snap, 2 * (size_t)std::min(capacity, SNAP_QUERIES) * ((size_t)1 << g) * 2 * N);
   }
-  if (t_gsw == 8) A(&b->part, cap * db->n2 * (size_t)(C / 2) * 8 * N);  // split-fold partial sums
+  if (t_gsw == 8) {
+    A(&b->part, cap * db->n2 * (size_t)(C / 2) * 8 * N);  // split-fold partial sums
+    const size_t fq = (size_t)std::min(capacity, FOLD_QUERIES) * db->n2 * std::max(C / 4, 1);
+    A(&b->fold_alt, fq * 4 * N);
+    A(&b->part_alt, fq * 16 * N);
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->io, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
@@ -1170,8 +1181,17 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
     const bool fast = b->fold_mode != 0;
     const int mx = 2 * b->t_gsw;
     const size_t level = (size_t)2 * mx * 2 * N;
+    // Latency form for batches of at most FOLD_QUERIES queries (automatic choice only).
+    const bool latency = fast && b->t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] == 0 && n <= FOLD_QUERIES &&
+                         b->fold_alt && b->part_alt;
+    if (latency) {
+      const size_t alt = (size_t)std::max(C / 4, 1);
+      HIP_TRY(launch_fold_latency(b->acc, (size_t)C * 4 * N, b->fold_alt, alt * 4 * N, b->part, (size_t)(C / 2) * 8 * N,
+                                  b->part_alt, alt * 16 * N, db->n2, n, db->nu2, b->fkeys, level, b->f_q, ctx->tables,
+                                  s));
+    }
     int num_per = C;
-    for (int cur = 0; cur < db->nu2; cur++) {
+    for (int cur = 0; cur < db->nu2 && !latency; cur++) {
       num_per >>= 1;
       const size_t off = (size_t)(db->nu2 - 1 - cur) * level;
       if (fast && b->t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)

@@ -1019,8 +1019,9 @@ __device__ __forceinline__ void fold_residue(const uint32_t (&dw)[16 / NW][8], u
 
 // Fold epilogue: tiles 2m + row hold the NTT-domain sums (P layout, < q) of
 // row `row` mod q_m; waves 0-3 take them out of the NTT domain, combine the
-// residues by CRT and add the result to Y (raw hi/lo planes) mod Q.
-__device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t *__restrict__ Y,
+// residues by CRT, add Ysrc (raw hi/lo planes) mod Q and write the result to Y
+// (Ysrc may be Y).
+__device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t *Y, const uint32_t *Ysrc,
                                           const uint32_t *__restrict__ tw_base, const ModConst &mc0,
                                           const ModConst &mc1, int wave, int lane) {
   uint32_t s[32];
@@ -1045,6 +1046,7 @@ __device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t 
     const int m = wave >> 1, row = wave & 1;
     const uint32_t *xo = tiles[wave ^ 2];
     uint32_t *yh = Y + (size_t)row * 2 * N, *yl = yh + N;
+    const uint32_t *sh = Ysrc + (size_t)row * 2 * N, *sl = sh + N;
 #pragma unroll
     for (int ii = 0; ii < 16; ii++) {
       const int z = 64 * (16 * m + ii) + lane;
@@ -1052,7 +1054,7 @@ __device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t 
       const uint32_t x0 = m ? oth : own, x1 = m ? own : oth;
       uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
       x = (x >= QQ) ? x - QQ : x;
-      x += ((uint64_t)yh[z] << 32) | yl[z];
+      x += ((uint64_t)sh[z] << 32) | sl[z];
       x = (x >= QQ) ? x - QQ : x;
       yh[z] = (uint32_t)(x >> 32);
       yl[z] = (uint32_t)x;
@@ -1107,7 +1109,7 @@ void k_fold8(uint32_t *__restrict__ accs, size_t set_stride, int n_trials, int n
 #pragma unroll
       for (int e = 0; e < PPT; e++) tiles[2 * m + row][PPT * tid + e] = sum[m][row][e];
   __syncthreads();
-  fold_tail(tiles, Y, tw_base, mc0, mc1, wave, lane);
+  fold_tail(tiles, Y, Y, tw_base, mc0, mc1, wave, lane);
 }
 
 // ---- split fold level: partial products + finish ----------------------------
@@ -1195,10 +1197,41 @@ void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_tri
     for (int e = 0; e < PPT; e++) dst[(size_t)row * N + e] = redc64(acc[row][e], q, qinv);
 }
 
+// Tiles 0-3 (256 threads): tile w = sum over g < NG of the polynomials at
+// src(w, g) (layout P, canonical residues mod q_(w >> qshift & 1)).  The loads
+// of four rounds are issued together: a round trip to memory is the cost here.
+template <int NG, int QSHIFT, class Src>
+__device__ __forceinline__ void sum_partials4(uint32_t (*tiles)[LDS_TILE], const Src &src, const ModConst &mc0,
+                                              const ModConst &mc1, int tid) {
+#pragma unroll
+  for (int half = 0; half < 2; half++) {
+    uint4 v[4][NG];
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int w = (4 * half + it) >> 1, k4 = tid + ((4 * half + it) & 1) * 256;
+#pragma unroll
+      for (int g = 0; g < NG; g++) v[it][g] = reinterpret_cast<const uint4 *>(src(w, g))[k4];
+    }
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int w = (4 * half + it) >> 1, k4 = tid + ((4 * half + it) & 1) * 256;
+      const uint32_t q = ((w >> QSHIFT) & 1) ? mc1.q : mc0.q;
+      uint4 a = v[it][0];
+#pragma unroll
+      for (int g = 1; g < NG; g++)  // partial sums are < q: add and reduce once per term
+        a = make_uint4(canon(a.x + v[it][g].x, q), canon(a.y + v[it][g].y, q), canon(a.z + v[it][g].z, q),
+                       canon(a.w + v[it][g].w, q));
+      reinterpret_cast<uint4 *>(tiles[w])[k4] = a;
+    }
+  }
+}
+
 // k_fold_finish<NG>: one 4-wave block per (pair i, trial, query): tile 2m + row =
-// sum over g of part[m][g][row] mod q_m, then the fold epilogue on Y.
+// sum over g of part[m][g][row] mod q_m, then the fold epilogue: Y = Ysrc +
+// from_ntt(sums), Y = accs[set][i], Ysrc = ysrc[set][i] (ysrc may be accs).
 template <int NG>
-__global__ __launch_bounds__(256) void k_fold_finish(uint32_t *__restrict__ accs, size_t set_stride, int n_trials,
+__global__ __launch_bounds__(256) void k_fold_finish(uint32_t *accs, size_t set_stride, int n_trials,
+                                                     const uint32_t *ysrc, size_t ysrc_set_stride,
                                                      const uint32_t *__restrict__ part, size_t part_set_stride,
                                                      const uint32_t *__restrict__ tw_base, ModConst mc0,
                                                      ModConst mc1) {
@@ -1208,21 +1241,184 @@ __global__ __launch_bounds__(256) void k_fold_finish(uint32_t *__restrict__ accs
   const int i = blockIdx.x;
   const int set = blockIdx.z * n_trials + blockIdx.y;
   uint32_t *Y = accs + (size_t)set * set_stride + (size_t)i * 4 * N;
+  const uint32_t *Ysrc = ysrc + (size_t)set * ysrc_set_stride + (size_t)i * 4 * N;
   const uint32_t *pp = part + (size_t)set * part_set_stride + (size_t)i * 2 * NG * 2 * N;  // [m][g][row][P]
-  for (int u = tid; u < N; u += 256) {  // 4 tiles x N/4 uint4
-    const int w = u / (N / 4), k4 = u % (N / 4);
-    const int m = w >> 1, row = w & 1;
-    const uint32_t q = m ? mc1.q : mc0.q;
-    uint4 a = reinterpret_cast<const uint4 *>(pp + (size_t)((m * NG + 0) * 2 + row) * N)[k4];
+  // tile w = 2 m + row
+  sum_partials4<NG, 1>(
+      tiles, [&](int w, int g) { return pp + (size_t)(((w >> 1) * NG + g) * 2 + (w & 1)) * N; }, mc0, mc1, tid);
+  __syncthreads();
+  fold_tail(tiles, Y, Ysrc, tw_base, mc0, mc1, wave, lane);
+}
+
+// ---- fold, latency form -------------------------------------------------------
+// For batches whose fold grids leave most CUs idle: level 0 runs k_fold_part<2>
+// alone, every later level L runs one k_fold_fused launch, and one
+// k_fold_finish closes the last level: nu2 + 1 launches instead of 2 nu2.
+// A k_fold_fused block (pair i of level L, residue m, row j, digit half h,
+// trial, query; 4 waves, one transform per SIMD in each phase) owns the key
+// columns c = j + 2 k, k = 4 h .. 4 h + 3.  It first finishes level L - 1 for
+// row j of the two elements the pair reads, X = element num_per + i and Y =
+// element i of level L - 1:
+//   raw(e)_j = src_acc[e]_j + from_ntt(sum over groups of src_part[e][m'][group][j])
+// (wave 2 e' + m': one inverse transform each, CRT with the wave of the other
+// residue), forms its 4 digit differences of X_j - Y_j from those raw values
+// and runs the partial products of level L for its columns into dst_part
+// [pair][m][group = 2 j + h][row][P] (16N words per pair).  The (m = 0, h = 0)
+// blocks of a pair also write raw(Y) (row j each) to dst_acc[i], the Y that
+// level L + 1 (or the finish) adds.  The blocks of a pair repeat the inverse
+// transforms of their row (4 times), which costs nothing in a launch that
+// leaves CUs idle.  src and dst buffers alternate between levels, so no block
+// reads what another block of the same launch writes.
+// NGP: groups of partial sums per pair at level L - 1 (2 after k_fold_part<2>).
+template <int NGP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_fold_fused(const uint32_t *__restrict__ src_acc, size_t src_acc_stride, uint32_t *__restrict__ dst_acc,
+                  size_t dst_acc_stride, const uint32_t *__restrict__ src_part, size_t src_part_stride,
+                  uint32_t *__restrict__ dst_part, size_t dst_part_stride, int n_trials, int num_per,
+                  const uint32_t *__restrict__ cpos, size_t key_qstride, const uint32_t *__restrict__ tw_base,
+                  ModConst mc0, ModConst mc1) {
+  constexpr int NW = 4, NK = 4, PPT = N / (NW * 64);
+  __shared__ uint32_t tiles[NW][LDS_TILE];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int i = blockIdx.x;
+  const int h = blockIdx.y & 1, j = (blockIdx.y >> 1) & 1, m = (blockIdx.y >> 2) & 1, trial = blockIdx.y >> 3;
+  const int set = blockIdx.z * n_trials + trial;
+  // ---- finish of level L - 1, row j: tile 2 e + mm = sum of the partials of element e, residue mm
+  const int e = wave >> 1, mm = wave & 1;
+  // this wave's Y_old values (row j of element e), loaded ahead of the transforms
+  uint32_t yo[2][16];
+  {
+    const uint32_t *sh = src_acc + (size_t)set * src_acc_stride + (size_t)(e ? i : num_per + i) * 4 * N +
+                         (size_t)j * 2 * N;
 #pragma unroll
-    for (int g = 1; g < NG; g++) {  // partial sums are < q: add and reduce once per term
-      const uint4 b = reinterpret_cast<const uint4 *>(pp + (size_t)((m * NG + g) * 2 + row) * N)[k4];
-      a = make_uint4(canon(a.x + b.x, q), canon(a.y + b.y, q), canon(a.z + b.z, q), canon(a.w + b.w, q));
+    for (int ii = 0; ii < 16; ii++) {
+      const int z = 64 * (16 * mm + ii) + lane;
+      yo[0][ii] = sh[z];
+      yo[1][ii] = sh[N + z];
     }
-    reinterpret_cast<uint4 *>(tiles[w])[k4] = a;
+  }
+  {
+    const uint32_t *pb = src_part + (size_t)set * src_part_stride;
+    sum_partials4<NGP, 0>(
+        tiles,
+        [&](int w, int gp) {
+          return pb + (size_t)((w >> 1) ? i : num_per + i) * 4 * NGP * N + (size_t)(((w & 1) * NGP + gp) * 2 + j) * N;
+        },
+        mc0, mc1, tid);
   }
   __syncthreads();
-  fold_tail(tiles, Y, tw_base, mc0, mc1, wave, lane);
+  const ModConst mcm = mm ? mc1 : mc0;
+  uint32_t s[32];
+  load_p(s, tiles[wave], lane);
+  wave_lds_sync();
+  ntt_inv_wave(s, tw_base + (mm * 2 + 1) * TW_WORDS, tiles[wave], mcm.q, lane);
+#pragma unroll
+  for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mcm.crt_c, mcm.crt_cp, mcm.q), mcm.q);
+  __syncthreads();
+#pragma unroll
+  for (int ii = 0; ii < 16; ii++) tiles[wave][ii * 64 + lane] = s[16 * (1 - mm) + ii];
+  __syncthreads();
+  // raw value of coefficient z = 64 (16 mm + ii) + lane of row j of element e
+  uint64_t xr[16];
+  {
+    const uint32_t *xo = tiles[wave ^ 1];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+      const uint32_t own = s[16 * mm + ii], oth = xo[ii * 64 + lane];
+      const uint32_t x0 = mm ? oth : own, x1 = mm ? own : oth;
+      uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
+      x = (x >= QQ) ? x - QQ : x;
+      x += ((uint64_t)yo[0][ii] << 32) | yo[1][ii];
+      xr[ii] = (x >= QQ) ? x - QQ : x;
+    }
+  }
+  __syncthreads();  // the exchange tiles are dead
+  if (e == 1) {  // Y: raw values to LDS for the X waves; the (m = 0, h = 0) blocks store them
+    uint32_t *ty = tiles[wave];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+      ty[ii * 64 + lane] = (uint32_t)(xr[ii] >> 32);
+      ty[1024 + ii * 64 + lane] = (uint32_t)xr[ii];
+    }
+    if (m == 0 && h == 0) {
+      uint32_t *dh = dst_acc + (size_t)set * dst_acc_stride + (size_t)i * 4 * N + (size_t)j * 2 * N;
+#pragma unroll
+      for (int ii = 0; ii < 16; ii++) {
+        const int z = 64 * (16 * mm + ii) + lane;
+        dh[z] = (uint32_t)(xr[ii] >> 32);
+        dh[N + z] = (uint32_t)xr[ii];
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t *dig = &tiles[0][0];  // [k - 4 h][DIG_WORDS], 2048 words in tile 0
+  if (e == 0) {  // X: digits 4 h .. 4 h + 3 of X_j - Y_j, coefficient 64 (4 i4 + b) + lane at byte b of word i4
+    const uint32_t *ty = tiles[wave + 2];
+    uint32_t packed[NK][4];
+#pragma unroll
+    for (int kl = 0; kl < NK; kl++)
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) packed[kl][q4] = 0;
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+      const uint64_t x = xr[ii];
+      const uint64_t y = ((uint64_t)ty[ii * 64 + lane] << 32) | ty[1024 + ii * 64 + lane];
+#pragma unroll
+      for (int kl = 0; kl < NK; kl++) {
+        const int sh = 7 * (NK * h + kl);
+        const int d = (int)((x >> sh) & 127u) - (int)((y >> sh) & 127u);
+        packed[kl][ii >> 2] |= ((uint32_t)d & 0xFFu) << (8 * (ii & 3));
+      }
+    }
+#pragma unroll
+    for (int kl = 0; kl < NK; kl++)
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) dig[kl * DIG_WORDS + (4 * mm + q4) * 64 + lane] = packed[kl][q4];
+  }
+  __syncthreads();
+  // ---- partial products of level L: wave w transforms the digit of column c = j + 2 (4 h + w)
+  uint32_t dw[8];
+#pragma unroll
+  for (int i4 = 0; i4 < 8; i4++) dw[i4] = dig[wave * DIG_WORDS + i4 * 64 + lane];
+  __syncthreads();  // the digit bytes are dead from here on
+  const uint32_t q = m ? mc1.q : mc0.q, mu1 = m ? mc1.mu1 : mc0.mu1, qinv = m ? mc1.qinv_neg : mc0.qinv_neg;
+  {
+    uint32_t r[32];
+#pragma unroll
+    for (int i4 = 0; i4 < 8; i4++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) r[4 * i4 + b] = (uint32_t)((int32_t)(int8_t)(dw[i4] >> (8 * b)) + (int32_t)q);
+    // inputs in (q - 128, q + 128) < 2q: stages 1-7 and 9-11 skip the reduction (outputs < 10q)
+    ntt_fwd_wave_t<7, true>(r, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
+    store_tile_p_full(tiles[wave], r, lane, q, mu1);
+  }
+  __syncthreads();
+  const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
+  uint64_t acc[2][PPT];
+#pragma unroll
+  for (int rr = 0; rr < 2; rr++)
+#pragma unroll
+    for (int ee = 0; ee < PPT; ee++) acc[rr][ee] = 0;
+#pragma unroll
+  for (int w2 = 0; w2 < NW; w2++) {
+    const int c = j + 2 * (NK * h + w2);
+    const uint32_t *k0 = kp + ((size_t)(m * 16 + c) * 2) * N + PPT * tid;
+    const uint32_t *k1 = k0 + N;
+    const uint32_t *tl = &tiles[w2][PPT * tid];
+#pragma unroll
+    for (int ee = 0; ee < PPT; ee++) {
+      const uint32_t d = tl[ee];
+      acc[0][ee] += (uint64_t)d * k0[ee];
+      acc[1][ee] += (uint64_t)d * k1[ee];
+    }
+  }
+  uint32_t *dst =
+      dst_part + (size_t)set * dst_part_stride + ((((size_t)i * 2 + m) * 4 + 2 * j + h) * 2) * N + PPT * tid;
+#pragma unroll
+  for (int rr = 0; rr < 2; rr++)
+#pragma unroll
+    for (int ee = 0; ee < PPT; ee++) dst[(size_t)rr * N + ee] = redc64(acc[rr][ee], q, qinv);
 }
 
 // ---- Regev -> GSW conversion ------------------------------------------------
@@ -1995,8 +2191,8 @@ hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_q
           accs, set_stride, n_trials, num_per, cpos, key_qstride, part, part_set_stride, t.tw, t.mc[0], t.mc[1]);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
-      k_fold_finish<4><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, part, part_set_stride, t.tw, t.mc[0],
-                                            t.mc[1]);
+      k_fold_finish<4><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, accs, set_stride, part, part_set_stride,
+                                            t.tw, t.mc[0], t.mc[1]);
       return hipGetLastError();
     }
     k_fold_part<2><<<dim3(num_per, n_trials * 4, n_queries), 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos,
@@ -2004,7 +2200,8 @@ hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_q
                                                                            t.mc[0], t.mc[1]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_fold_finish<2><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, part, part_set_stride, t.tw, t.mc[0], t.mc[1]);
+    k_fold_finish<2><<<grid, 256, 0, s>>>(accs, set_stride, n_trials, accs, set_stride, part, part_set_stride, t.tw,
+                                          t.mc[0], t.mc[1]);
     return hipGetLastError();
   }
   if (fold_expand_waves(pairs, t, variant, true) == 16)
@@ -2012,6 +2209,49 @@ hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_q
                                       t.mc[1]);
   else
     k_fold8<8><<<grid, 512, 0, s>>>(accs, set_stride, n_trials, num_per, cpos, key_qstride, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_latency(uint32_t *accs, size_t set_stride, uint32_t *alt_acc, size_t alt_set_stride,
+                               uint32_t *part, size_t part_set_stride, uint32_t *alt_part, size_t alt_part_set_stride,
+                               int n_trials, int n_queries, int nu2, const uint32_t *fkeys, size_t level_words,
+                               size_t key_qstride, const DevTables &t, hipStream_t s) {
+  int num_per = 1 << (nu2 - 1);
+  const uint32_t *keys0 = fkeys + (size_t)(nu2 - 1) * level_words;  // level 0 uses the last key level
+  k_fold_part<2><<<dim3(num_per, n_trials * 4, n_queries), 512, 0, s>>>(accs, set_stride, n_trials, num_per, keys0,
+                                                                         key_qstride, part, part_set_stride, t.tw,
+                                                                         t.mc[0], t.mc[1]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint32_t *src_acc = accs, *src_part = part;
+  size_t src_as = set_stride, src_ps = part_set_stride;
+  int ngp = 2;  // partial-sum groups per pair of the previous level
+  for (int cur = 1; cur < nu2; cur++) {
+    num_per >>= 1;
+    uint32_t *dst_acc = (cur & 1) ? alt_acc : accs, *dst_part = (cur & 1) ? alt_part : part;
+    const size_t dst_as = (cur & 1) ? alt_set_stride : set_stride;
+    const size_t dst_ps = (cur & 1) ? alt_part_set_stride : part_set_stride;
+    const dim3 grid(num_per, n_trials * 8, n_queries);
+    const uint32_t *keys = fkeys + (size_t)(nu2 - 1 - cur) * level_words;
+    if (ngp == 2)
+      k_fold_fused<2><<<grid, 256, 0, s>>>(src_acc, src_as, dst_acc, dst_as, src_part, src_ps, dst_part, dst_ps,
+                                           n_trials, num_per, keys, key_qstride, t.tw, t.mc[0], t.mc[1]);
+    else
+      k_fold_fused<4><<<grid, 256, 0, s>>>(src_acc, src_as, dst_acc, dst_as, src_part, src_ps, dst_part, dst_ps,
+                                           n_trials, num_per, keys, key_qstride, t.tw, t.mc[0], t.mc[1]);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    src_acc = dst_acc; src_part = dst_part;
+    src_as = dst_as; src_ps = dst_ps;
+    ngp = 4;
+  }
+  const dim3 fgrid(1, n_trials, n_queries);
+  if (ngp == 2)
+    k_fold_finish<2><<<fgrid, 256, 0, s>>>(accs, set_stride, n_trials, src_acc, src_as, src_part, src_ps, t.tw,
+                                           t.mc[0], t.mc[1]);
+  else
+    k_fold_finish<4><<<fgrid, 256, 0, s>>>(accs, set_stride, n_trials, src_acc, src_as, src_part, src_ps, t.tw,
+                                           t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 

@@ -85,6 +85,15 @@ hipError_t launch_regev_to_gsw(bool mont, const uint32_t *v, size_t v_q, int n_q
 hipError_t launch_fold8(uint32_t *accs, size_t set_stride, int n_trials, int n_queries, int num_per,
                         const uint32_t *cpos, size_t key_qstride, const DevTables &t, int variant, uint32_t *part,
                         size_t part_set_stride, hipStream_t s);
+// Whole fold (fast form, gadget 2^7 x 8) in the latency form: k_fold_part<2> for
+// level 0, one k_fold_fused per later level, one k_fold_finish; the response of
+// set (query, trial) ends in accs[set][0].  alt_acc holds 2^nu2 / 4 ciphertexts
+// per set and alt_part 2^nu2 / 4 pairs of partial sums of 16N words per set;
+// part holds the first level's 2^nu2 / 2 pairs of 8N words.
+hipError_t launch_fold_latency(uint32_t *accs, size_t set_stride, uint32_t *alt_acc, size_t alt_set_stride,
+                               uint32_t *part, size_t part_set_stride, uint32_t *alt_part, size_t alt_part_set_stride,
+                               int n_trials, int n_queries, int nu2, const uint32_t *fkeys, size_t level_words,
+                               size_t key_qstride, const DevTables &t, hipStream_t s);
 hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, int num_out, int stop, int max_bits,
                           const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                           const DevTables &t, int variant, hipStream_t s);

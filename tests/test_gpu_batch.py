@@ -58,6 +58,41 @@ def test_batch_slots_match_oracle(gpu_ctx, oracle):
     db.close()
 
 
+@pytest.mark.parametrize("nu1,nu2", [(6, 1), (5, 2), (4, 5)])
+def test_fold_latency_form_matches_oracle_and_split(gpu_ctx, oracle, nu1, nu2):
+    """Runs of at most two queries fold in the latency form (one launch per level
+    past the first, finish of each level inside the next, alternating buffers);
+    with nu2 = 1, 2, 5 the chain ends in place or out of place and uses both
+    launch shapes.  The responses equal the oracle's and those of the split
+    form (fold variant 4), in one- and two-query runs."""
+    from spiral_pir.server import Batch
+
+    s, recs, db = _setup(gpu_ctx, nu1, nu2, 950 + nu2)
+    qs = _queries(s, 2, 3500 + 10 * nu2)
+    b = Batch(db, 3)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    db_ntt = oracle.db_encode(recs, s.n2, s.nu1, s.nu2)
+    exp = [oracle.answer(db_ntt, s.n2, s.nu1, s.nu2, s.t_gsw, s.t_exp, q, left, right, vf,
+                         oracle.folding_neg(vf, s.nu2, s.t_gsw)) for (c, idx, q, left, right, vf) in qs]
+    for n in (1, 2):
+        b.run(n)
+        fused = b.responses(n)
+        gpu_ctx.set_variant(1, 4)
+        try:
+            b.run(n)
+            split = b.responses(n)
+        finally:
+            gpu_ctx.set_variant(1, 0)
+        for slot in range(n):
+            np.testing.assert_array_equal(fused[slot], exp[slot], err_msg=f"n={n} slot {slot}")
+            np.testing.assert_array_equal(split[slot], exp[slot], err_msg=f"n={n} slot {slot} split")
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        np.testing.assert_array_equal(c.decode(fused[slot]), recs[idx])
+    b.close()
+    db.close()
+
+
 def test_same_result_alone_and_in_the_mix(gpu_ctx):
     """A query answered alone (batch of 1) and inside batches of different sizes and
     positions gives the same response."""
