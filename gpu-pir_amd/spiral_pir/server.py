@@ -98,6 +98,21 @@ class Database:
         assert d.size == self.shape.records * self.shape.n2 * 2 * 2048
         _lib.call("sp_db_load_ntt", self._h, _ptr(d))
 
+    def load_file(self, path: str) -> None:
+        """A database container (spiral_pir.dbfile): plaintext records or the NTT
+        form, memory-mapped and streamed to the device.  Its shape must be this
+        database's."""
+        from . import dbfile
+
+        shape, kind, payload = dbfile.read(path)
+        if (shape.nu1, shape.nu2, shape.n2) != (self.shape.nu1, self.shape.nu2, self.shape.n2):
+            raise ValueError(f"{path}: shape ({shape.nu1}, {shape.nu2}, n2 = {shape.n2}) is not this database's "
+                             f"({self.shape.nu1}, {self.shape.nu2}, n2 = {self.shape.n2})")
+        if kind == dbfile.KIND_RECORDS:
+            self.load_bytes(payload)
+        else:
+            self.load_ntt(payload)
+
     @property
     def device_bytes(self) -> int:
         return int(_lib.load().sp_db_device_bytes(self._h))
