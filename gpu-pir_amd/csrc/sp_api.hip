@@ -482,8 +482,8 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
     const uint32_t *lr = left_p + (size_t)r * key_level;
     const uint32_t *rr = right_p + (size_t)std::min(r, stop) * key_level;
     if (family == 4) {
-      HIP_TRY(launch_expand_split(v, q_stride, n_queries, r, stop, t_gsw * nu2, lr, rr, left_qstride, right_qstride,
-                                  neg1_r, dig, dig_qstride, ctx->tables, s));
+      HIP_TRY(launch_expand_split(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2, lr, rr, left_qstride,
+                                  right_qstride, dig, dig_qstride, ctx->tables, s));
       continue;
     }
     HIP_TRY(launch_expand_neg1(v, q_stride, n_queries, num_in, neg1_r, ctx->tables, s));

@@ -1861,10 +1861,37 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
 // Round r, gadget 2^7 x 8, in two launches so that one step spreads over
 // three blocks (one for its digits, one per residue for its transforms).
 //
+// Element i is skipped in round r: odd steps past stop_round (expansion.cuh:92-97).
+__device__ __forceinline__ bool expand_skipped(int i, int r, int stop, int max_bits) {
+  return stop > 0 && (i & 1) && (r > stop || (r == stop && (i >> 1) >= max_bits));
+}
+
+// dst (residue m, rows 0-1) = v_neg1 (.) src at 4 consecutive words from p0; the
+// row-0 result also goes to snap (if given).
+__device__ __forceinline__ void neg1_rows4(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                           uint32_t *__restrict__ snap, const uint32_t *__restrict__ neg1,
+                                           int m, int p0, const ModConst &mc) {
+  const uint4 w = *reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + p0);
+  const uint4 wp = *reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + N + p0);
+#pragma unroll
+  for (int row = 0; row < 2; row++) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(src + (size_t)(row * 2 + m) * N + p0);
+    uint4 y;
+    y.x = canon(shoup_mul(x.x, w.x, wp.x, mc.q), mc.q);
+    y.y = canon(shoup_mul(x.y, w.y, wp.y, mc.q), mc.q);
+    y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
+    y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
+    *reinterpret_cast<uint4 *>(dst + (size_t)(row * 2 + m) * N + p0) = y;
+    if (row == 0 && snap) *reinterpret_cast<uint4 *>(snap + (size_t)m * N + p0) = y;
+  }
+}
+
 // k_expand_digits: one 2-wave block per element i of the round.  For
 // i >= 2^r the element is formed first as v[i] = v_neg1[r] * v[i - 2^r] (all
 // four polynomials, written back; the reference computes the product for
-// every i, expansion.cuh:171-219).  For a working step, wave m then takes
+// every i, expansion.cuh:171-219), unless children_formed says that the
+// previous round's launch already formed the working children (then only the
+// extra blocks form theirs).  For a working step, wave m then takes
 // residue m of v_i.a out of the NTT domain, and the block forms the CRT
 // value, applies tau_t (a negated zero becomes Q) and writes the 8 digit
 // bytes of every coefficient to dig[b] (b = the step's index in the working
@@ -1872,7 +1899,8 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
 // the working steps, then `extra` elements that only take the v_neg1
 // product: the odd i >= 2^r skipped at this round, i = odd_start + 2e + 1.
 __global__ __launch_bounds__(128) void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
-                                                       int work, int odd_start, const uint32_t *__restrict__ neg1_r,
+                                                       int work, int odd_start, int children_formed,
+                                                       const uint32_t *__restrict__ neg1_r,
                                                        uint32_t *__restrict__ dig, size_t dig_qstride,
                                                        const uint32_t *__restrict__ tw_base, ModConst mc0,
                                                        ModConst mc1) {
@@ -1888,7 +1916,7 @@ __global__ __launch_bounds__(128) void k_expand_digits(uint32_t *__restrict__ v,
   uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
   uint32_t *vi = vq + (size_t)i * 4 * N;
   uint32_t a[32];
-  if (i >= half) {
+  if (i >= half && !(step && children_formed)) {
     // wave m: rows 0 and 1 of residue m of v_neg1 * v[i - 2^r]
     const uint32_t *src = vq + (size_t)(i - half) * 4 * N;
     const uint4 *w4 = reinterpret_cast<const uint4 *>(neg1_r + (size_t)m * 2 * N);
@@ -1939,10 +1967,10 @@ __global__ __launch_bounds__(128) void k_expand_digits(uint32_t *__restrict__ v,
 // wave k transforms digit k of the step for residue m; then every thread
 // updates residue m of v_i at its 4 positions (expand_update).
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
-void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, const uint32_t *__restrict__ dig,
-                     size_t dig_qstride, const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
-                     size_t left_qstride, size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0,
-                     ModConst mc1) {
+void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits,
+                     const uint32_t *__restrict__ neg1_next, const uint32_t *__restrict__ dig, size_t dig_qstride,
+                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
+                     size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
   __shared__ uint32_t tiles[8][LDS_TILE];
   const int b = blockIdx.x, half = 1 << r, m = blockIdx.z;
   const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
@@ -1966,6 +1994,13 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   store_tile_p_full(tiles[wave], rr, lane, q, mu1);
   __syncthreads();
   expand_update<4>(vi, W, tiles, 0, m, 4 * tid, t, q, qinv);
+  // Round r + 1's child v[i + 2^(r+1)] = v_neg1[r+1] (.) v_i (residue m, this
+  // thread's positions, just written by this thread), unless i is skipped in
+  // round r + 1; k_expand_digits of round r + 1 then reads it as formed.
+  if (neg1_next && !expand_skipped(i, r + 1, stop, max_bits)) {
+    const ModConst mcm = m ? mc1 : mc0;
+    neg1_rows4(vi, vi + (size_t)(2 * half) * 4 * N, nullptr, neg1_next, m, 4 * tid, mcm);
+  }
 }
 
 // ---- latency form of an expansion round (few steps) -------------------------
@@ -1983,31 +2018,8 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
 //   * blocks past the working steps form the v_neg1 children of skipped
 //     elements (odd, past stop_round), which no block of the round writes.
 // Round 0's snapshot and child come from k_expand_prep.  A run uses this form
-// for a prefix of the rounds, then the split kernels (which recompute the
-// children themselves).
-__device__ __forceinline__ bool expand_skipped(int i, int r, int stop, int max_bits) {
-  return stop > 0 && (i & 1) && (r > stop || (r == stop && (i >> 1) >= max_bits));
-}
-
-// dst (residue m, rows 0-1) = v_neg1 (.) src at 4 consecutive words from p0; the
-// row-0 result also goes to snap (if given).
-__device__ __forceinline__ void neg1_rows4(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                           uint32_t *__restrict__ snap, const uint32_t *__restrict__ neg1,
-                                           int m, int p0, const ModConst &mc) {
-  const uint4 w = *reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + p0);
-  const uint4 wp = *reinterpret_cast<const uint4 *>(neg1 + (size_t)m * 2 * N + N + p0);
-#pragma unroll
-  for (int row = 0; row < 2; row++) {
-    const uint4 x = *reinterpret_cast<const uint4 *>(src + (size_t)(row * 2 + m) * N + p0);
-    uint4 y;
-    y.x = canon(shoup_mul(x.x, w.x, wp.x, mc.q), mc.q);
-    y.y = canon(shoup_mul(x.y, w.y, wp.y, mc.q), mc.q);
-    y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
-    y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
-    *reinterpret_cast<uint4 *>(dst + (size_t)(row * 2 + m) * N + p0) = y;
-    if (row == 0 && snap) *reinterpret_cast<uint4 *>(snap + (size_t)m * N + p0) = y;
-  }
-}
+// for a prefix of the rounds, then the split kernels, whose first round finds
+// its working children formed by this epilogue.
 
 // Round 0 of the latency form: v[0] = the query (when `query` is given; else
 // v[0] is in place), S_0[0] = v[0].a, v[1] = v_neg1[0] (.) v[0], S_0[1] = v[1].a.
@@ -2143,18 +2155,24 @@ hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int 
   return hipGetLastError();
 }
 
-hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int stop, int max_bits,
+hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
-                               const uint32_t *neg1_r, uint32_t *dig, size_t dig_qstride, const DevTables &t,
-                               hipStream_t s) {
+                               uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s) {
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
-  k_expand_digits<<<dim3(work + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, work, odd_start, neg1_r, dig,
-                                                                dig_qstride, t.tw, t.mc[0], t.mc[1]);
+  const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
+  const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
+  // From round 1 on, the previous round's update (or latency-form) launch has
+  // formed the children of its working elements; the extra blocks still form
+  // the children of skipped elements (and, at stop_round, recompute children
+  // already formed: same parent, same value).
+  k_expand_digits<<<dim3(work + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, work, odd_start, r > 0 ? 1 : 0,
+                                                                neg1_r, dig, dig_qstride, t.tw, t.mc[0], t.mc[1]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_expand_update<<<dim3(work, n_queries, 2), 512, 0, s>>>(v, q_stride, r, stop, dig, dig_qstride, left, right,
-                                                           left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
+  k_expand_update<<<dim3(work, n_queries, 2), 512, 0, s>>>(v, q_stride, r, stop, max_bits, neg1_next, dig, dig_qstride,
+                                                           left, right, left_qstride, right_qstride, t.tw, t.mc[0],
+                                                           t.mc[1]);
   return hipGetLastError();
 }
 

@@ -115,10 +115,11 @@ int expand_fused_blocks(int r, int stop, int max_bits);
 hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s);
-hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int stop, int max_bits,
+// Round r of a run of `end` rounds in the split form (digits, then per-residue
+// transforms and update; the update also forms round r + 1's children).
+hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
-                               const uint32_t *neg1_r, uint32_t *dig, size_t dig_qstride, const DevTables &t,
-                               hipStream_t s);
+                               uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s);
 
 // int8-limb MFMA database multiply (C >= 32, R multiple of 32).
 //   D: database limbs [t][m][p][R/32][4][C][32] (k_db_to_limbs, one-time)
