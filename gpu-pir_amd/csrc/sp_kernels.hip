@@ -2474,20 +2474,27 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma(const uint8_t *__restrict__ 
 // block per CU and loads one j block ahead the multiply ran at the latency-bound
 // ~1.4 TB/s of database limbs.  Here each thread loads three 16-byte pieces per
 // j block (selector limbs 0-1 and 2-3 of one row, database limbs of one
-// column) into a ring of PD register slots; the block's 8 waves read their
-// fragments from a double-buffered LDS stage.  Rows past I and columns past C
-// load row / column 0 and are zeroed.
+// column) into a ring of PD register slots, and the block's 8 waves read their
+// fragments from LDS stages.  With one barrier per j block, the two waves of a
+// SIMD used to read their fragments together and then wait for each other's
+// multiplies (MFMA busy 43% at 64 queries); now the fragments of block jb + 1
+// are read while the multiplies of block jb run: three LDS stages hold blocks
+// jb (done), jb + 1 (being read) and jb + 2 (being written).  Rows past I and
+// columns past C load row / column 0 and are zeroed.
 struct MfmaPiece {
   uint4 a0, a1, d;
 };
 
-template <int PD>
+// JBC: the number of j blocks when known at compile time (the loop is then
+// unrolled whole, so that the waits for the ring's loads count exactly the
+// loads issued after them; 0 = R / 32 at run time).
+template <int PD, int JBC>
 __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
                                                          uint32_t *__restrict__ outp, int T, int R, int C, int I,
                                                          int n_ig, int n_cg, ModConst mc0, ModConst mc1,
                                                          const uint32_t *__restrict__ pw) {
-  __shared__ uint4 sA[2][4 * 128 * 2];
-  __shared__ uint4 sD[2][4 * 64 * 2];
+  __shared__ uint4 sA[3][4 * 128 * 2];
+  __shared__ uint4 sD[3][4 * 64 * 2];
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   int bid = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
@@ -2496,7 +2503,7 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
   const int ig = bid % n_ig; bid /= n_ig;
   const int p = bid % N;
   const int m = bid / N;
-  const int JB = R / 32;
+  const int JB = JBC ? JBC : R / 32;
   const int ar = (tid >> 1) & 127, al = tid >> 8, hh = tid & 1;
   const int dl = tid >> 7, dc = (tid >> 1) & 63;
   const bool a_ok = ig * 128 + ar < I, d_ok = cg * 64 + dc < C;
@@ -2506,56 +2513,74 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
   const uint8_t *a1 = a0 + (size_t)2 * I * 32;
   const uint8_t *d0 = D + ((((size_t)t * 2 + m) * N + p) * JB * 4 * C + (size_t)dl * C + (d_ok ? cg * 64 + dc : 0)) * 32 +
                       16 * hh;
+  auto masked = [](uint4 x, uint32_t mk) { return make_uint4(x.x & mk, x.y & mk, x.z & mk, x.w & mk); };
+  auto load = [&](MfmaPiece &pc, int jb) {
+    pc.a0 = *reinterpret_cast<const uint4 *>(a0 + jb * a_jb);
+    pc.a1 = *reinterpret_cast<const uint4 *>(a1 + jb * a_jb);
+    pc.d = *reinterpret_cast<const uint4 *>(d0 + jb * d_jb);
+  };
+  auto stage = [&](const MfmaPiece &pc, int st) {
+    sA[st][tid] = masked(pc.a0, ma);
+    sA[st][tid + 512] = masked(pc.a1, ma);
+    sD[st][tid] = masked(pc.d, md);
+  };
+  const int rw = wave >> 1, cw = wave & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  auto frags = [&](v4i (&fa)[4], v4i (&fd)[4], int st) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const uint4 x = sA[st][(a * 128 + rw * 32 + l32) * 2 + h];
+      const uint4 y = sD[st][(a * 64 + cw * 32 + l32) * 2 + h];
+      fa[a] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+      fd[a] = v4i{(int)y.x, (int)y.y, (int)y.z, (int)y.w};
+    }
+  };
+  // Prologue: blocks 0 and 1 staged, blocks 2 .. PD + 1 in the ring (slot b % PD).
+  {
+    MfmaPiece p0, p1;
+    load(p0, 0);
+    load(p1, min(1, JB - 1));
+    stage(p0, 0);
+    stage(p1, 1);
+  }
   MfmaPiece ring[PD];
 #pragma unroll
   for (int k = 0; k < PD; k++) {
-    const int jb = min(k, JB - 1);
-    ring[k].a0 = *reinterpret_cast<const uint4 *>(a0 + jb * a_jb);
-    ring[k].a1 = *reinterpret_cast<const uint4 *>(a1 + jb * a_jb);
-    ring[k].d = *reinterpret_cast<const uint4 *>(d0 + jb * d_jb);
+    const int b = 2 + ((k - 2 + 2 * PD) % PD);  // the block of ring slot k among 2 .. PD + 1
+    load(ring[k], min(b, JB - 1));
   }
-  auto masked = [](uint4 x, uint32_t mk) { return make_uint4(x.x & mk, x.y & mk, x.z & mk, x.w & mk); };
-  sA[0][tid] = masked(ring[0].a0, ma);
-  sA[0][tid + 512] = masked(ring[0].a1, ma);
-  sD[0][tid] = masked(ring[0].d, md);
   v16i acc[7];
 #pragma unroll
   for (int w = 0; w < 7; w++)
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[w][v] = 0;
-  const int rw = wave >> 1, cw = wave & 1;
-  const int h = lane >> 5, l32 = lane & 31;
+  __syncthreads();
+  v4i ca[4], cd[4];
+  frags(ca, cd, 0);
+  // PD divides JB (both powers of two, PD <= JB): no partial ring pass.
+#pragma unroll
   for (int jb0 = 0; jb0 < JB; jb0 += PD) {
 #pragma unroll
     for (int k = 0; k < PD; k++) {
       const int jb = jb0 + k;
-      if (jb < JB) {
-        // slot k held block jb (already in LDS): refill it with block jb + PD
-        const int jn = min(jb + PD, JB - 1);
-        ring[k].a0 = *reinterpret_cast<const uint4 *>(a0 + jn * a_jb);
-        ring[k].a1 = *reinterpret_cast<const uint4 *>(a1 + jn * a_jb);
-        ring[k].d = *reinterpret_cast<const uint4 *>(d0 + jn * d_jb);
-        __syncthreads();
-        const uint4 *la = sA[jb & 1], *ld = sD[jb & 1];
-        v4i fa[4], fd[4];
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-          const uint4 x = la[(a * 128 + rw * 32 + l32) * 2 + h];
-          const uint4 y = ld[(a * 64 + cw * 32 + l32) * 2 + h];
-          fa[a] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
-          fd[a] = v4i{(int)y.x, (int)y.y, (int)y.z, (int)y.w};
-        }
+      {
+        __syncthreads();  // stage (jb + 1) % 3 written; stage (jb + 2) % 3 no longer read
+        v4i na[4], nd[4];
+        frags(na, nd, (jb + 1) % 3);  // past the last block: a stale stage, never used
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
           for (int b = 0; b < 4; b++)
-            acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fd[b], acc[a + b], 0, 0, 0);
-        // block jb + 1 (slot k + 1) into the other stage; never read after the last block
-        const MfmaPiece &nx = ring[(k + 1) % PD];
-        const int st = (jb + 1) & 1;
-        sA[st][tid] = masked(nx.a0, ma);
-        sA[st][tid + 512] = masked(nx.a1, ma);
-        sD[st][tid] = masked(nx.d, md);
+            acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca[a], cd[b], acc[a + b], 0, 0, 0);
+        // block jb + 2 (ring slot (jb + 2) % PD) into its stage, then that slot
+        // loads block jb + 2 + PD
+        MfmaPiece &sl = ring[(k + 2) % PD];
+        stage(sl, (jb + 2) % 3);
+        load(sl, min(jb + 2 + PD, JB - 1));
+        // keep the loads here: sunk toward their use they would leave too few in flight
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int a = 0; a < 4; a++) { ca[a] = na[a]; cd[a] = nd[a]; }
       }
     }
   }
@@ -2627,10 +2652,23 @@ hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_
   if (e != hipSuccess) return e;
   const int n_ig = (I + 127) / 128, n_cg = (C + 63) / 64;
   const unsigned blocks = (unsigned)(2 * N * n_ig * n_cg * T);  // 2N is a multiple of 8
+  const int JB = R / 32;  // ring depth: 4 j blocks ahead, or all of them when there are fewer (a power of two)
   if (ring == 0)
     k_dbmul_mfma<<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
-  else
-    k_dbmul_mfma_ring<4><<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
+  else {
+#define SP_RING(PDV, JBV) \
+  k_dbmul_mfma_ring<PDV, JBV><<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw)
+    switch (JB) {
+      case 1: SP_RING(1, 1); break;
+      case 2: SP_RING(2, 2); break;
+      case 4: SP_RING(4, 4); break;
+      case 8: SP_RING(4, 8); break;
+      case 16: SP_RING(4, 16); break;
+      case 32: SP_RING(4, 32); break;
+      default: SP_RING(4, 0); break;
+    }
+#undef SP_RING
+  }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int TIC = T * I * C;
