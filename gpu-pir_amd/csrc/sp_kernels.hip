@@ -1898,13 +1898,13 @@ __device__ __forceinline__ void neg1_rows4(const uint32_t *__restrict__ src, uin
 // order of k_expand8, layout as the digit bytes there).  Grid x: [0, work)
 // the working steps, then `extra` elements that only take the v_neg1
 // product: the odd i >= 2^r skipped at this round, i = odd_start + 2e + 1.
-__global__ __launch_bounds__(128) void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
+void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
                                                        int work, int odd_start, int children_formed,
                                                        const uint32_t *__restrict__ neg1_r,
                                                        uint32_t *__restrict__ dig, size_t dig_qstride,
                                                        const uint32_t *__restrict__ tw_base, ModConst mc0,
                                                        ModConst mc1) {
-  __shared__ uint32_t xm[2 * N];
   __shared__ uint32_t tiles[2][LDS_TILE];
   const int b = blockIdx.x, half = 1 << r;
   const bool step = b < work;
@@ -1934,21 +1934,36 @@ __global__ __launch_bounds__(128) void k_expand_digits(uint32_t *__restrict__ v,
         y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
         y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
         d4[c * 64 + lane] = y;
-        if (row == 0) { a[4 * c] = y.x; a[4 * c + 1] = y.y; a[4 * c + 2] = y.z; a[4 * c + 3] = y.w; }
       }
     }
-  } else {
-    load_p(a, vi + (size_t)m * N, lane);
   }
   if (!step) return;  // uniform per block
+  load_p(a, vi + (size_t)m * N, lane);  // (a child just formed above is this lane's own stores)
   ntt_inv_wave(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, lane);
 #pragma unroll
-  for (int ii = 0; ii < 32; ii++) xm[m * N + 64 * ii + lane] = canon(shoup_mul(a[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  for (int ii = 0; ii < 32; ii++) a[ii] = canon(shoup_mul(a[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  // Wave m combines the coefficients 64 (16 m + jj) + lane: it hands the
+  // other half to the other wave through its own tile and takes the other
+  // residue of its half from the other wave's tile (tiles of 2112 words: the
+  // block needs no separate CRT buffer, so more blocks fit per CU).
+  uint32_t own[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) {
+    own[jj] = m ? a[16 + jj] : a[jj];
+    tiles[m][jj * 64 + lane] = m ? a[jj] : a[16 + jj];
+  }
   __syncthreads();
+  uint32_t oth[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) oth[jj] = tiles[1 - m][jj * 64 + lane];
+  __syncthreads();  // the tiles become the digit bytes
   const uint32_t t = (uint32_t)(N >> r) + 1u;
   uint8_t *db = reinterpret_cast<uint8_t *>(&tiles[0][0]);  // 4096 words over tiles 0-1
-  for (int z = tid; z < N; z += 128) {
-    uint64_t x = (uint64_t)xm[z] * Q1 + (uint64_t)xm[N + z] * Q0;
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) {
+    const int z = 64 * (16 * m + jj) + lane;
+    const uint32_t x0 = m ? oth[jj] : own[jj], x1 = m ? own[jj] : oth[jj];
+    uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
     x = (x >= QQ) ? x - QQ : x;
     const uint32_t zt = (uint32_t)z * t;
     const uint32_t dst = zt & (N - 1);
