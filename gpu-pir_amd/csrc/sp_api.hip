@@ -98,8 +98,10 @@ static void build_tw(uint32_t *out, int m, int inv) {
   std::vector<uint32_t> T(N);
   for (uint32_t i = 0; i < (uint32_t)N; i++) T[i] = (uint32_t)powmod(root, brv11_h(i), q);
   std::fill(out, out + TW_WORDS, 0u);
+  // forward tables: -w mod 2^32 in place of w (ct_bf* use y * (-w) + ...)
+  const auto wv = [&](uint32_t w) { return inv ? w : 0u - w; };
   for (int idx = 1; idx < 32; idx++) {
-    out[2 * idx] = T[idx];
+    out[2 * idx] = wv(T[idx]);
     out[2 * idx + 1] = shoup_companion(T[idx], q);
   }
   for (int lane = 0; lane < 64; lane++) {
@@ -112,7 +114,7 @@ static void build_tw(uint32_t *out, int m, int inv) {
       else if (s < 15) idx = 256 + 8 * L + (s - 7);
       else if (s < 31) idx = 512 + 16 * L + (s - 15);
       else idx = 1024 + 32 * L + 16 * b + (s - 31);
-      out[TW_A_WORDS + (2 * s) * 64 + lane] = T[idx];
+      out[TW_A_WORDS + (2 * s) * 64 + lane] = wv(T[idx]);
       out[TW_A_WORDS + (2 * s + 1) * 64 + lane] = shoup_companion(T[idx], q);
     }
   }

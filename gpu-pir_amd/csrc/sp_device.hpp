@@ -40,6 +40,8 @@ constexpr int LDS_TILE = 2112;  // words of wave-private transpose scratch
 //   [0, 64)          phase-A twiddles: (w, w') for index 1..31, uniform per wave
 //   [64, 64+47*128)  phase-B slots: slot s -> w at 64 + (2s)*64 + lane,
 //                    w' at 64 + (2s+1)*64 + lane
+// w' is the Shoup companion floor(w 2^32 / q); the forward tables hold
+// -w mod 2^32 in place of w (the forward butterflies' multiply-add form).
 constexpr int TW_A_WORDS = 64;
 constexpr int TW_B_SLOTS = 47;
 constexpr int TW_WORDS = TW_A_WORDS + TW_B_SLOTS * 2 * 64;  // 6080
@@ -80,31 +82,43 @@ __device__ __forceinline__ uint32_t canon4(uint32_t x, uint32_t q) {
   return min(x, x - q);
 }
 
-__device__ __forceinline__ void ct_bf(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
+// Forward butterflies take the twiddle negated: nw = -w mod 2^32 (the
+// forward tables hold nw in place of w, with the Shoup companion wp of w).
+// Then nt = y * nw + floor(y * wp / 2^32) * q is -t mod 2^32 for the Shoup
+// product t = y * w mod q in [0, 2q), which costs one multiply and one
+// multiply-add (v_mad_u64_u32), and y' = x - t + 2q is one three-input add:
+// 5 VALU instructions for a lazy butterfly instead of 7.  (The inverse
+// butterfly keeps the subtract form: a multiply-add there saves one
+// instruction of 9 but costs registers that some kernels cannot spare.)
+__device__ __forceinline__ uint32_t shoup_mul_neg(uint32_t y, uint32_t nw, uint32_t wp, uint32_t q) {
+  return y * nw + __umulhi(y, wp) * q;
+}
+
+__device__ __forceinline__ void ct_bf(uint32_t &x, uint32_t &y, uint32_t nw, uint32_t wp, uint32_t q) {
   // Harvey forward butterfly: inputs [0, 4q), outputs [0, 4q).
   const uint32_t tq = 2 * q;
   const uint32_t xr = reduce_2q(x, tq);
-  const uint32_t t = shoup_mul(y, w, wp, q);
-  y = xr - t + tq;
-  x = xr + t;
+  const uint32_t nt = shoup_mul_neg(y, nw, wp, q);
+  y = xr + tq + nt;
+  x = xr - nt;
 }
 
 // Forward butterfly without the reduction of x: x' = x + t, y' = x - t + 2q
 // with t = y * w mod q in [0, 2q).  If x, y < B q then x', y' < (B + 2) q.
-__device__ __forceinline__ void ct_bf_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
-  const uint32_t t = shoup_mul(y, w, wp, q);
-  y = x - t + 2 * q;
-  x = x + t;
+__device__ __forceinline__ void ct_bf_lazy(uint32_t &x, uint32_t &y, uint32_t nw, uint32_t wp, uint32_t q) {
+  const uint32_t nt = shoup_mul_neg(y, nw, wp, q);
+  y = x + 2 * q + nt;
+  x = x - nt;
 }
 
 // Forward butterfly for any x < 2^32: x is first brought to [0, 2q) by a
 // Shoup multiplication by 1 (mu1 = floor(2^32 / q)); outputs in [0, 4q).
-__device__ __forceinline__ void ct_bf_full(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
+__device__ __forceinline__ void ct_bf_full(uint32_t &x, uint32_t &y, uint32_t nw, uint32_t wp, uint32_t q,
                                            uint32_t mu1) {
   const uint32_t xr = x - __umulhi(x, mu1) * q;
-  const uint32_t t = shoup_mul(y, w, wp, q);
-  y = xr - t + 2 * q;
-  x = xr + t;
+  const uint32_t nt = shoup_mul_neg(y, nw, wp, q);
+  y = xr + 2 * q + nt;
+  x = xr - nt;
 }
 
 // Stage st (1..11) of a forward transform whose first LAZY stages skip the
