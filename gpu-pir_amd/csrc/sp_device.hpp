@@ -87,9 +87,7 @@ __device__ __forceinline__ uint32_t canon4(uint32_t x, uint32_t q) {
 // Then nt = y * nw + floor(y * wp / 2^32) * q is -t mod 2^32 for the Shoup
 // product t = y * w mod q in [0, 2q), which costs one multiply and one
 // multiply-add (v_mad_u64_u32), and y' = x - t + 2q is one three-input add:
-// 5 VALU instructions for a lazy butterfly instead of 7.  (The inverse
-// butterfly keeps the subtract form: a multiply-add there saves one
-// instruction of 9 but costs registers that some kernels cannot spare.)
+// 5 VALU instructions for a lazy butterfly instead of 7.
 __device__ __forceinline__ uint32_t shoup_mul_neg(uint32_t y, uint32_t nw, uint32_t wp, uint32_t q) {
   return y * nw + __umulhi(y, wp) * q;
 }
@@ -137,12 +135,24 @@ __device__ __forceinline__ void ct_stage_bf(int st, uint32_t &x, uint32_t &y, ui
     ct_bf(x, y, w, wp, q);
 }
 
+// -q mod 2^32, kept opaque so that the compiler does not fold y + h * (-q)
+// back into y - h * q (a multiply and a subtract instead of one multiply-add).
+__device__ __forceinline__ uint32_t neg_q(uint32_t q) {
+  uint32_t nq = 0u - q;
+  asm("" : "+v"(nq));
+  return nq;
+}
+
+// Harvey inverse butterfly: inputs [0, 2q), outputs [0, 2q).  MADD computes
+// the Shoup product as v * w + floor(v * wp / 2^32) * (-q) (8 VALU
+// instructions instead of 9); it holds more registers, so only kernels with
+// room to spare use it.
+template <bool MADD = false>
 __device__ __forceinline__ void gs_bf(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
-  // Harvey inverse butterfly: inputs [0, 2q), outputs [0, 2q).
   const uint32_t tq = 2 * q;
   const uint32_t u = reduce_2q(x + y, tq);
   const uint32_t v = x - y + tq;
-  y = shoup_mul(v, w, wp, q);
+  y = MADD ? v * w + __umulhi(v, wp) * neg_q(q) : shoup_mul(v, w, wp, q);
   x = u;
 }
 
@@ -258,12 +268,13 @@ __device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *
 // Inverse negacyclic NTT (without the factor N^-1) of one polynomial.
 // In:  r[j] = NTT position 32*lane + j, values in [0, 2q).
 // Out: r[i] = N * coefficient (64*i + lane) mod q, values in [0, 2q).
+template <bool MADD = false>
 __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                              uint32_t *__restrict__ lds, uint32_t q, int lane) {
   const int L = lane >> 1, b = lane & 1;
 #pragma unroll
   for (int k = 0; k < 16; k++)
-    gs_bf(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
+    gs_bf<MADD>(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
   uint32_t n[32];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -281,7 +292,7 @@ __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *
     for (int blk = 0; blk < nblk; blk++) {
       const uint32_t w = tw_b(tw, nblk - 1 + blk, 0, lane), wp = tw_b(tw, nblk - 1 + blk, 1, lane);
 #pragma unroll
-      for (int i = 0; i < half; i++) gs_bf(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q);
+      for (int i = 0; i < half; i++) gs_bf<MADD>(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q);
     }
   }
 #pragma unroll
@@ -299,7 +310,7 @@ __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *
     for (int blk = 0; blk < nblk; blk++) {
       const uint32_t w = tw_a(tw, nblk + blk, 0), wp = tw_a(tw, nblk + blk, 1);
 #pragma unroll
-      for (int i = 0; i < half; i++) gs_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+      for (int i = 0; i < half; i++) gs_bf<MADD>(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
     }
   }
 }

@@ -1030,7 +1030,7 @@ __device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t 
     const ModConst mc = m ? mc1 : mc0;
     load_p(s, tiles[wave], lane);
     wave_lds_sync();
-    ntt_inv_wave(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+    ntt_inv_wave<true>(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
 #pragma unroll
     for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   }
@@ -1312,7 +1312,7 @@ void k_fold_fused(const uint32_t *__restrict__ src_acc, size_t src_acc_stride, u
   uint32_t s[32];
   load_p(s, tiles[wave], lane);
   wave_lds_sync();
-  ntt_inv_wave(s, tw_base + (mm * 2 + 1) * TW_WORDS, tiles[wave], mcm.q, lane);
+  ntt_inv_wave<true>(s, tw_base + (mm * 2 + 1) * TW_WORDS, tiles[wave], mcm.q, lane);
 #pragma unroll
   for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mcm.crt_c, mcm.crt_cp, mcm.q), mcm.q);
   __syncthreads();
@@ -1732,7 +1732,7 @@ __device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, u
     const ModConst mc = m ? mc1 : mc0;
     uint32_t rr[32];
     load_p(rr, vi + (size_t)m * N, lane);
-    ntt_inv_wave(rr, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+    ntt_inv_wave<true>(rr, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
 #pragma unroll
     for (int ii = 0; ii < 32; ii++) xm[m * N + 64 * ii + lane] = canon(shoup_mul(rr[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   }
@@ -1939,7 +1939,7 @@ void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   }
   if (!step) return;  // uniform per block
   load_p(a, vi + (size_t)m * N, lane);  // (a child just formed above is this lane's own stores)
-  ntt_inv_wave(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, lane);
+  ntt_inv_wave<true>(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, lane);
 #pragma unroll
   for (int ii = 0; ii < 32; ii++) a[ii] = canon(shoup_mul(a[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   // Wave m combines the coefficients 64 (16 m + jj) + lane: it hands the
