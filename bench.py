@@ -41,45 +41,61 @@ def db_mul_algorithmic_bytes(shape) -> int:
     return db + sel + out
 
 
+def _cpu_query_seconds(O, shape, q, left, right, vf, sample_cols: int, threads: int) -> dict:
+    """Seconds per query of the CPU oracle on `threads` threads, from a bounded
+    sample: full expansion, the database multiply of one sub-database restricted
+    to `sample_cols` columns, the fold of one sub-database."""
+    O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        v = O.expand(q, shape.nu1, shape.nu2, shape.t_gsw, shape.t_exp, left, right)
+        t_exp = time.perf_counter() - t0
+        g = np.random.default_rng(4242)
+        R = shape.rows
+        db = np.empty((sample_cols, R, 2, 2048), np.uint32)
+        db[:, :, 0] = g.integers(0, 268369921, size=(sample_cols, R, 2048), dtype=np.uint32)
+        db[:, :, 1] = g.integers(0, 249561089, size=(sample_cols, R, 2048), dtype=np.uint32)
+        sel = np.ascontiguousarray(v[0:2 * R:2])
+        nu2s = int(np.log2(sample_cols))
+        t0 = time.perf_counter()
+        O.db_mul(db, sel, shape.nu1, nu2s, nthreads=None if threads == 1 else threads)
+        t_db = (time.perf_counter() - t0) * (shape.cols / sample_cols) * shape.n2
+        accs = g.integers(0, 2**32, size=(shape.cols, 2, 2, 2048), dtype=np.uint32)
+        accs[:, :, 0] &= np.uint32(0x00FFFFFF)  # raw values below 2^56 > Q is fine for timing
+        vneg = O.folding_neg(vf, shape.nu2, shape.t_gsw)
+        t0 = time.perf_counter()
+        O.fold(accs, shape.nu2, shape.t_gsw, vf, vneg)
+        t_fold = (time.perf_counter() - t0) * shape.n2
+    finally:
+        O.set_threads(1)
+    return {"expand": t_exp, "db_mul": t_db, "fold": t_fold}
+
+
 def cpu_baseline(shape, q, left, right, vf, sample_cols: int) -> dict:
-    """Time the CPU oracle (single thread) on a bounded sample of one query and
-    extrapolate to the whole query: full expansion, the database multiply of one
-    sub-database restricted to `sample_cols` columns, the fold of one
-    sub-database."""
+    """The CPU oracle timed on a bounded sample of one query, extrapolated to the
+    whole query, on all of the host's cores given to this process (OpenMP over
+    the elements of an expansion round, the (column, coefficient) items of the
+    database multiply and the pairs of a fold level) and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     O.lib()
-    t0 = time.perf_counter()
-    v = O.expand(q, shape.nu1, shape.nu2, shape.t_gsw, shape.t_exp, left, right)
-    t_exp = time.perf_counter() - t0
-    g = np.random.default_rng(4242)
-    R = shape.rows
-    db = np.empty((sample_cols, R, 2, 2048), np.uint32)
-    db[:, :, 0] = g.integers(0, 268369921, size=(sample_cols, R, 2048), dtype=np.uint32)
-    db[:, :, 1] = g.integers(0, 249561089, size=(sample_cols, R, 2048), dtype=np.uint32)
-    sel = np.ascontiguousarray(v[0:2 * R:2])
-    nu2s = int(np.log2(sample_cols))
-    t0 = time.perf_counter()
-    O.db_mul(db, sel, shape.nu1, nu2s)
-    t_db = (time.perf_counter() - t0) * (shape.cols / sample_cols) * shape.n2
-    accs = g.integers(0, 2**32, size=(shape.cols, 2, 2, 2048), dtype=np.uint32)
-    accs[:, :, 0] &= np.uint32(0x00FFFFFF)  # raw values below 2^56 > Q is fine for timing
-    vneg = O.folding_neg(vf, shape.nu2, shape.t_gsw)
-    t0 = time.perf_counter()
-    O.fold(accs, shape.nu2, shape.t_gsw, vf, vneg)
-    t_fold = (time.perf_counter() - t0) * shape.n2
-    t_q = t_exp + t_db + t_fold
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+    one = _cpu_query_seconds(O, shape, q, left, right, vf, sample_cols, 1)
+    many = _cpu_query_seconds(O, shape, q, left, right, vf, sample_cols, threads) if threads > 1 else one
+    t1, tn = sum(one.values()), sum(many.values())
+    scale = (f"DB multiply of 1 of {shape.n2} sub-databases x {sample_cols} of {shape.cols} columns "
+             f"(x{shape.n2 * shape.cols // sample_cols}), fold of 1 sub-database (x{shape.n2})")
     return {
-        "value": 1.0 / t_q,
+        "value": 1.0 / tn,
         "unit": "queries/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle/spiral_oracle.c on the host, 1 thread: full expansion ({t_exp:.2f} s), "
-                   f"DB multiply of 1 of {shape.n2} sub-databases x {sample_cols} of {shape.cols} columns "
-                   f"(x{shape.n2 * shape.cols // sample_cols} -> {t_db:.2f} s), fold of 1 sub-database "
-                   f"(x{shape.n2} -> {t_fold:.2f} s); {t_q:.2f} s per query"),
-        "stage_s": {"expand": t_exp, "db_mul": t_db, "fold": t_fold},
+        "sample": (f"oracle/spiral_oracle.c on the host, {threads} threads: full expansion ({many['expand']:.2f} s), "
+                   f"{scale} -> {many['db_mul']:.2f} + {many['fold']:.2f} s; {tn:.2f} s per query. "
+                   f"One thread: {one['expand']:.2f} + {one['db_mul']:.2f} + {one['fold']:.2f} s = {t1:.2f} s per query"),
+        "stage_s": many,
+        "single_thread": {"value": 1.0 / t1, "unit": "queries/s", "cores": 1, "stage_s": one},
     }
 
 

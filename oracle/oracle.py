@@ -56,6 +56,7 @@ def lib():
             "or_db_mul": ([P, P, P, I, I], None),
             "or_db_mul_omp": ([P, P, P, I, I, I], None),
             "or_fold": ([P, I, I, P, P], None),
+            "or_set_threads": ([I], None),
             "or_expansion_g": ([I, I, I], I),
             "or_expansion_stop_round": ([I, I], I),
             "or_v_neg1": ([I, P], None),
@@ -160,6 +161,11 @@ def db_mul(db, first_dim, nu1: int, nu2: int, nthreads: int | None = None) -> np
     else:
         lib().or_db_mul_omp(_p(db), _p(first_dim), _p(out), nu1, nu2, nthreads)
     return out
+
+
+def set_threads(n: int) -> None:
+    """Threads of expand() and fold() (default 1); results do not depend on it."""
+    lib().or_set_threads(int(n))
 
 
 def fold(accs, nu2: int, t_gsw: int, v_folding, v_folding_neg) -> np.ndarray:

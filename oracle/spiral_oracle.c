@@ -57,6 +57,7 @@ static uint32_t g_tw[2][2][OR_N]; /* [m][inverse][i] = psi^(+-brv11(i)) */
 static uint32_t g_ninv[2];
 static uint32_t g_crt_c[2];       /* c0 = q2^-1 mod q1, c1 = q1^-1 mod q2 */
 static int g_init = 0;
+static int g_threads = 1; /* threads of or_expand / or_fold (or_set_threads) */
 
 static uint64_t powmod(uint64_t b, uint64_t e, uint64_t q) {
     uint64_t r = 1 % q;
@@ -212,6 +213,9 @@ void or_automorph(const uint32_t *raw_in, uint32_t *raw_out, uint32_t t) {
 }
 
 /* gpu-pir/gadget.cuh:37-39: bits_per = ceil(MOD_BITS / num_elems), MOD_BITS = 56. */
+/* Threads used by or_expand and or_fold (default 1); results do not depend on it. */
+void or_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
 int or_gadget_bits(int num_elems) { return (56 + num_elems - 1) / num_elems; }
 
 /* gpu-pir/gadget.cuh:55-74 / cpu_version/gadget.py:116-134: row j + k*rdim of
@@ -312,13 +316,18 @@ void or_fold(uint32_t *accs, int nu2, int t_gsw,
     const int mx = 2 * t_gsw;
     const size_t ct_words = 4 * OR_N;
     const size_t level_words = (size_t)2 * mx * 2 * OR_N;
-    uint32_t sum[4 * OR_N];
     int num_per = 1 << nu2;
     for (int cur_dim = 0; cur_dim < nu2; cur_dim++) {
         num_per /= 2;
         const uint32_t *c_pos = v_folding + (size_t)(nu2 - 1 - cur_dim) * level_words;
         const uint32_t *c_neg = v_folding_neg + (size_t)(nu2 - 1 - cur_dim) * level_words;
+        /* pairs of a level are independent: pair i reads elements i and
+         * num_per + i and writes element i only */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic) num_threads(g_threads)
+#endif
         for (int i = 0; i < num_per; i++) {
+            uint32_t sum[4 * OR_N];
             memset(sum, 0, sizeof sum);
             or_extprod_acc(accs + (size_t)(num_per + i) * ct_words, 2, mx, c_pos, sum);
             or_extprod_acc(accs + (size_t)i * ct_words, 2, mx, c_neg, sum);
@@ -364,11 +373,16 @@ void or_expand(uint32_t *v, int nu1, int nu2, int t_gsw, int t_exp,
     const size_t ct_words = 4 * OR_N;
     const size_t key_words = (size_t)2 * t_exp * 2 * OR_N;
     if (rounds < 0 || rounds > g) rounds = g;
-    uint32_t neg1[2 * OR_N], raw[4 * OR_N], au[4 * OR_N], acc[4 * OR_N], aue[2 * OR_N];
+    uint32_t neg1[2 * OR_N];
     for (int r = 0; r < rounds; r++) {
         const int num_in = 1 << r;
         const uint32_t t = (uint32_t)(OR_N / num_in + 1);
         or_v_neg1(r, neg1);
+        /* the elements of a round are independent: each step reads and writes
+         * only its own element (and, above, its own child) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+#endif
         for (int i = 0; i < num_in; i++) {
             const uint32_t *src = v + (size_t)i * ct_words;
             uint32_t *dst = v + (size_t)(num_in + i) * ct_words;
@@ -379,7 +393,11 @@ void or_expand(uint32_t *v, int nu1, int nu2, int t_gsw, int t_exp,
                         dst[o] = (uint32_t)((uint64_t)neg1[m * OR_N + z] * src[o] % Q_[m]);
                     }
         }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic) num_threads(g_threads)
+#endif
         for (int i = 0; i < 2 * num_in; i++) {
+            uint32_t raw[4 * OR_N], au[4 * OR_N], acc[4 * OR_N], aue[2 * OR_N];
             if (stop > 0 && r > stop && (i % 2) == 1) continue;
             if (stop > 0 && r == stop && (i % 2) == 1 && (i / 2) >= max_bits) continue;
             const uint32_t *w = (r != 0 && (i % 2) == 0) ? left + (size_t)r * key_words

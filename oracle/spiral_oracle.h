@@ -86,6 +86,10 @@ void or_db_mul_omp(const uint32_t *db, const uint32_t *first_dim, uint32_t *out,
 
 /* In-place fold of 2^nu2 raw Ciphertexts into accs[0].  v_folding and
  * v_folding_neg are CiphertextHalf[nu2][2 crt][2*t_gsw]. */
+/* Threads used by or_expand and or_fold (default 1; OpenMP over the
+ * independent elements of a round / pairs of a level): results do not depend
+ * on it. */
+void or_set_threads(int n);
 void or_fold(uint32_t *accs, int nu2, int t_gsw,
              const uint32_t *v_folding, const uint32_t *v_folding_neg);
 

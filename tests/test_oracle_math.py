@@ -223,3 +223,28 @@ def test_fold_picks_by_bit(oracle):
     vf = np.stack([zero[0], gad[1]])
     out = oracle.fold(accs, nu2, t_gsw, vf, oracle.folding_neg(vf, nu2, t_gsw))
     np.testing.assert_array_equal(out[0], accs[2])
+
+
+def test_threads_do_not_change_results(oracle):
+    """The oracle's expansion and fold give the same words on 1 and 4 threads
+    (the CPU baseline times them on all host cores)."""
+    g = rng(130)
+    nu1, nu2, t_gsw, t_exp = 6, 2, 8, 8
+    gexp = oracle.expansion_g(nu1, nu2, t_gsw)
+    stop = oracle.expansion_stop_round(nu2, t_gsw)
+    query = rand_residues(g, 2).reshape(2, 2, N)
+    left = rand_residues(g, gexp * 16).reshape(gexp, 2, 8, 2, N)
+    right = rand_residues(g, (stop + 1) * 16).reshape(stop + 1, 2, 8, 2, N)
+    accs = rand_raw(g, 8).reshape(4, 2, 2, N)
+    vf = rand_residues(g, nu2 * 32).reshape(nu2, 2, 16, 2, N)
+    vneg = oracle.folding_neg(vf, nu2, t_gsw)
+    out = {}
+    try:
+        for n in (1, 4):
+            oracle.set_threads(n)
+            out[n] = (oracle.expand(query, nu1, nu2, t_gsw, t_exp, left, right),
+                      oracle.fold(accs, nu2, t_gsw, vf, vneg))
+    finally:
+        oracle.set_threads(1)
+    np.testing.assert_array_equal(out[1][0], out[4][0])
+    np.testing.assert_array_equal(out[1][1], out[4][1])
