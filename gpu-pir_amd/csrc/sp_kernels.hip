@@ -1756,6 +1756,9 @@ __device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, u
 // v_i += W g^-1(tau a) + (0, NTT(tau as_e)) for residue m.  tau(as_e) in the
 // NTT domain is a permutation of positions; it is gathered before `bar`
 // (a block barrier), after which the residue is written.
+// The 8 digit tiles hold the forward transforms' lazy outputs (< 8q, not
+// reduced): with keys < q each sum of 8 products is below 64 q^2 < 2^62 and
+// its REDC below 5q, which redc64 brings to [0, q).
 template <int PPT>
 __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
                                               uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,
@@ -1846,7 +1849,7 @@ void k_expand8(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int m
       // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
       const uint32_t mu1 = m ? mc1.mu1 : mc0.mu1;
       ntt_fwd_wave_t<8, true>(rr, twm, tiles[wave], q, mu1, lane);
-      store_tile_p_full(tiles[wave], rr, lane, q, mu1);
+      store_p(tiles[wave], rr, lane);  // lazy (< 8q): expand_update reduces after the MAC
     }
     __syncthreads();
     // residue updated by this thread, and its positions
@@ -2006,7 +2009,7 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   }
   // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
   ntt_fwd_wave_t<8, true>(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
-  store_tile_p_full(tiles[wave], rr, lane, q, mu1);
+  store_p(tiles[wave], rr, lane);  // lazy (< 8q): expand_update reduces after the MAC
   __syncthreads();
   expand_update<4>(vi, W, tiles, 0, m, 4 * tid, t, q, qinv);
   // Round r + 1's child v[i + 2^(r+1)] = v_neg1[r+1] (.) v_i (residue m, this
@@ -2099,7 +2102,7 @@ void k_expand_fused(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, 
       for (int bb = 0; bb < 4; bb++) rr[4 * i4 + bb] = (dw[i4] >> (8 * bb)) & 0xFFu;
     // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
     ntt_fwd_wave_t<8, true>(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], mcm.q, mcm.mu1, lane);
-    store_tile_p_full(tiles[wave], rr, lane, mcm.q, mcm.mu1);
+    store_p(tiles[wave], rr, lane);  // lazy (< 8q): expand_update reduces after the MAC
   }
   __syncthreads();
   const int p0 = 4 * tid;
