@@ -1142,6 +1142,7 @@ void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_tri
                  const uint32_t *__restrict__ cpos, size_t key_qstride, uint32_t *__restrict__ part,
                  size_t part_set_stride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
   constexpr int NW = 16 / NG, NK = 8 / NG;
+  static_assert(NW <= 8, "lazy digit tiles: at most 8 products per REDC");
   constexpr int PPT = N / (NW * 64);
   __shared__ uint32_t tiles[NW][LDS_TILE];
   uint32_t *dig = &tiles[0][0];  // 2 polys x NK digit polynomials, 1024 * NK words
@@ -1170,7 +1171,9 @@ void k_fold_part(const uint32_t *__restrict__ accs, size_t set_stride, int n_tri
       for (int b = 0; b < 4; b++) r[4 * i4 + b] = (uint32_t)((int32_t)(int8_t)(dw[i4] >> (8 * b)) + (int32_t)q);
     // inputs in (q - 128, q + 128) < 2q: stages 1-7 and 9-11 skip the reduction (outputs < 10q)
     ntt_fwd_wave_t<7, true>(r, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
-    store_tile_p_full(tiles[wave], r, lane, q, mu1);
+    // stored lazy: a sum of at most 8 products with keys < q is below 80 q^2
+    // < 2^63 and its REDC below 6q, which redc64 brings to [0, q)
+    store_p(tiles[wave], r, lane);
   }
   __syncthreads();
   uint64_t acc[2][PPT];
@@ -1391,7 +1394,9 @@ void k_fold_fused(const uint32_t *__restrict__ src_acc, size_t src_acc_stride, u
       for (int b = 0; b < 4; b++) r[4 * i4 + b] = (uint32_t)((int32_t)(int8_t)(dw[i4] >> (8 * b)) + (int32_t)q);
     // inputs in (q - 128, q + 128) < 2q: stages 1-7 and 9-11 skip the reduction (outputs < 10q)
     ntt_fwd_wave_t<7, true>(r, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
-    store_tile_p_full(tiles[wave], r, lane, q, mu1);
+    // stored lazy: a sum of at most 8 products with keys < q is below 80 q^2
+    // < 2^63 and its REDC below 6q, which redc64 brings to [0, q)
+    store_p(tiles[wave], r, lane);
   }
   __syncthreads();
   const uint32_t *kp = cpos + (size_t)blockIdx.z * key_qstride;
