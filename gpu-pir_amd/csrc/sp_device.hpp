@@ -26,6 +26,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <utility>
 
 namespace sp {
 
@@ -313,6 +314,83 @@ __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *
       for (int i = 0; i < half; i++) gs_bf<MADD>(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
     }
   }
+}
+
+template <int... I, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F &&f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, n - 1>).
+template <int n, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(std::make_integer_sequence<int, n>{}, f);
+}
+
+// Inverse transform with lazy sums, for callers that multiply every output by
+// a Shoup constant next (any 32-bit input): inputs in [0, 2q), outputs < 8q.
+// Stage g (1..11) takes inputs below B = 2q, 4q or 8q (g mod 3 = 1, 2, 0):
+// v = x - y + B and the Shoup product (multiply-add form) is below 2q; the sum
+// x + y (below 2B) is reduced by a Barrett step only when g is a multiple of 3,
+// so two stages in three skip the reduction (2 of their 8 instructions).
+template <bool REDUCE>
+__device__ __forceinline__ void gs_bf_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t nq,
+                                           uint32_t mu1, uint32_t B) {
+  const uint32_t s = x + y;
+  const uint32_t v = x - y + B;
+  y = v * w + __umulhi(v, wp) * nq;
+  x = REDUCE ? s + __umulhi(s, mu1) * nq : s;
+}
+template <int G>
+__device__ __forceinline__ void gs_stage_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
+                                              uint32_t nq, uint32_t mu1) {
+  constexpr int PH = G % 3;  // 1: inputs < 2q, 2: < 4q, 0: < 8q (reduce)
+  gs_bf_lazy<PH == 0>(x, y, w, wp, nq, mu1, (PH == 1 ? 2u : PH == 2 ? 4u : 8u) * q);
+}
+__device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
+                                                  uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane) {
+  const int L = lane >> 1, b = lane & 1;
+  const uint32_t nq = neg_q(q);
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    gs_stage_lazy<1>(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q, nq, mu1);
+  uint32_t n[32];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t send = b ? r[2 * k] : r[2 * k + 1];
+    const uint32_t recv = swap_lane_pair(send);
+    n[k] = b ? recv : r[2 * k];
+    n[16 + k] = b ? r[2 * k + 1] : recv;
+  }
+  // Strides 2 .. 32 in registers (stages 2-6).
+  static_for<5>([&](auto ss) {
+    constexpr int s = decltype(ss)::value;
+    constexpr int half = 1 << s, nblk = 16 >> s;
+#pragma unroll
+    for (int blk = 0; blk < nblk; blk++) {
+      const uint32_t w = tw_b(tw, nblk - 1 + blk, 0, lane), wp = tw_b(tw, nblk - 1 + blk, 1, lane);
+#pragma unroll
+      for (int i = 0; i < half; i++)
+        gs_stage_lazy<2 + s>(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q, nq, mu1);
+    }
+  });
+#pragma unroll
+  for (int i = 0; i < 32; i++) lds[tile_at(L, 2 * i + b)] = n[i];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < 32; i++) r[i] = lds[tile_at(i, lane)];
+  wave_lds_sync();
+  // Phase A: strides 64 .. 1024 (stages 7-11).
+  static_for<5>([&](auto ss) {
+    constexpr int s = decltype(ss)::value;
+    constexpr int half = 1 << s, nblk = 16 >> s;
+#pragma unroll
+    for (int blk = 0; blk < nblk; blk++) {
+      const uint32_t w = tw_a(tw, nblk + blk, 0), wp = tw_a(tw, nblk + blk, 1);
+#pragma unroll
+      for (int i = 0; i < half; i++)
+        gs_stage_lazy<7 + s>(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, nq, mu1);
+    }
+  });
 }
 
 // ---- internal NTT-domain layout P --------------------------------------

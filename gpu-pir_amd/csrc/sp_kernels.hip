@@ -299,15 +299,6 @@ __global__ __launch_bounds__(256) void k_db_encode(const uint8_t *__restrict__ b
 //   db[t][m][j][g][k][P],   g < C / CG,  k < W.
 // At C >= 8 the multiply streams 7/8 of the bytes of the plain u32 form.
 // ---------------------------------------------------------------------------
-template <int... I, class F>
-__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F &&f) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-// f(integral_constant<int, 0>), ..., f(integral_constant<int, n - 1>).
-template <int n, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-  static_for_impl(std::make_integer_sequence<int, n>{}, f);
-}
 
 template <int c, int W>
 __device__ __forceinline__ uint32_t unpack28(const uint32_t (&w)[W]) {
@@ -1030,7 +1021,7 @@ __device__ __forceinline__ void fold_tail(uint32_t (*tiles)[LDS_TILE], uint32_t 
     const ModConst mc = m ? mc1 : mc0;
     load_p(s, tiles[wave], lane);
     wave_lds_sync();
-    ntt_inv_wave<true>(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+    ntt_inv_wave_lazy(s, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, mc.mu1, lane);
 #pragma unroll
     for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   }
@@ -1315,7 +1306,7 @@ void k_fold_fused(const uint32_t *__restrict__ src_acc, size_t src_acc_stride, u
   uint32_t s[32];
   load_p(s, tiles[wave], lane);
   wave_lds_sync();
-  ntt_inv_wave<true>(s, tw_base + (mm * 2 + 1) * TW_WORDS, tiles[wave], mcm.q, lane);
+  ntt_inv_wave_lazy(s, tw_base + (mm * 2 + 1) * TW_WORDS, tiles[wave], mcm.q, mcm.mu1, lane);
 #pragma unroll
   for (int ii = 0; ii < 32; ii++) s[ii] = canon(shoup_mul(s[ii], mcm.crt_c, mcm.crt_cp, mcm.q), mcm.q);
   __syncthreads();
@@ -1737,7 +1728,7 @@ __device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, u
     const ModConst mc = m ? mc1 : mc0;
     uint32_t rr[32];
     load_p(rr, vi + (size_t)m * N, lane);
-    ntt_inv_wave<true>(rr, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+    ntt_inv_wave_lazy(rr, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, mc.mu1, lane);
 #pragma unroll
     for (int ii = 0; ii < 32; ii++) xm[m * N + 64 * ii + lane] = canon(shoup_mul(rr[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   }
@@ -1947,7 +1938,7 @@ void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   }
   if (!step) return;  // uniform per block
   load_p(a, vi + (size_t)m * N, lane);  // (a child just formed above is this lane's own stores)
-  ntt_inv_wave<true>(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, lane);
+  ntt_inv_wave_lazy(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, mc.mu1, lane);
 #pragma unroll
   for (int ii = 0; ii < 32; ii++) a[ii] = canon(shoup_mul(a[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   // Wave m combines the coefficients 64 (16 m + jj) + lane: it hands the
