@@ -144,16 +144,12 @@ __device__ __forceinline__ uint32_t neg_q(uint32_t q) {
   return nq;
 }
 
-// Harvey inverse butterfly: inputs [0, 2q), outputs [0, 2q).  MADD computes
-// the Shoup product as v * w + floor(v * wp / 2^32) * (-q) (8 VALU
-// instructions instead of 9); it holds more registers, so only kernels with
-// room to spare use it.
-template <bool MADD = false>
+// Harvey inverse butterfly: inputs [0, 2q), outputs [0, 2q).
 __device__ __forceinline__ void gs_bf(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q) {
   const uint32_t tq = 2 * q;
   const uint32_t u = reduce_2q(x + y, tq);
   const uint32_t v = x - y + tq;
-  y = MADD ? v * w + __umulhi(v, wp) * neg_q(q) : shoup_mul(v, w, wp, q);
+  y = shoup_mul(v, w, wp, q);
   x = u;
 }
 
@@ -269,13 +265,12 @@ __device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *
 // Inverse negacyclic NTT (without the factor N^-1) of one polynomial.
 // In:  r[j] = NTT position 32*lane + j, values in [0, 2q).
 // Out: r[i] = N * coefficient (64*i + lane) mod q, values in [0, 2q).
-template <bool MADD = false>
 __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                              uint32_t *__restrict__ lds, uint32_t q, int lane) {
   const int L = lane >> 1, b = lane & 1;
 #pragma unroll
   for (int k = 0; k < 16; k++)
-    gs_bf<MADD>(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
+    gs_bf(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
   uint32_t n[32];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -293,7 +288,7 @@ __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *
     for (int blk = 0; blk < nblk; blk++) {
       const uint32_t w = tw_b(tw, nblk - 1 + blk, 0, lane), wp = tw_b(tw, nblk - 1 + blk, 1, lane);
 #pragma unroll
-      for (int i = 0; i < half; i++) gs_bf<MADD>(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q);
+      for (int i = 0; i < half; i++) gs_bf(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q);
     }
   }
 #pragma unroll
@@ -311,7 +306,7 @@ __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *
     for (int blk = 0; blk < nblk; blk++) {
       const uint32_t w = tw_a(tw, nblk + blk, 0), wp = tw_a(tw, nblk + blk, 1);
 #pragma unroll
-      for (int i = 0; i < half; i++) gs_bf<MADD>(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
+      for (int i = 0; i < half; i++) gs_bf(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q);
     }
   }
 }
