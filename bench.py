@@ -361,8 +361,8 @@ def run_serving(torch, mods, dbs, rates, seconds: float) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--nu1", type=int, default=9)
     ap.add_argument("--nu2", type=int, default=6)
     ap.add_argument("--index", type=int, default=12345)
