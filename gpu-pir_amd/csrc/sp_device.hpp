@@ -327,27 +327,35 @@ __device__ __forceinline__ void static_for(F &&f) {
 // v = x - y + B and the Shoup product (multiply-add form) is below 2q; the sum
 // x + y (below 2B) is reduced by a Barrett step only when g is a multiple of 3,
 // so two stages in three skip the reduction (2 of their 8 instructions).
-template <bool REDUCE>
-__device__ __forceinline__ void gs_bf_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t nq,
-                                           uint32_t mu1, uint32_t B) {
+// MAD = false: the products as plain multiplies and subtracts (no 64-bit
+// multiply-add destinations: for kernels whose register allocation they upset).
+template <bool REDUCE, bool MAD>
+__device__ __forceinline__ void gs_bf_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
+                                           uint32_t nq, uint32_t mu1, uint32_t B) {
   const uint32_t s = x + y;
   const uint32_t v = x - y + B;
-  y = v * w + __umulhi(v, wp) * nq;
-  x = REDUCE ? s + __umulhi(s, mu1) * nq : s;
+  if (MAD) {
+    y = v * w + __umulhi(v, wp) * nq;
+    x = REDUCE ? s + __umulhi(s, mu1) * nq : s;
+  } else {
+    y = shoup_mul(v, w, wp, q);
+    x = REDUCE ? s - __umulhi(s, mu1) * q : s;
+  }
 }
-template <int G>
+template <int G, bool MAD>
 __device__ __forceinline__ void gs_stage_lazy(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, uint32_t q,
                                               uint32_t nq, uint32_t mu1) {
   constexpr int PH = G % 3;  // 1: inputs < 2q, 2: < 4q, 0: < 8q (reduce)
-  gs_bf_lazy<PH == 0>(x, y, w, wp, nq, mu1, (PH == 1 ? 2u : PH == 2 ? 4u : 8u) * q);
+  gs_bf_lazy<PH == 0, MAD>(x, y, w, wp, q, nq, mu1, (PH == 1 ? 2u : PH == 2 ? 4u : 8u) * q);
 }
+template <bool MAD = true>
 __device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                                   uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane) {
   const int L = lane >> 1, b = lane & 1;
-  const uint32_t nq = neg_q(q);
+  const uint32_t nq = MAD ? neg_q(q) : 0u;
 #pragma unroll
   for (int k = 0; k < 16; k++)
-    gs_stage_lazy<1>(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q, nq, mu1);
+    gs_stage_lazy<1, MAD>(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q, nq, mu1);
   uint32_t n[32];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -365,7 +373,7 @@ __device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint3
       const uint32_t w = tw_b(tw, nblk - 1 + blk, 0, lane), wp = tw_b(tw, nblk - 1 + blk, 1, lane);
 #pragma unroll
       for (int i = 0; i < half; i++)
-        gs_stage_lazy<2 + s>(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q, nq, mu1);
+        gs_stage_lazy<2 + s, MAD>(n[blk * 2 * half + i], n[blk * 2 * half + i + half], w, wp, q, nq, mu1);
     }
   });
 #pragma unroll
@@ -383,7 +391,7 @@ __device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint3
       const uint32_t w = tw_a(tw, nblk + blk, 0), wp = tw_a(tw, nblk + blk, 1);
 #pragma unroll
       for (int i = 0; i < half; i++)
-        gs_stage_lazy<7 + s>(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, nq, mu1);
+        gs_stage_lazy<7 + s, MAD>(r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, nq, mu1);
     }
   });
 }

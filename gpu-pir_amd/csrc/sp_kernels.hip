@@ -78,7 +78,7 @@ __global__ __launch_bounds__(128) void k_from_ntt(const uint32_t *__restrict__ n
         r[4 * c] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
       }
     }
-    ntt_inv_wave(r, tw, lds, mc.q, lane);
+    ntt_inv_wave_lazy<false>(r, tw, lds, mc.q, mc.mu1, lane);
 #pragma unroll
     for (int i = 0; i < 32; i++) xm[m][64 * i + lane] = canon(shoup_mul(r[i], mc.crt_c, mc.crt_cp, mc.q), mc.q);
     __syncthreads();
