@@ -1511,7 +1511,7 @@ __global__ __launch_bounds__(512) void k_regev_to_gsw(const uint32_t *__restrict
     const ModConst mc = m ? mc1 : mc0;
     uint32_t r[32];
     load_p(r, ct + (size_t)(row * 2 + m) * N, lane);
-    ntt_inv_wave(r, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+    ntt_inv_wave_lazy(r, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, mc.mu1, lane);
 #pragma unroll
     for (int i = 0; i < 32; i++)
       xm[(row * 2 + m) * N + 64 * i + lane] = canon(shoup_mul(r[i], mc.crt_c, mc.crt_cp, mc.q), mc.q);
@@ -1663,7 +1663,7 @@ __global__ __launch_bounds__(512) void k_pack_response(const uint32_t *__restric
     uint32_t sv[32];
     load_p(sv, tiles[wave], lane);
     wave_lds_sync();
-    ntt_inv_wave(sv, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, lane);
+    ntt_inv_wave_lazy(sv, tw_base + (m * 2 + 1) * TW_WORDS, tiles[wave], mc.q, mc.mu1, lane);
 #pragma unroll
     for (int ii = 0; ii < 32; ii++) tiles[wave][64 * ii + lane] = canon(shoup_mul(sv[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
   }
