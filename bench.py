@@ -29,7 +29,7 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes per k_db_mul launch for one query, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy kernel's readings (FETCH x2.000, WRITE x1.000);
 # profiles/db_mul_hbm_traffic_nu9_6.md.  Measured shapes only.
-DB_MUL_TRAFFIC_BYTES = {(9, 6): (1852408 + 8192) * 1024}
+DB_MUL_TRAFFIC_BYTES = {(9, 6): (1852411 + 8192) * 1024}
 
 
 def db_mul_algorithmic_bytes(shape) -> int:
