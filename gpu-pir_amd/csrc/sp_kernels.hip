@@ -1789,8 +1789,12 @@ __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const u
     uint32_t *dst = vi + (size_t)(row * 2 + m) * N + p0;
 #pragma unroll
     for (int e = 0; e < PPT; e++) {
-      uint32_t x = dst[e] + redc64(acc[row][e], q, qinv);
+      // REDC without its own reduction: below 5q (sums below 64 q^2); with
+      // the old value (< q) and, on row 1, the gathered e term (< q): below 7q.
+      const uint32_t mq = (uint32_t)acc[row][e] * qinv;
+      uint32_t x = dst[e] + (uint32_t)((acc[row][e] + (uint64_t)mq * q) >> 32);
       if (row) x += ase[e];
+      x = min(x, x - 4 * q);
       x = min(x, x - 2 * q);
       dst[e] = canon(x, q);
     }
