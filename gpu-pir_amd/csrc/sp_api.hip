@@ -307,7 +307,7 @@ extern "C" int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *de
   HIP_TRY(db.alloc(db_trial_words(R, C) * 4));
   HIP_TRY(sel.alloc((size_t)R * 4 * N * 4));
   HIP_TRY(out.alloc((size_t)C * 4 * N * 4));
-  HIP_TRY(launch_db_from_ref(dev_db, stage.u32(), R, C, s));
+  HIP_TRY(launch_db_from_ref(dev_db, stage.u32(), R, C, ctx->tables, s));
   HIP_TRY(launch_db_pack(stage.u32(), db.u32(), 1, R, 0, R, C, s));
   HIP_TRY(launch_nat_to_p(dev_first_dim, sel.u32(), (size_t)R * 4, 1, 0, ctx->tables, s));
   HIP_TRY(launch_db_mul(db.u32(), sel.u32(), 4 * N, 0, out.u32(), 0, 1, 1, R, C, ctx->tables, s));
@@ -595,6 +595,18 @@ extern "C" int sp_db_destroy(sp_db *db) {
 
 extern "C" size_t sp_db_device_bytes(const sp_db *db) { return db ? db->bytes : 0; }
 
+extern "C" int sp_db_build_limbs(sp_db *db, size_t *limb_bytes) {
+  if (!db) return fail(SP_ERR_ARG, "sp_db_build_limbs: NULL");
+  if (limb_bytes) *limb_bytes = 0;
+  if (!mfma_shape_ok(db)) return SP_OK;
+  HIP_TRY(hipSetDevice(db->ctx->device));
+  int rc = db_limbs(db, db->ctx->stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(db->ctx->stream));
+  if (limb_bytes) *limb_bytes = db->limb_bytes;
+  return SP_OK;
+}
+
 extern "C" int sp_db_load_bytes(sp_db *db, const uint8_t *host_bytes) {
   if (!db || !host_bytes) return fail(SP_ERR_ARG, "sp_db_load_bytes: NULL");
   db->limbs_valid = false;
@@ -633,7 +645,7 @@ extern "C" int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt) {
   HIP_TRY(wide.alloc(trial_words * 4));
   for (int t = 0; t < db->n2; t++) {
     HIP_TRY(hipMemcpyAsync(stage.p, host_ntt + t * trial_words, trial_words * 4, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_db_from_ref(stage.u32(), wide.u32(), R, C, ctx->stream));
+    HIP_TRY(launch_db_from_ref(stage.u32(), wide.u32(), R, C, ctx->tables, ctx->stream));
     HIP_TRY(launch_db_pack(wide.u32(), db->d_db + t * db_trial_words(R, C), 1, R, 0, R, C, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
   }
@@ -643,6 +655,17 @@ extern "C" int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt) {
 // ---------------------------------------------------------------------------
 // batched answer pipeline (a single query is a batch of capacity 1)
 // ---------------------------------------------------------------------------
+// What a captured stage chain depends on besides the buffers (which never move
+// while the batch lives): the run size, conversion, packing, and the kernel
+// variant of every stage.  A change of any of them captures a new graph.
+struct GraphKey {
+  int n = 0, packed = 0, pack_q2 = 0, dbmul = 0, expand = 0, fold = 0;
+  bool operator==(const GraphKey &o) const {
+    return n == o.n && packed == o.packed && pack_q2 == o.pack_q2 && dbmul == o.dbmul && expand == o.expand &&
+           fold == o.fold;
+  }
+};
+
 struct sp_batch {
   sp_db *db = nullptr;
   int cap = 0, t_gsw = 0, t_exp = 0, g = 0, stop = 0;
@@ -659,9 +682,11 @@ struct sp_batch {
   size_t stage_words = 0;
   hipStream_t io = nullptr;       // uploads / readouts
   hipStream_t cap_stream = nullptr;
-  hipEvent_t done = nullptr;      // recorded after every run
-  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  hipEvent_t done = nullptr;      // recorded after every run (and asynchronous readout)
+  hipEvent_t loaded = nullptr;    // recorded on io after every load; runs wait for it on the device
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   std::vector<hipGraph_t> graph_defs;
+  int last_dbmul = 0;             // database-multiply variant of the last run (1 VALU, 2 MFMA)
   uint32_t *bulk = nullptr;       // bulk-load staging: [cap][4N] queries then [cap][f_q] v_folding
   uint32_t *dig = nullptr;        // split-expansion digit scratch [cap][dig_q]
   uint32_t *snap = nullptr;       // latency-form expansion snapshots, 2 sets x snap_q x SNAP_QUERIES
@@ -674,12 +699,13 @@ struct sp_batch {
   std::vector<int> packed;        // per slot: 1 = packed query (v_folding converted on the device)
   int *d_packed = nullptr;        // device copy of `packed`
   int pack_q2 = 0;                // response packing: 0 off, else bits of the rescaled row 0
+  int pack_q2_run = 0;            // pack_q2 of the last run (what the packed responses hold)
   uint32_t *pk = nullptr;         // packing keys per slot [cap][r][m][k][row][P], 2^32-scaled
   size_t pk_q = 0;
   std::vector<int> haspk;         // per slot: 1 = packing keys loaded
   int *d_haspk = nullptr;
-  uint8_t *pout = nullptr;        // packed responses [cap][pout_q] bytes
-  size_t pout_q = 0;
+  uint8_t *pout = nullptr;        // packed responses [cap][bytes(q2)]: sized once for q2 = 32, never moved
+  uint32_t *rstage = nullptr;     // asynchronous raw readout: responses gathered [cap][n2][4N]
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
@@ -714,7 +740,9 @@ static void batch_free(sp_batch *b) {
   if (b->snap) (void)hipFree(b->snap);
   if (b->pk) (void)hipFree(b->pk);
   if (b->pout) (void)hipFree(b->pout);
+  if (b->rstage) (void)hipFree(b->rstage);
   if (b->done) (void)hipEventDestroy(b->done);
+  if (b->loaded) (void)hipEventDestroy(b->loaded);
   if (b->io) (void)hipStreamDestroy(b->io);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   delete b;
@@ -778,6 +806,7 @@ extern "C" int sp_batch_create(sp_db *db, int capacity, int t_gsw, int t_exp, sp
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->io, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->loaded, hipEventDisableTiming);
   if (e != hipSuccess) {
     batch_free(b);
     return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_batch_create: %s", hipGetErrorString(e));
@@ -805,12 +834,19 @@ static int upload_p(sp_batch *b, const uint32_t *host, uint32_t *dst, size_t n_p
 }
 
 // Records the form of slots [first, first + n) (1 packed, 0 with fold vectors)
-// on the host and, for the conversion kernel, on the device (io stream).
+// on the host and, for the conversion kernel, on the device (io stream; the
+// values travel as kernel arguments, so nothing on the host must outlive the call).
 static int set_slot_form(sp_batch *b, int first, int n, int packed) {
   for (int i = 0; i < n; i++) b->packed[(size_t)first + i] = packed;
-  HIP_TRY(hipMemcpyAsync(b->d_packed + first, b->packed.data() + first, (size_t)n * sizeof(int),
-                         hipMemcpyHostToDevice, b->io));
-  HIP_TRY(hipStreamSynchronize(b->io));
+  HIP_TRY(launch_set_slot_flags(b->d_packed + first, n, b->packed.data() + first, b->io));
+  return SP_OK;
+}
+
+// Ends a load: later runs wait for the io stream's work on the device; a
+// blocking load also waits on the host, after which the host buffers may be reused.
+static int finish_load(sp_batch *b, bool async) {
+  HIP_TRY(hipEventRecord(b->loaded, b->io));
+  if (!async) HIP_TRY(hipStreamSynchronize(b->io));
   return SP_OK;
 }
 
@@ -842,6 +878,7 @@ static int batch_load(sp_batch *b, int slot, const uint32_t *host_query, const u
   }
   if ((rc = set_slot_form(b, slot, 1, 0))) return rc;
   if ((rc = load_slot_packing(b, slot, 1, nullptr))) return rc;
+  if ((rc = finish_load(b, false))) return rc;
   b->fold_mode = mode;
   return SP_OK;
 }
@@ -880,9 +917,7 @@ static int load_slot_packing(sp_batch *b, int first, int n, sp_keys *const *keys
       HIP_TRY(hipMemcpyAsync(b->pk + ((size_t)first + i) * b->pk_q, keys[i]->pk, b->pk_q * 4,
                              hipMemcpyDeviceToDevice, b->io));
   }
-  HIP_TRY(hipMemcpyAsync(b->d_haspk + first, b->haspk.data() + first, (size_t)n * sizeof(int),
-                         hipMemcpyHostToDevice, b->io));
-  HIP_TRY(hipStreamSynchronize(b->io));
+  HIP_TRY(launch_set_slot_flags(b->d_haspk + first, n, b->haspk.data() + first, b->io));
   return SP_OK;
 }
 
@@ -962,8 +997,8 @@ extern "C" int sp_keys_set_conversion(sp_keys *k, int t_conv, const uint32_t *ho
 // device staging area (asynchronous DMA when the host buffers are pinned),
 // one layout conversion per array covers all n queries, keys are copied
 // device to device.
-extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
-                                  sp_keys *const *keys, const uint32_t *const *host_v_folding) {
+static int batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *host_queries, sp_keys *const *keys,
+                           const uint32_t *const *host_v_folding, bool async) {
   if (!b || !host_queries || !keys || !host_v_folding) return fail(SP_ERR_ARG, "sp_batch_load_many: NULL");
   if (n < 1 || first < 0 || first + n > b->cap)
     return fail(SP_ERR_ARG, "sp_batch_load_many: slots [%d, %d) of %d", first, first + n, b->cap);
@@ -976,13 +1011,13 @@ extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t 
   }
   sp_ctx *ctx = b->db->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
-  HIP_TRY(hipEventSynchronize(b->done));  // the previous run must be done with the slots
   const size_t qw = (size_t)4 * N;
   if (!b->bulk) {
     hipError_t e = hipMalloc(&b->bulk, (size_t)b->cap * (qw + b->f_q) * 4);
     if (e != hipSuccess)
       return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "bulk staging: %s", hipGetErrorString(e));
   }
+  HIP_TRY(hipStreamWaitEvent(b->io, b->done, 0));  // the previous run must be done with the slots
   uint32_t *sq = b->bulk, *sf = b->bulk + (size_t)b->cap * qw;
   const size_t f0 = (size_t)first;
   for (int i = 0; i < n; i++) {
@@ -995,12 +1030,22 @@ extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t 
     HIP_TRY(hipMemcpyAsync(b->left + (f0 + i) * b->l_q, keys[i]->left, b->l_q * 4, hipMemcpyDeviceToDevice, b->io));
     HIP_TRY(hipMemcpyAsync(b->right + (f0 + i) * b->r_q, keys[i]->right, b->r_q * 4, hipMemcpyDeviceToDevice, b->io));
   }
-  HIP_TRY(hipStreamSynchronize(b->io));
   int rc = set_slot_form(b, first, n, 0);
   if (!rc) rc = load_slot_packing(b, first, n, keys);
+  if (!rc) rc = finish_load(b, async);
   if (rc) return rc;
   b->fold_mode = 1;
   return SP_OK;
+}
+
+extern "C" int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                                  sp_keys *const *keys, const uint32_t *const *host_v_folding) {
+  return batch_load_many(b, first, n, host_queries, keys, host_v_folding, false);
+}
+
+extern "C" int sp_batch_load_many_async(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                                        sp_keys *const *keys, const uint32_t *const *host_v_folding) {
+  return batch_load_many(b, first, n, host_queries, keys, host_v_folding, true);
 }
 
 extern "C" int sp_keys_set_packing(sp_keys *k, int t_conv, const uint32_t *host_w) {
@@ -1035,17 +1080,14 @@ extern "C" int sp_batch_set_packing(sp_batch *b, int q2_bits) {
   if (rc) return rc;
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   HIP_TRY(hipEventSynchronize(b->done));
-  const size_t bytes = packed_response_bytes(q2_bits);
-  if (bytes > b->pout_q) {
-    if (b->pout) (void)hipFree(b->pout);
-    b->pout = nullptr;
-    b->pout_q = 0;
-    hipError_t e = hipMalloc(&b->pout, (size_t)b->cap * bytes);
+  // Sized once for the largest response (q2_bits = 32) and never reallocated:
+  // captured graphs keep pointing at it whatever q2_bits later runs use.
+  if (!b->pout) {
+    hipError_t e = hipMalloc(&b->pout, (size_t)b->cap * packed_response_bytes(32));
     if (e != hipSuccess) {
       b->pout = nullptr;
       return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "sp_batch_set_packing: %s", hipGetErrorString(e));
     }
-    b->pout_q = bytes;
   }
   if (!b->pk) {
     hipError_t e = hipMalloc(&b->pk, (size_t)b->cap * b->pk_q * 4);
@@ -1058,24 +1100,46 @@ extern "C" int sp_batch_set_packing(sp_batch *b, int q2_bits) {
   return SP_OK;
 }
 
+// Checks for a readout of the packed responses of slots [0, n) of the last run.
+static int check_read_packed(const sp_batch *b, int n, const char *who) {
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "%s: n = %d", who, n);
+  if (!b->pack_q2_run) return fail(SP_ERR_STATE, "%s: the last run did not pack its responses", who);
+  if (b->pack_q2_run != b->pack_q2)  // callers size the readout by the current setting
+    return fail(SP_ERR_STATE, "%s: packing changed to q2_bits %d after a run at %d; run again", who, b->pack_q2,
+                b->pack_q2_run);
+  for (int i = 0; i < n; i++)
+    if (!b->haspk[i]) return fail(SP_ERR_STATE, "%s: slot %d has no packing keys", who, i);
+  return SP_OK;
+}
+
 extern "C" int sp_batch_read_packed(sp_batch *b, int n, uint8_t *host_out) {
   if (!b || !host_out) return fail(SP_ERR_ARG, "sp_batch_read_packed: NULL");
-  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_read_packed: n = %d", n);
-  if (!b->pack_q2) return fail(SP_ERR_STATE, "sp_batch_read_packed: packing is off");
-  for (int i = 0; i < n; i++)
-    if (!b->haspk[i]) return fail(SP_ERR_STATE, "sp_batch_read_packed: slot %d has no packing keys", i);
+  int rc = check_read_packed(b, n, "sp_batch_read_packed");
+  if (rc) return rc;
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   HIP_TRY(hipEventSynchronize(b->done));
-  const size_t bytes = packed_response_bytes(b->pack_q2);
-  for (int i = 0; i < n; i++)
-    HIP_TRY(hipMemcpyAsync(host_out + (size_t)i * bytes, b->pout + (size_t)i * b->pout_q, bytes,
-                           hipMemcpyDeviceToHost, b->io));
+  // packed responses of a run are contiguous, packed_response_bytes(q2 of that run) apart
+  HIP_TRY(hipMemcpyAsync(host_out, b->pout, (size_t)n * packed_response_bytes(b->pack_q2_run), hipMemcpyDeviceToHost,
+                         b->io));
   HIP_TRY(hipStreamSynchronize(b->io));
   return SP_OK;
 }
 
-extern "C" int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
-                                    sp_keys *const *keys) {
+extern "C" int sp_batch_read_packed_async(sp_batch *b, int n, uint8_t *host_out, void *stream) {
+  if (!b || !host_out) return fail(SP_ERR_ARG, "sp_batch_read_packed_async: NULL");
+  int rc = check_read_packed(b, n, "sp_batch_read_packed_async");
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(b->db->ctx->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamWaitEvent(s, b->done, 0));
+  HIP_TRY(hipMemcpyAsync(host_out, b->pout, (size_t)n * packed_response_bytes(b->pack_q2_run), hipMemcpyDeviceToHost,
+                         s));
+  HIP_TRY(hipEventRecord(b->done, s));
+  return SP_OK;
+}
+
+static int batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                             sp_keys *const *keys, bool async) {
   if (!b || !host_queries || !keys) return fail(SP_ERR_ARG, "sp_batch_load_packed: NULL");
   if (n < 1 || first < 0 || first + n > b->cap)
     return fail(SP_ERR_ARG, "sp_batch_load_packed: slots [%d, %d) of %d", first, first + n, b->cap);
@@ -1089,13 +1153,13 @@ extern "C" int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_
   }
   sp_ctx *ctx = b->db->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
-  HIP_TRY(hipEventSynchronize(b->done));  // the previous run must be done with the slots
   const size_t qw = (size_t)4 * N;
   hipError_t e = hipSuccess;
   if (!b->bulk) e = hipMalloc(&b->bulk, (size_t)b->cap * (qw + b->f_q) * 4);
   if (e == hipSuccess && !b->conv) e = hipMalloc(&b->conv, (size_t)b->cap * b->conv_q * 4);
   if (e != hipSuccess)
     return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "packed-load buffers: %s", hipGetErrorString(e));
+  HIP_TRY(hipStreamWaitEvent(b->io, b->done, 0));  // the previous run must be done with the slots
   const size_t f0 = (size_t)first;
   for (int i = 0; i < n; i++)
     HIP_TRY(hipMemcpyAsync(b->bulk + (size_t)i * qw, host_queries[i], qw * 4, hipMemcpyHostToDevice, b->io));
@@ -1106,12 +1170,22 @@ extern "C" int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_
     HIP_TRY(hipMemcpyAsync(b->conv + (f0 + i) * b->conv_q, keys[i]->conv, b->conv_q * 4, hipMemcpyDeviceToDevice,
                            b->io));
   }
-  HIP_TRY(hipStreamSynchronize(b->io));
   int rc = set_slot_form(b, first, n, 1);
   if (!rc) rc = load_slot_packing(b, first, n, keys);
+  if (!rc) rc = finish_load(b, async);
   if (rc) return rc;
   b->fold_mode = 1;
   return SP_OK;
+}
+
+extern "C" int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                                    sp_keys *const *keys) {
+  return batch_load_packed(b, first, n, host_queries, keys, false);
+}
+
+extern "C" int sp_batch_load_packed_async(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                                          sp_keys *const *keys) {
+  return batch_load_packed(b, first, n, host_queries, keys, true);
 }
 
 // Whether any of slots [0, n) holds a packed query (the run then converts).
@@ -1167,6 +1241,7 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
   }
   if (stages & 2) {
     const int var = dbmul_variant(b, n);
+    b->last_dbmul = var;
     if (var == 2) {
       int rc = db_limbs(db, s);
       if (rc) return rc;
@@ -1206,7 +1281,7 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
     }
     if (b->pack_q2)
       HIP_TRY(launch_pack_response(b->acc, b->acc_q, (size_t)C * 4 * N, n, b->pk, b->pk_q, b->d_haspk, b->pout,
-                                   b->pout_q, b->pack_q2, ctx->tables, s));
+                                   packed_response_bytes(b->pack_q2), b->pack_q2, ctx->tables, s));
   }
   return SP_OK;
 }
@@ -1217,8 +1292,10 @@ extern "C" int sp_batch_run(sp_batch *b, int n, int stages, void *stream) {
   if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_run: nothing loaded");
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamWaitEvent(s, b->loaded, 0));  // the slots' uploads, on the device
   int rc = batch_enqueue(b, n, stages, s);
   if (rc) return rc;
+  if (stages & 4) b->pack_q2_run = b->pack_q2;
   HIP_TRY(hipEventRecord(b->done, s));
   return SP_OK;
 }
@@ -1228,8 +1305,13 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   if (!b) return fail(SP_ERR_ARG, "sp_batch_graph_launch: NULL");
   if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_graph_launch: n = %d, capacity %d", n, b->cap);
   if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_graph_launch: nothing loaded");
-  const int packed = slots_packed(b, n);
-  const int gkey = n | (packed << 20) | (b->pack_q2 << 21);  // conversion and packing change the graph
+  GraphKey gkey;
+  gkey.n = n;
+  gkey.packed = slots_packed(b, n);
+  gkey.pack_q2 = b->pack_q2;
+  gkey.dbmul = dbmul_variant(b, n);
+  gkey.expand = b->db->ctx->variant[SP_STAGE_EXPAND];
+  gkey.fold = b->db->ctx->variant[SP_STAGE_FOLD];
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   hipGraphExec_t exec = nullptr;
   for (auto &ge : b->graphs)
@@ -1260,8 +1342,18 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
     b->graph_defs.push_back(graph);
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamWaitEvent(s, b->loaded, 0));  // the slots' uploads, on the device
   HIP_TRY(hipGraphLaunch(exec, s));
+  b->last_dbmul = gkey.dbmul;
+  b->pack_q2_run = gkey.pack_q2;
   HIP_TRY(hipEventRecord(b->done, s));
+  return SP_OK;
+}
+
+extern "C" int sp_batch_last_run(const sp_batch *b, int *dbmul_variant, int *graphs_cached) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_last_run: NULL");
+  if (dbmul_variant) *dbmul_variant = b->last_dbmul;
+  if (graphs_cached) *graphs_cached = (int)b->graphs.size();
   return SP_OK;
 }
 
@@ -1333,8 +1425,15 @@ extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
     if (key[i] > 0 && g->b[i]->fold_mode < 0) return fail(SP_ERR_STATE, "sp_group_launch: batch %d not loaded", i);
     active += key[i] > 0;
   }
-  for (int i = 0; i < count; i++)  // the graph also depends on conversion and packing per batch
-    key.push_back(key[i] > 0 ? slots_packed(g->b[i], key[i]) | (g->b[i]->pack_q2 << 1) : 0);
+  for (int i = 0; i < count; i++) {  // the graph also depends on conversion, packing and kernel variants
+    const sp_batch *b = g->b[i];
+    const bool on = key[i] > 0;
+    key.push_back(on ? slots_packed(b, key[i]) : 0);
+    key.push_back(on ? b->pack_q2 : 0);
+    key.push_back(on ? dbmul_variant(g->b[i], key[i]) : 0);
+    key.push_back(on ? b->db->ctx->variant[SP_STAGE_EXPAND] : 0);
+    key.push_back(on ? b->db->ctx->variant[SP_STAGE_FOLD] : 0);
+  }
   if (!active) return SP_OK;
   HIP_TRY(hipSetDevice(g->b[0]->db->ctx->device));
   hipGraphExec_t exec = nullptr;
@@ -1377,10 +1476,17 @@ extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
     g->graph_defs.push_back(graph);
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (int i = 0; i < count; i++)
+    if (key[i] > 0) HIP_TRY(hipStreamWaitEvent(s, g->b[i]->loaded, 0));  // uploads of every active batch
   HIP_TRY(hipGraphLaunch(exec, s));
   HIP_TRY(hipEventRecord(g->done, s));
   for (int i = 0; i < count; i++)
-    if (key[i] > 0) HIP_TRY(hipEventRecord(g->b[i]->done, s));
+    if (key[i] > 0) {
+      sp_batch *b = g->b[i];
+      b->last_dbmul = dbmul_variant(b, key[i]);
+      b->pack_q2_run = b->pack_q2;
+      HIP_TRY(hipEventRecord(b->done, s));
+    }
   return SP_OK;
 }
 
@@ -1500,6 +1606,35 @@ extern "C" int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out) {
     HIP_TRY(hipMemcpyAsync(host_out + r0 * 4 * N, b->stage, nr * 4 * N * 4, hipMemcpyDeviceToHost, b->io));
     HIP_TRY(hipStreamSynchronize(b->io));
   }
+  return SP_OK;
+}
+
+// Responses of slots [0, n) of the last run, copied to host_out on `stream`
+// after that run (no host wait): gathered on the device into a staging buffer
+// of the batch, then one contiguous copy.  sp_batch_poll / _synchronize cover
+// the copy; host_out (pinned for a true DMA) must stay valid until then.
+extern "C" int sp_batch_read_responses_async(sp_batch *b, int n, uint32_t *host_out, void *stream) {
+  if (!b || !host_out) return fail(SP_ERR_ARG, "sp_batch_read_responses_async: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_read_responses_async: n = %d", n);
+  sp_db *db = b->db;
+  HIP_TRY(hipSetDevice(db->ctx->device));
+  const size_t rows = (size_t)n * db->n2;
+  if (!b->rstage) {
+    hipError_t e = hipMalloc(&b->rstage, (size_t)b->cap * db->n2 * 4 * N * 4);
+    if (e != hipSuccess) {
+      b->rstage = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SP_ERR_NOMEM : SP_ERR_HIP, "response staging: %s", hipGetErrorString(e));
+    }
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamWaitEvent(s, b->done, 0));
+  const int C = 1 << db->nu2;
+  for (size_t r0 = 0; r0 < rows; r0 += 65535) {
+    const size_t nr = std::min<size_t>(65535, rows - r0);
+    HIP_TRY(launch_gather_rows(b->acc + r0 * C * 4 * N, (size_t)C * 4 * N, (int)nr, b->rstage + r0 * 4 * N, s));
+  }
+  HIP_TRY(hipMemcpyAsync(host_out, b->rstage, rows * 4 * N * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipEventRecord(b->done, s));
   return SP_OK;
 }
 

@@ -206,19 +206,20 @@ __global__ __launch_bounds__(256) void k_extprod(const uint32_t *__restrict__ in
 // ---------------------------------------------------------------------------
 // Layout conversion natural <-> P (optionally scaling residues by 2^32)
 // ---------------------------------------------------------------------------
-// Polys are indexed k = 0..n-1; poly k is residue (k / m_div) & 1.
+// Polys are indexed k = 0..n-1; poly k is residue (k / m_div) & 1.  Any
+// 32-bit input word is accepted: the output is canonical, x mod q (mont = 0)
+// or x 2^32 mod q (mont = 1), so a non-canonical residue from a caller cannot
+// break the lazy-reduction bounds of the kernels that read the result.
 __global__ __launch_bounds__(256) void k_nat_to_p(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                                   size_t n, int m_div, int mont, ModConst mc0, ModConst mc1) {
   const size_t total = n * N;
   for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
     const size_t k = g / N;
     const int p = (int)(g % N);
-    uint32_t x = in[k * N + z_of_p(p)];
-    if (mont) {
-      const ModConst &mc = ((k / m_div) & 1) ? mc1 : mc0;
-      x = canon(shoup_mul(x, mc.two32, mc.two32_p, mc.q), mc.q);
-    }
-    out[g] = x;
+    const uint32_t x = in[k * N + z_of_p(p)];
+    const ModConst &mc = ((k / m_div) & 1) ? mc1 : mc0;
+    // Shoup multiplication by 2^32 mod q, or by 1 (companion floor(2^32 / q)): [0, 2q), then canonical
+    out[g] = mont ? canon(shoup_mul(x, mc.two32, mc.two32_p, mc.q), mc.q) : canon(shoup_mul(x, 1u, mc.mu1, mc.q), mc.q);
   }
 }
 
@@ -245,9 +246,11 @@ __global__ __launch_bounds__(256) void k_p_to_nat_unscale(const uint32_t *__rest
   }
 }
 
-// Reference Database layout [c][j][m][z] -> internal [m][j][c][p] (one trial).
+// Reference Database layout [c][j][m][z] -> internal [m][j][c][p] (one trial),
+// residues reduced mod q_m: the packed resident form holds 28 bits per
+// residue, so a word >= 2^28 from a caller must not reach k_db_pack.
 __global__ __launch_bounds__(256) void k_db_from_ref(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                     int R, int C) {
+                                                     int R, int C, ModConst mc0, ModConst mc1) {
   const size_t total = (size_t)2 * R * C * N;
   for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (size_t)gridDim.x * 256) {
     const int p = (int)(g % N);
@@ -255,7 +258,8 @@ __global__ __launch_bounds__(256) void k_db_from_ref(const uint32_t *__restrict_
     const int c = (int)(rest % C); rest /= C;
     const int j = (int)(rest % R); rest /= R;
     const int m = (int)rest;
-    out[g] = in[(((size_t)c * R + j) * 2 + m) * N + z_of_p(p)];
+    const ModConst &mc = m ? mc1 : mc0;
+    out[g] = canon(shoup_mul(in[(((size_t)c * R + j) * 2 + m) * N + z_of_p(p)], 1u, mc.mu1, mc.q), mc.q);
   }
 }
 
@@ -791,8 +795,8 @@ hipError_t launch_p_to_nat_unscale(const uint32_t *in, uint32_t *out, size_t n, 
   return hipGetLastError();
 }
 
-hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s) {
-  k_db_from_ref<<<grid_for((size_t)2 * R * C * N, 256), 256, 0, s>>>(in, out, R, C);
+hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, const DevTables &t, hipStream_t s) {
+  k_db_from_ref<<<grid_for((size_t)2 * R * C * N, 256), 256, 0, s>>>(in, out, R, C, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 
@@ -877,6 +881,29 @@ hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint
 }
 
 // One ciphertext (4N words) per blockIdx.y, gathered into consecutive rows.
+// Per-slot flags written by a kernel whose arguments carry the values, so that
+// an asynchronous load needs no host buffer that outlives the call.
+struct SlotFlags {
+  int v[SLOT_FLAGS_PER_LAUNCH];
+};
+
+__global__ __launch_bounds__(64) void k_set_slot_flags(int *__restrict__ dst, int n, SlotFlags f) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = f.v[i];
+}
+
+hipError_t launch_set_slot_flags(int *dst, int n, const int *vals, hipStream_t s) {
+  for (int i0 = 0; i0 < n; i0 += SLOT_FLAGS_PER_LAUNCH) {
+    const int k = n - i0 < SLOT_FLAGS_PER_LAUNCH ? n - i0 : SLOT_FLAGS_PER_LAUNCH;
+    SlotFlags f{};
+    for (int i = 0; i < k; i++) f.v[i] = vals[i0 + i];
+    k_set_slot_flags<<<1, 64, 0, s>>>(dst + i0, k, f);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 __global__ __launch_bounds__(256) void k_gather_rows(const uint32_t *__restrict__ src, size_t src_stride,
                                                      uint32_t *__restrict__ dst) {
   const uint4 *from = reinterpret_cast<const uint4 *>(src + (size_t)blockIdx.y * src_stride);
@@ -1753,8 +1780,11 @@ __device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, u
 // NTT domain is a permutation of positions; it is gathered before `bar`
 // (a block barrier), after which the residue is written.
 // The 8 digit tiles hold the forward transforms' lazy outputs (< 8q, not
-// reduced): with keys < q each sum of 8 products is below 64 q^2 < 2^62 and
-// its REDC below 5q, which redc64 brings to [0, q).
+// reduced): with keys < q each sum of 8 products is below 64 q^2 < 2^62, and
+// its REDC, taken without a reduction of its own, is below 5q.  The old value
+// of v_i (< q) and, on row 1, the gathered e term (< q) are added to it (< 7q
+// < 8q), and one reduction (conditional subtractions of 4q, 2q, q) brings the
+// sum to [0, q).  Precondition: v_i, and so the gathered e term, canonical.
 template <int PPT>
 __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
                                               uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,

@@ -37,8 +37,9 @@ inline size_t db_trial_words(int R, int C) {
   return (size_t)2 * R * (C / db_group_cols(C)) * db_group_words(C) * N;
 }
 
-// Reference Database layout [c][j][m][z] (one trial) -> u32 staging [m][j][c][P].
-hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, hipStream_t s);
+// Reference Database layout [c][j][m][z] (one trial) -> u32 staging [m][j][c][P],
+// every word reduced mod q_m (any 32-bit input is accepted).
+hipError_t launch_db_from_ref(const uint32_t *in, uint32_t *out, int R, int C, const DevTables &t, hipStream_t s);
 // Plaintext records rec0 .. rec0 + nrec - 1 (rec0 and nrec multiples of C) ->
 // u32 staging [t][m][nrec / C][C][P].
 hipError_t launch_db_encode(const uint8_t *bytes, size_t rec0, size_t nrec, uint32_t *db, int n2, int C,
@@ -59,6 +60,9 @@ hipError_t launch_expand_action(uint32_t *v, size_t q_stride, int n_queries, int
                                 size_t right_qstride, int t_exp, int bits, const DevTables &t, hipStream_t s);
 hipError_t launch_copy_queries(const uint32_t *query, size_t query_qstride, uint32_t *v, size_t v_qstride, int nq,
                                hipStream_t s);
+// dst[i] = vals[i] for i < n, on the device (vals is read on the host during the call).
+constexpr int SLOT_FLAGS_PER_LAUNCH = 64;
+hipError_t launch_set_slot_flags(int *dst, int n, const int *vals, hipStream_t s);
 // rows of 4N words at src + r * src_stride -> contiguous dst (response gather for readout).
 hipError_t launch_gather_rows(const uint32_t *src, size_t src_stride, int rows, uint32_t *dst, hipStream_t s);
 

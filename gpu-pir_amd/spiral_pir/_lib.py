@@ -57,6 +57,7 @@ SIGNATURES = {
     "sp_db_load_bytes": [P, P],
     "sp_db_load_ntt": [P, P],
     "sp_db_destroy": [P],
+    "sp_db_build_limbs": [P, PSZ],
     "sp_query_create": [P, I, I, PP],
     "sp_query_load": [P, P, P, P, P, P],
     "sp_query_load_packed": [P, P, P],
@@ -84,6 +85,11 @@ SIGNATURES = {
     "sp_batch_read_packed": [P, I, P],
     "sp_batch_load_many": [P, I, I, P, P, P],
     "sp_batch_load_packed": [P, I, I, P, P],
+    "sp_batch_load_many_async": [P, I, I, P, P, P],
+    "sp_batch_load_packed_async": [P, I, I, P, P],
+    "sp_batch_read_responses_async": [P, I, P, P],
+    "sp_batch_read_packed_async": [P, I, P, P],
+    "sp_batch_last_run": [P, PI, PI],
     "sp_group_create": [P, I, PP],
     "sp_group_launch": [P, PI, P],
     "sp_group_synchronize": [P],

@@ -42,6 +42,15 @@ def flatten(cts: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(cts[:, 0])
 
 
+def ciphertexts_to_key_layout(cts: np.ndarray) -> np.ndarray:
+    """[level][col] Ciphertexts (as group_to_ciphertext gives for a key file) ->
+    CiphertextHalf[level][crt][col][row][N], the v_folding / expansion-key layout:
+    gpu-pir/fold_test.cu:56-66 copies ciphertext j of level i row by row into
+    v_folding[i][crt][j]."""
+    c = np.asarray(cts).reshape(cts.shape[0], cts.shape[1], 2, 2, POLY_LEN)
+    return np.ascontiguousarray(c.transpose(0, 3, 1, 2, 4))
+
+
 def matrices_to_key_layout(polys: np.ndarray, cols: int) -> np.ndarray:
     """Row-major 2 x cols NTT matrices (one per level) -> CiphertextHalf[level][crt][col][row][N]
     (the v_folding / expansion-key layout, structures.cuh:173-183)."""

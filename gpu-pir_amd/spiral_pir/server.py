@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .params import T_CONV, Shape
+from .params import MODULI, T_CONV, Shape
 
 STAGE_EXPAND = 1
 STAGE_DB_MUL = 2
@@ -36,6 +36,27 @@ def _ptr(a) -> ctypes.c_void_p:
         return ctypes.c_void_p(0)
     assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.uint32, np.uint8)
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def _sized(a, words: int, what: str) -> np.ndarray:
+    """a as a contiguous u32 array of exactly `words` words (the byte count the
+    library copies for it), else ValueError: a short array would make the
+    library read past its end."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    if a.size != words:
+        raise ValueError(f"{what}: {a.size} words, expected {words}")
+    return a
+
+
+def check_residues(ntt: np.ndarray, what: str, chunk: int = 1 << 24) -> None:
+    """Raises ValueError unless every residue of an NTT-form array [..][2][2048]
+    (residue m of q_m) is canonical; checked chunk by chunk (memory maps stay lazy)."""
+    v = ntt.reshape(-1, 2, 2048)
+    step = max(1, chunk // (2 * 2048))
+    for k in range(0, v.shape[0], step):
+        blk = v[k:k + step]
+        if not ((blk[:, 0] < MODULI[0]).all() and (blk[:, 1] < MODULI[1]).all()):
+            raise ValueError(f"{what}: residues out of range in polynomials {k}..{k + blk.shape[0] - 1}")
 
 
 class Context:
@@ -92,10 +113,13 @@ class Database:
         assert r.size == self.shape.records * self.shape.record_bytes
         _lib.call("sp_db_load_bytes", self._h, _ptr(r))
 
-    def load_ntt(self, db_ntt: np.ndarray) -> None:
-        """Reference Database layout per sub-database: [n2][2^nu2][2^nu1][2][N] u32."""
-        d = np.ascontiguousarray(db_ntt, dtype=np.uint32)
-        assert d.size == self.shape.records * self.shape.n2 * 2 * 2048
+    def load_ntt(self, db_ntt: np.ndarray, check: bool = True) -> None:
+        """Reference Database layout per sub-database: [n2][2^nu2][2^nu1][2][N] u32,
+        canonical residues (checked unless check=False; the library reduces any
+        word mod q_m in any case)."""
+        d = _sized(db_ntt, self.shape.records * self.shape.n2 * 2 * 2048, "load_ntt")
+        if check:
+            check_residues(d, "load_ntt")
         _lib.call("sp_db_load_ntt", self._h, _ptr(d))
 
     def load_file(self, path: str) -> None:
@@ -111,11 +135,25 @@ class Database:
         if kind == dbfile.KIND_RECORDS:
             self.load_bytes(payload)
         else:
-            self.load_ntt(payload)
+            check_residues(payload, path)
+            self.load_ntt(payload, check=False)
 
     @property
     def device_bytes(self) -> int:
         return int(_lib.load().sp_db_device_bytes(self._h))
+
+    @property
+    def limb_bytes(self) -> int:
+        """HBM the int8-limb copy for the MFMA multiply takes once a batch uses it
+        (4 limb bytes per residue; only shapes with 2^nu1, 2^nu2 >= 32 have one)."""
+        s = self.shape
+        return s.db_ntt_bytes if (s.nu1 >= 5 and s.nu2 >= 5) else 0
+
+    def build_limbs(self) -> int:
+        """Build the int8-limb copy now (else on first MFMA use); returns its bytes."""
+        n = ctypes.c_size_t()
+        _lib.call("sp_db_build_limbs", self._h, ctypes.byref(n))
+        return n.value
 
     def close(self) -> None:
         if self._h:
@@ -135,14 +173,15 @@ class Query:
 
     def load(self, query: np.ndarray, left: np.ndarray, right: np.ndarray, v_folding: np.ndarray,
              v_folding_neg: np.ndarray | None = None) -> None:
-        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (query, left, right, v_folding)]
-        neg = None if v_folding_neg is None else np.ascontiguousarray(v_folding_neg, dtype=np.uint32)
+        arrs = _query_arrays(self.db.shape, "Query.load", query, left, right, v_folding)
+        neg = None if v_folding_neg is None else _sized(v_folding_neg, self.db.shape.fold_words,
+                                                       "Query.load: v_folding_neg")
         _lib.call("sp_query_load", self._h, *[_ptr(a) for a in arrs], _ptr(neg))
 
     def load_packed(self, query: np.ndarray, keys: "Keys") -> None:
         """A packed query (client.query_packed) with resident keys that carry a
         conversion key: the fold vectors are converted on the device."""
-        q = np.ascontiguousarray(query, dtype=np.uint32)
+        q = _sized(query, self.db.shape.query_words, "Query.load_packed: query")
         _lib.call("sp_query_load_packed", self._h, _ptr(q), keys.handle)
 
     def set_packing(self, q2_bits: int) -> None:
@@ -197,9 +236,10 @@ class Batch:
         h = ctypes.c_void_p()
         _lib.call("sp_batch_create", db.handle, capacity, s.t_gsw, s.t_exp, ctypes.byref(h))
         self._h = h
+        self._hold: list = []  # host arrays an asynchronous load or readout still reads or writes
 
     def load(self, slot: int, query, left, right, v_folding) -> None:
-        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (query, left, right, v_folding)]
+        arrs = _query_arrays(self.db.shape, "Batch.load", query, left, right, v_folding)
         _lib.call("sp_batch_load", self._h, slot, *[_ptr(a) for a in arrs])
 
     def run(self, n: int, stages: int = STAGES_ALL, stream=None) -> None:
@@ -210,12 +250,23 @@ class Batch:
 
     def synchronize(self) -> None:
         _lib.call("sp_batch_synchronize", self._h)
+        self._hold = []
 
     def poll(self) -> bool:
-        """True when the last run has finished (does not wait)."""
+        """True when the last run (and any asynchronous readout queued after it)
+        has finished (does not wait)."""
         d = ctypes.c_int()
         _lib.call("sp_batch_poll", self._h, ctypes.byref(d))
+        if d.value:
+            self._hold = []
         return bool(d.value)
+
+    def last_run(self) -> dict:
+        """Database-multiply kernel of the last run ("valu" / "mfma" / None) and
+        the number of captured stage-chain graphs."""
+        v, g = ctypes.c_int(), ctypes.c_int()
+        _lib.call("sp_batch_last_run", self._h, ctypes.byref(v), ctypes.byref(g))
+        return {"dbmul": {0: None, 1: "valu", 2: "mfma"}[v.value], "graphs": g.value}
 
     def tune(self, n: int) -> dict:
         """Autotune the database multiply for n queries (after one expansion has run)."""
@@ -224,40 +275,70 @@ class Batch:
         return {"valu_ms": a.value, "mfma_ms": b.value if b.value >= 0 else None,
                 "chosen": {1: "valu", 2: "mfma"}[c.value]}
 
-    def load_many(self, first: int, queries, keys: "list[Keys]", v_folding) -> None:
+    def load_many(self, first: int, queries, keys: "list[Keys]", v_folding, asynchronous: bool = False) -> None:
         """Slots [first, first + len(keys)): queries[i] ([2][2][2048] u32), resident keys[i],
         v_folding[i] (u32, fold_words).  Arrays in pinned memory make the transfers
-        asynchronous DMA; they are not retained after the call."""
+        asynchronous DMA.  Blocking by default; with asynchronous=True the uploads
+        are queued behind the batch's previous run and the call returns at once
+        (the arrays are kept until the next run has finished)."""
         n = len(keys)
         s = self.db.shape
-        qs = [np.ascontiguousarray(q, dtype=np.uint32) for q in queries]
-        vfs = [np.ascontiguousarray(v, dtype=np.uint32) for v in v_folding]
+        qs = [_sized(q, s.query_words, "load_many: query") for q in queries]
+        vfs = [_sized(v, s.fold_words, "load_many: v_folding") for v in v_folding]
         if len(qs) != n or len(vfs) != n:
             raise ValueError("load_many: one query, key set and v_folding per slot")
-        for q, v in zip(qs, vfs):
-            if q.size != s.query_words or v.size != s.fold_words:
-                raise ValueError("load_many: query or v_folding of the wrong size")
+        self._check_keys(keys, "load_many")
         P = ctypes.c_void_p
         qa = (P * n)(*[q.ctypes.data for q in qs])
         va = (P * n)(*[v.ctypes.data for v in vfs])
         ka = (P * n)(*[k.handle.value for k in keys])
-        _lib.call("sp_batch_load_many", self._h, first, n, qa, ka, va)
+        if asynchronous:
+            self._hold += qs + vfs
+        _lib.call("sp_batch_load_many_async" if asynchronous else "sp_batch_load_many", self._h, first, n, qa, ka, va)
 
-    def load_packed(self, first: int, queries, keys: "list[Keys]") -> None:
+    def load_packed(self, first: int, queries, keys: "list[Keys]", asynchronous: bool = False) -> None:
         """Slots [first, first + len(keys)): packed queries (client.query_packed) with
         resident keys that carry a conversion key; each slot's fold vectors are
-        converted from its expanded query on the device."""
+        converted from its expanded query on the device.  asynchronous as for
+        load_many."""
         n = len(keys)
         s = self.db.shape
-        qs = [np.ascontiguousarray(q, dtype=np.uint32) for q in queries]
+        qs = [_sized(q, s.query_words, "load_packed: query") for q in queries]
         if len(qs) != n:
             raise ValueError("load_packed: one query and key set per slot")
-        if any(q.size != s.query_words for q in qs):
-            raise ValueError("load_packed: query of the wrong size")
+        self._check_keys(keys, "load_packed")
         P = ctypes.c_void_p
         qa = (P * n)(*[q.ctypes.data for q in qs])
         ka = (P * n)(*[k.handle.value for k in keys])
-        _lib.call("sp_batch_load_packed", self._h, first, n, qa, ka)
+        if asynchronous:
+            self._hold += qs
+        _lib.call("sp_batch_load_packed_async" if asynchronous else "sp_batch_load_packed", self._h, first, n, qa, ka)
+
+    def _check_keys(self, keys, who: str) -> None:
+        for k in keys:
+            if not isinstance(k, Keys) or not k.handle:
+                raise ValueError(f"{who}: keys must be open Keys objects")
+
+    def read_responses_async(self, n: int, out: np.ndarray, stream=None) -> None:
+        """Queue the copy of the responses of slots [0, n) of the last run into `out`
+        (u32, n x n2 x 2 x 2 x 2048 words, pinned for DMA) on `stream`; complete when
+        poll() returns True."""
+        s = self.db.shape
+        o = out.reshape(-1)
+        if o.dtype != np.uint32 or not o.flags["C_CONTIGUOUS"] or o.size < n * s.response_words:
+            raise ValueError("read_responses_async: out must be contiguous u32 of n x response_words")
+        self._hold.append(out)
+        _lib.call("sp_batch_read_responses_async", self._h, n, _ptr(o), ctypes.c_void_p(stream or 0))
+
+    def read_packed_async(self, n: int, out: np.ndarray, stream=None) -> None:
+        """Queue the copy of the compact responses of slots [0, n) of the last run
+        into `out` (u8, n x packed_response_bytes, pinned for DMA) on `stream`."""
+        o = out.reshape(-1)
+        need = n * self.db.shape.packed_response_bytes(self.q2_bits)
+        if o.dtype != np.uint8 or not o.flags["C_CONTIGUOUS"] or o.size < need:
+            raise ValueError("read_packed_async: out must be contiguous u8 of n x packed_response_bytes")
+        self._hold.append(out)
+        _lib.call("sp_batch_read_packed_async", self._h, n, _ptr(o), ctypes.c_void_p(stream or 0))
 
     def set_packing(self, q2_bits: int) -> None:
         """Response packing + modulus switching for later runs (0: off)."""
@@ -304,8 +385,8 @@ class Keys:
         s = db.shape
         self.db = db
         h = ctypes.c_void_p()
-        lft = np.ascontiguousarray(left, dtype=np.uint32)
-        rgt = np.ascontiguousarray(right, dtype=np.uint32)
+        lft = _sized(left, s.left_words, "Keys: left")
+        rgt = _sized(right, s.right_words, "Keys: right")
         _lib.call("sp_keys_create", db.handle, s.t_gsw, s.t_exp, _ptr(lft), _ptr(rgt), ctypes.byref(h))
         self._h = h
         self.packed = conv is not None
@@ -360,6 +441,12 @@ def answer(db: Database, query, left, right, v_folding) -> np.ndarray:
     """One full answer (expansion, database multiply, fold) -> n2 raw ciphertexts."""
     s = db.shape
     out = np.zeros((s.n2, 2, 2, 2048), np.uint32)
-    arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (query, left, right, v_folding)]
+    arrs = _query_arrays(s, "answer", query, left, right, v_folding)
     _lib.call("sp_answer", db.handle, s.t_gsw, s.t_exp, *[_ptr(a) for a in arrs], _ptr(out))
     return out
+
+
+def _query_arrays(s: Shape, who: str, query, left, right, v_folding) -> list:
+    """The four host arrays of a query with host keys, size-checked against the shape."""
+    return [_sized(query, s.query_words, f"{who}: query"), _sized(left, s.left_words, f"{who}: left"),
+            _sized(right, s.right_words, f"{who}: right"), _sized(v_folding, s.fold_words, f"{who}: v_folding")]

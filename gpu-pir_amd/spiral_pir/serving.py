@@ -5,20 +5,23 @@ whose fold vectors the device converts from the query itself, from a client
 whose keys are resident on the device) arrive at any time.  A scheduler
 thread keeps, per database ("lane"), a few batch buffers of `capacity` slots,
 each with its own stream.  Whenever a buffer is free and requests are waiting,
-it takes up to `capacity` of them (of one form: packed or with fold vectors),
-loads them with one bulk call, and replays
-the captured stage chain of the smallest size class that holds them.  Requests
-complete when their batch finishes.  Buffers of one lane and of different lanes
-run on the device at the same time, so queries that arrive while others are
-being answered are admitted as soon as a buffer frees up, and their work
-overlaps on the GPU.
+it takes up to `capacity` of them (of one form: packed or with fold vectors)
+and queues, without waiting on the host, their upload (one bulk call on the
+buffer's io stream), the replay of the captured stage chain of the smallest
+size class that holds them (which waits for the upload on the device), and
+the copy of the responses into the buffer's pinned host array.  Requests
+complete when the buffer's event reports that copy done.  Buffers of one lane
+and of different lanes run on the device at the same time, so queries that
+arrive while others are being answered are admitted as soon as a buffer frees
+up, and their transfers and kernels overlap on the GPU.
 
 With `pack_q2_bits` set, every response is packed and modulus-switched on the
 device (Spiral's compact response) and requests complete with its bytes.
 
 Admission is by free device memory: the slot capacity of each lane is sized at
-start-up from the free HBM (hipMemGetInfo) and the device bytes one slot needs;
-requests beyond the free slots wait in the lane's queue.
+start-up from the free HBM (hipMemGetInfo, after the int8-limb copies of the
+databases that the MFMA multiply reads are built) and the device bytes one
+slot needs; requests beyond the free slots wait in the lane's queue.
 
 The reference has no serving layer (it answers one query from test files);
 this module is the host-side loop over the library's batch, keys and graph
@@ -40,15 +43,16 @@ SIZE_CLASSES = (1, 2, 4, 8, 16, 32, 64, 128)
 
 def slot_device_bytes(shape: Shape) -> int:
     """Device bytes one batch slot holds: query, expanded v, copy of the client's
-    keys, fold vectors, accumulators, staging, and its share of the int8-limb
-    multiply workspaces (selector limbs and products of its two rows)."""
+    keys, fold vectors, accumulators, upload and response staging, and its share
+    of the int8-limb multiply workspaces (selector limbs and products of its two
+    rows).  The database's own limb copy is counted once per database, not here."""
     n = POLY_LEN
     v = (1 << shape.g) * 4 * n * 4
     keys = (shape.left_words + shape.right_words) * 4
     fold = shape.fold_words * 4
     acc = shape.n2 * shape.cols * 4 * n * 4
     query = 4 * n * 4
-    staging = (4 * n + shape.fold_words) * 4
+    staging = (4 * n + shape.fold_words) * 4 + shape.response_words * 4
     mfma = 2 * n * shape.rows * 4 * 2 + 2 * n * shape.n2 * 2 * shape.cols * 4
     return v + keys + fold + acc + query + staging + mfma
 
@@ -61,10 +65,11 @@ def size_class(n: int, capacity: int) -> int:
 
 
 class Request:
-    """One query in flight; `wait()` returns its n2 response ciphertexts."""
+    """One query in flight; `wait()` returns its n2 response ciphertexts (or, on a
+    server that packs responses, the compact response bytes)."""
 
     __slots__ = ("lane", "client", "query", "v_folding", "tag", "t_submit", "t_launch", "t_done", "response",
-                 "error", "_ev")
+                 "error", "batch_size", "_ev")
 
     def __init__(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray, tag=None):
         self.lane = lane
@@ -77,7 +82,13 @@ class Request:
         self.t_done = None
         self.response = None
         self.error = None
+        self.batch_size = 0
         self._ev = threading.Event()
+
+    def fail(self, e: BaseException) -> None:
+        if not self._ev.is_set():
+            self.error = e
+            self._ev.set()
 
     def wait(self, timeout: float | None = None) -> np.ndarray:
         if not self._ev.wait(timeout):
@@ -94,6 +105,17 @@ class Request:
     def latency(self) -> float:
         return self.t_done - self.t_submit
 
+    @property
+    def queue_time(self) -> float:
+        """Submission to launch: waiting for a free batch buffer."""
+        return self.t_launch - self.t_submit
+
+    @property
+    def device_time(self) -> float:
+        """Launch to completion seen by the scheduler: upload, the stage chain, the
+        response copy (and the batches ahead of it on the device)."""
+        return self.t_done - self.t_launch
+
 
 class _Lane:
     def __init__(self, name: str, db: Database, capacity: int, buffers: int, pack_q2_bits: int = 0):
@@ -106,6 +128,14 @@ class _Lane:
         if pack_q2_bits:
             for b in self.batches:
                 b.set_packing(pack_q2_bits)
+        # pinned host arrays the responses of each buffer are copied into
+        if pack_q2_bits:
+            nb = db.shape.packed_response_bytes(pack_q2_bits)
+            self.out = [pinned_u8(capacity * nb).reshape(capacity, nb) for _ in range(buffers)]
+        else:
+            rw = db.shape.response_words
+            self.out = [pinned_u32(capacity * rw).reshape(capacity, db.shape.n2, 2, 2, POLY_LEN)
+                        for _ in range(buffers)]
         self.streams = [torch.cuda.Stream() for _ in range(buffers)]
         self.free = collections.deque(range(buffers))
         self.inflight: dict[int, list[Request]] = {}
@@ -134,6 +164,9 @@ class Server:
                  poll_interval: float = 1e-4, pack_q2_bits: int = 0):
         import torch
 
+        for db in dbs.values():  # the MFMA multiply's limb copies count against the free HBM
+            db.build_limbs()
+        torch.cuda.synchronize()
         free, total = torch.cuda.mem_get_info()
         weights = weights or {name: 1.0 for name in dbs}
         wsum = sum(weights[name] for name in dbs)
@@ -189,6 +222,10 @@ class Server:
     def submit(self, lane: str, client, query: np.ndarray, v_folding: np.ndarray | None = None, tag=None) -> Request:
         """Queue one query; v_folding None means a packed query (the client's
         keys must carry a conversion key)."""
+        if self._error is not None:
+            raise RuntimeError("the serving loop has failed") from self._error
+        if self._thread is not None and not self._thread.is_alive():
+            raise RuntimeError("the serving loop is not running")
         if lane not in self.lanes:
             raise KeyError(f"no database lane {lane!r}")
         if client not in self.lanes[lane].keys:
@@ -248,16 +285,21 @@ class Server:
                 if not progressed:
                     self._wake.wait(self.poll_interval)
                     self._wake.clear()
-        except BaseException as e:  # fail every waiting request, surface the error on stop()
+        except BaseException as e:  # fail every request not answered, surface the error on stop()
             self._error = e
+            with self._lock:
+                queued = list(self._queue)
+                self._queue.clear()
             for ln in self.lanes.values():
                 for reqs in ln.inflight.values():
                     for r in reqs:
-                        r.error = e
-                        r._ev.set()
+                        r.fail(e)
+                ln.inflight.clear()
                 for r in ln.pending:
-                    r.error = e
-                    r._ev.set()
+                    r.fail(e)
+                ln.pending.clear()
+            for r in queued:
+                r.fail(e)
 
     def _prepare(self, ln: _Lane, buf: int, c: int) -> None:
         if (buf, c) in ln.ready:
@@ -268,6 +310,8 @@ class Server:
         ln.ready.add((buf, c))
 
     def _launch(self, ln: _Lane) -> None:
+        """Queue one batch on a free buffer: upload, stage chain, response copy.
+        Nothing here waits for the device."""
         buf = ln.free.popleft()
         packed = ln.pending[0].v_folding is None
         reqs = []
@@ -275,24 +319,30 @@ class Server:
             reqs.append(ln.pending.popleft())
         n = len(reqs)
         batch = ln.batches[buf]
+        stream = ln.streams[buf].cuda_stream
+        t = time.perf_counter()
         try:
             if packed:
-                batch.load_packed(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs])
+                batch.load_packed(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs], asynchronous=True)
             else:
                 batch.load_many(0, [r.query for r in reqs], [ln.keys[r.client] for r in reqs],
-                                [r.v_folding for r in reqs])
+                                [r.v_folding for r in reqs], asynchronous=True)
             c = size_class(n, ln.capacity)
             self._prepare(ln, buf, c)
-            batch.graph_launch(c, ln.streams[buf].cuda_stream)
+            batch.graph_launch(c, stream)
+            if self.pack_q2_bits:
+                batch.read_packed_async(n, ln.out[buf], stream)
+            else:
+                batch.read_responses_async(n, ln.out[buf], stream)
         except Exception as e:
             for r in reqs:
-                r.error = e
-                r._ev.set()
+                r.fail(e)
+            batch.synchronize()  # whatever was queued ends before the buffer is reused (raises if the device failed)
             ln.free.append(buf)
             return
-        t = time.perf_counter()
         for r in reqs:
             r.t_launch = t
+            r.batch_size = n
         ln.inflight[buf] = reqs
         ln.launched_batches += 1
         ln.launched_queries += n
@@ -301,15 +351,22 @@ class Server:
         done_any = False
         for buf in list(ln.inflight):
             batch = ln.batches[buf]
-            if not batch.poll():
-                continue
-            reqs = ln.inflight.pop(buf)
-            resp = batch.packed_responses(len(reqs)) if self.pack_q2_bits else batch.responses(len(reqs))
-            t = time.perf_counter()
-            for i, r in enumerate(reqs):
-                r.response = resp[i]
-                r.t_done = t
-                r._ev.set()
+            reqs = ln.inflight[buf]
+            try:
+                if not batch.poll():
+                    continue
+                del ln.inflight[buf]
+                out = ln.out[buf]
+                t = time.perf_counter()
+                for i, r in enumerate(reqs):
+                    r.response = out[i].copy()
+                    r.t_done = t
+                    r._ev.set()
+            except BaseException as e:  # a device error: this batch fails, then the loop
+                ln.inflight.pop(buf, None)
+                for r in reqs:
+                    r.fail(e)
+                raise
             ln.free.append(buf)
             done_any = True
         return done_any
@@ -320,6 +377,13 @@ def pinned_u32(n_words: int) -> np.ndarray:
     import torch
 
     return torch.empty(n_words, dtype=torch.int32, pin_memory=True).numpy().view(np.uint32)
+
+
+def pinned_u8(n_bytes: int) -> np.ndarray:
+    """A host byte array in pinned memory (asynchronous DMA of compact responses)."""
+    import torch
+
+    return torch.empty(n_bytes, dtype=torch.uint8, pin_memory=True).numpy()
 
 
 def poisson_times(rate: float, duration: float, rng: np.random.Generator) -> np.ndarray:
@@ -346,9 +410,17 @@ def drive(server: Server, arrivals: list[tuple[float, str, object, np.ndarray, n
 
 
 def latency_stats(reqs: list[Request]) -> dict:
+    """Latency percentiles (submission to completion), split into queue time
+    (submission to launch) and device time (launch to completion), the mean
+    batch size, and the sustained rate over the span of the requests."""
     lat = np.array([r.latency for r in reqs]) * 1e3
+    que = np.array([r.queue_time for r in reqs]) * 1e3
+    dev = np.array([r.device_time for r in reqs]) * 1e3
     t_first = min(r.t_submit for r in reqs)
     t_last = max(r.t_done for r in reqs)
     return {"completed": len(reqs), "sustained_qps": len(reqs) / (t_last - t_first),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "queue_ms": {"p50": float(np.percentile(que, 50)), "p99": float(np.percentile(que, 99))},
+            "device_ms": {"p50": float(np.percentile(dev, 50)), "p99": float(np.percentile(dev, 99))},
+            "mean_batch": float(np.mean([r.batch_size for r in reqs]))}

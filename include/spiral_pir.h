@@ -192,6 +192,13 @@ int sp_db_load_ntt(sp_db *db, const uint32_t *host_ntt);
 int sp_db_destroy(sp_db *db);
 /* HBM bytes of the resident (packed) NTT form. */
 size_t sp_db_device_bytes(const sp_db *db);
+/* Builds now the int8-limb copy of the database that the MFMA multiply reads
+ * (4 bytes per residue: as much HBM as the u32 NTT form), which is otherwise
+ * built on first use; *limb_bytes (may be NULL) gets its size, 0 for shapes
+ * the MFMA multiply does not take (2^nu1 or 2^nu2 < 32).  Lets a server
+ * count it before sizing its batches by free memory.  No reference
+ * counterpart (the reference multiplies on the CUDA cores only). */
+int sp_db_build_limbs(sp_db *db, size_t *limb_bytes);
 
 /* Per-query device state: the query ciphertext, its expansion and folding
  * keys (resident in HBM) and the workspace of one answer.  Replaces the
@@ -267,8 +274,11 @@ int sp_keys_set_packing(sp_keys *k, int t_conv, const uint32_t *host_w);
 /* Response packing for later runs of a batch (q2_bits 0: off).  With packing
  * on, the fold stage ends with sp_pack_response for every slot whose keys
  * carry packing keys (loaded with sp_batch_load_many / _packed), and
- * sp_batch_read_packed copies the compact responses of slots [0, n)
- * ([n][sp_packed_response_bytes] bytes).  No reference counterpart. */
+ * sp_batch_read_packed copies the compact responses of slots [0, n) of the
+ * last run ([n][sp_packed_response_bytes(q2_bits of that run)] bytes).  The
+ * device buffer is sized for q2_bits = 32 on first use and never moves, so
+ * graphs captured under one q2_bits stay valid when another is set.  No
+ * reference counterpart. */
 int sp_batch_set_packing(sp_batch *b, int q2_bits);
 int sp_batch_read_packed(sp_batch *b, int n, uint8_t *host_out);
 /* Loads slots [first, first + n) of a batch in one go: host_queries[i] the
@@ -287,6 +297,29 @@ int sp_batch_load_many(sp_batch *b, int first, int n, const uint32_t *const *hos
  * converted.  No reference counterpart (the reference's conversion is a stub). */
 int sp_batch_load_packed(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
                          sp_keys *const *keys);
+/* Asynchronous forms of sp_batch_load_many / sp_batch_load_packed for
+ * serving loops: the uploads are queued on the batch's io stream behind its
+ * previous run (a device-side wait) and the call returns at once; the next
+ * sp_batch_run / sp_batch_graph_launch / sp_group_launch waits for them on
+ * the device.  The host buffers (pinned for true DMA) must stay valid until
+ * that run has finished (sp_batch_poll).  No reference counterpart; the
+ * reference plans this host/device overlap (gpu-pir/README.md:169-194,
+ * gpu-pir/optplan.md:69-78) without building it. */
+int sp_batch_load_many_async(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                             sp_keys *const *keys, const uint32_t *const *host_v_folding);
+int sp_batch_load_packed_async(sp_batch *b, int first, int n, const uint32_t *const *host_queries,
+                               sp_keys *const *keys);
+/* Asynchronous readouts of the last run's responses of slots [0, n), queued
+ * on `stream` after that run: raw ([n][n2] Ciphertexts, as
+ * sp_batch_read_responses) or compact (as sp_batch_read_packed).  The copy
+ * is complete when sp_batch_poll reports the batch done; host_out (pinned
+ * for true DMA) must stay valid until then. */
+int sp_batch_read_responses_async(sp_batch *b, int n, uint32_t *host_out, void *stream);
+int sp_batch_read_packed_async(sp_batch *b, int n, uint8_t *host_out, void *stream);
+/* Diagnostics: the database-multiply variant the batch's last run used
+ * (1 VALU, 2 MFMA; 0 before any run) and the number of stage-chain graphs it
+ * holds (one per run size, conversion / packing setting and kernel variants). */
+int sp_batch_last_run(const sp_batch *b, int *dbmul_variant, int *graphs_cached);
 /* A packed query into a single-query object (slot 0); response packing and
  * the compact response of a single query, as for batches. */
 int sp_query_load_packed(sp_query *q, const uint32_t *host_query, sp_keys *keys);

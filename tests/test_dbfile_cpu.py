@@ -80,3 +80,36 @@ def test_dat_database_reader(tmp_path, oracle):
     w.tofile(p)
     with pytest.raises(ValueError, match="out of range"):
         dbfile.read_dat_database(p, s)
+
+
+def test_out_of_range_ntt_container_is_refused_before_upload(tmp_path, oracle):
+    """A container whose NTT payload holds a residue >= q_m (a corrupt or foreign
+    file; dbfile.write refuses to make one, so it is patched in place) is rejected
+    by Database.load_file before anything reaches the library, whose packed
+    28-bit form would otherwise spill the word into a neighbouring column."""
+    from spiral_pir import Shape, dbfile
+    from spiral_pir.server import Database
+
+    s = Shape(4, 2)
+    db = oracle.db_encode(_records(s, 15), s.n2, s.nu1, s.nu2)
+    p = str(tmp_path / "db.spdb")
+    dbfile.write(p, s, dbfile.KIND_NTT, db)
+    raw = bytearray(open(p, "rb").read())
+    word = dbfile.HEADER.size + 4 * (3 * 2 * N + N + 17)  # residue q2 of polynomial 3, coefficient 17
+    raw[word:word + 4] = (1 << 28).to_bytes(4, "little")
+    open(p, "wb").write(raw)
+    shape, kind, payload = dbfile.read(p)  # the container itself is well formed
+
+    class Target:
+        calls = []
+        shape = s
+
+        def load_ntt(self, payload, check=True):
+            self.calls.append("ntt")
+
+    t = Target()
+    with pytest.raises(ValueError, match="out of range"):
+        Database.load_file(t, p)
+    assert t.calls == []
+    with pytest.raises(ValueError, match="out of range"):
+        Database.load_ntt(t, payload)

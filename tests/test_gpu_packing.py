@@ -106,6 +106,47 @@ def test_batch_packed_responses(gpu_ctx, oracle):
     db.close()
 
 
+def test_packing_width_changes_between_graph_launches(gpu_ctx, oracle):
+    """q2_bits cycled 11 -> 20 -> 11 -> 32 -> 20 between graph launches: each width
+    has its own captured chain and the device buffer of the compact responses
+    never moves, so every launch's compact responses equal those of an eager run
+    and the oracle's packing of the raw responses, and decode.  A readout after
+    the width changed without a new run is refused."""
+    import torch
+    from spiral_pir import SpiralError
+    from spiral_pir.client import Client
+    from spiral_pir.server import Batch, Keys
+
+    s, recs, db = _db(gpu_ctx, 6, 2, 8150)
+    c = Client(s, 8250)
+    left, right = c.keygen()
+    w = c.packing_keys()
+    keys = Keys(db, left, right, c.conversion_key(), w)
+    idxs = (7, 190)
+    b = Batch(db, 2)
+    b.load_packed(0, [c.query_packed(i) for i in idxs], [keys, keys])
+    stream = torch.cuda.current_stream().cuda_stream
+    for q2 in (11, 20, 11, 32, 20):
+        b.set_packing(q2)
+        (torch.randn(1 << 16, device="cuda") * 2).sum().item()
+        b.graph_launch(2, stream)
+        got = b.packed_responses(2)
+        raw = b.responses(2)
+        b.run(2)
+        np.testing.assert_array_equal(b.packed_responses(2), got, err_msg=f"q2 {q2}: graph vs eager")
+        for slot, idx in enumerate(idxs):
+            np.testing.assert_array_equal(got[slot], oracle.pack_response(raw[slot], 2, w, 4, q2),
+                                          err_msg=f"q2 {q2} slot {slot}")
+            np.testing.assert_array_equal(c.decode_packed(got[slot], q2), recs[idx])
+    assert b.last_run()["graphs"] == 3  # one chain per width
+    b.set_packing(11)
+    with pytest.raises(SpiralError, match="packing changed"):
+        b.packed_responses(2)
+    b.close()
+    keys.close()
+    db.close()
+
+
 def test_packed_query_and_response_nu9_6(gpu_ctx, oracle):
     """(9, 6): the whole Spiral server path for one query (expansion, Regev -> GSW
     conversion, database multiply, fold, packing, modulus switching) decodes."""

@@ -297,6 +297,32 @@ def test_answer_config1_shape(gpu_ctx, oracle):
     _answer_case(gpu_ctx, oracle, 9, 6, 1001, 2001, 12345)
 
 
+def test_sp_answer_matches_oracle(gpu_ctx, oracle):
+    """sp_answer, the one-call host entry point (host query, keys and fold vectors
+    in, n2 raw response ciphertexts out), at (6, 2) and (9, 6): equal to the
+    oracle's answer; wrong-size inputs are refused before the library call."""
+    from spiral_pir import Shape
+    from spiral_pir.client import Client
+    from spiral_pir.server import Database, answer
+
+    for (nu1, nu2, seed, index) in ((6, 2, 1000, 37), (9, 6, 1001, 12345)):
+        s = Shape(nu1, nu2)
+        recs = rng(seed).integers(0, 256, size=(s.records, s.n2, N), dtype=np.uint8)
+        db = Database(gpu_ctx, s)
+        db.load_bytes(recs)
+        c = Client(s, 2000 + seed)
+        left, right = c.keygen()
+        qct, vf = c.query(index)
+        got = answer(db, qct, left, right, vf)
+        exp = oracle.answer(oracle.db_encode(recs, s.n2, nu1, nu2), s.n2, nu1, nu2, s.t_gsw, s.t_exp, qct, left,
+                            right, vf, oracle.folding_neg(vf, nu2, s.t_gsw))
+        np.testing.assert_array_equal(got, exp, err_msg=f"({nu1},{nu2})")
+        np.testing.assert_array_equal(c.decode(got), recs[index])
+        with pytest.raises(ValueError, match="v_folding"):
+            answer(db, qct, left, right, vf.reshape(-1)[:-1])
+        db.close()
+
+
 def test_query_rerun_is_identical(gpu_ctx):
     """Running the same resident query twice gives the same response (stage 1 restarts
     from the stored query)."""

@@ -85,6 +85,67 @@ def test_batch_load_many_matches_per_slot_load(gpu_ctx):
     db.close()
 
 
+def test_async_loads_and_readouts_match_blocking(gpu_ctx):
+    """Asynchronous bulk loads (queued behind the previous run on the device) and
+    asynchronous readouts into pinned memory give the blocking path's responses:
+    packed queries; fold-vector queries in two loads before one run; and a
+    second load queued right after a launch, before that launch has finished,
+    which must not disturb the first run's results."""
+    import torch
+    from spiral_pir.client import Client
+    from spiral_pir.server import Batch, Keys
+    from spiral_pir.serving import pinned_u32
+
+    s, recs, db = _db(gpu_ctx, 6, 4, 1250)
+    clients = [Client(s, 5250 + k) for k in range(2)]
+    keys, mats = [], []
+    for c in clients:
+        left, right = c.keygen()
+        keys.append(Keys(db, left, right, c.conversion_key()))
+        mats.append((left, right))
+    idxs = [int(i) for i in rng(125).integers(0, s.records, size=8)]
+    packed = [clients[j % 2].query_packed(idxs[j]) for j in range(8)]
+    vec = [clients[j % 2].query(idxs[j]) for j in range(8)]
+    ks = [keys[j % 2] for j in range(8)]
+    ref = Batch(db, 4)
+    ref.load_packed(0, packed[:4], ks[:4])
+    ref.run(4)
+    ref_p1 = ref.responses(4)
+    ref.load_packed(0, packed[4:], ks[4:])
+    ref.run(4)
+    ref_p2 = ref.responses(4)
+    ref.load_many(0, [q for q, _ in vec[:4]], ks[:4], [v for _, v in vec[:4]])
+    ref.run(4)
+    ref_v = ref.responses(4)
+    ref.close()
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    b = Batch(db, 4)
+    out1, out2, out3 = (pinned_u32(4 * s.response_words) for _ in range(3))
+    b.load_packed(0, packed[:4], ks[:4], asynchronous=True)
+    b.graph_launch(4, sp)
+    b.read_responses_async(4, out1, sp)
+    b.load_packed(0, packed[4:], ks[4:], asynchronous=True)  # queued while the first run may still be going
+    b.graph_launch(4, sp)
+    b.read_responses_async(4, out2, sp)
+    b.load_many(0, [q for q, _ in vec[:2]], ks[:2], [v for _, v in vec[:2]], asynchronous=True)
+    b.load_many(2, [q for q, _ in vec[2:4]], ks[2:4], [v for _, v in vec[2:4]], asynchronous=True)
+    b.run(4, stream=sp)
+    b.read_responses_async(4, out3, sp)
+    while not b.poll():
+        pass
+    shape = (4, s.n2, 2, 2, N)
+    np.testing.assert_array_equal(out1.reshape(shape), ref_p1)
+    np.testing.assert_array_equal(out2.reshape(shape), ref_p2)
+    np.testing.assert_array_equal(out3.reshape(shape), ref_v)
+    for j in range(4):
+        np.testing.assert_array_equal(clients[j % 2].decode(out2.reshape(shape)[j]), recs[idxs[4 + j]])
+    b.close()
+    for k in keys:
+        k.close()
+    db.close()
+
+
 def test_group_launch_three_databases(gpu_ctx):
     """Batches on three databases of different shapes in one graph launch, replayed
     twice with eager work between: every slot equals its query answered alone."""

@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -84,3 +85,73 @@ def test_shape_helpers():
     assert s.db_device_bytes == 7 * 2 ** 28
     assert Shape(6, 2).db_device_bytes == Shape(6, 2).db_ntt_bytes
     assert s.records * s.record_bytes == 2 ** 28
+
+
+class _FakeDb:
+    """Stands in for a Database in argument checks that fail before any library call."""
+
+    def __init__(self, shape):
+        self.shape = shape
+        self.handle = None
+        self.calls = []
+
+    def load_ntt(self, payload, check=True):
+        self.calls.append(("ntt", payload.shape))
+
+    def load_bytes(self, payload):
+        self.calls.append(("bytes", payload.shape))
+
+
+def _arrays(s):
+    return (np.zeros(s.query_words, np.uint32), np.zeros(s.left_words, np.uint32),
+            np.zeros(s.right_words, np.uint32), np.zeros(s.fold_words, np.uint32))
+
+
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
+def test_host_inputs_of_the_wrong_size_are_refused(which):
+    """Query.load / Batch.load / answer / Keys check every host array against the
+    shape before the library copies the byte count the shape implies."""
+    from spiral_pir import Shape
+    from spiral_pir.server import Batch, Keys, Query, answer
+
+    s = Shape(6, 2)
+    db = _FakeDb(s)
+    arrs = list(_arrays(s))
+    arrs[which] = arrs[which][:-1]
+    q = Query.__new__(Query)
+    q.db, q._h = db, None
+    b = Batch.__new__(Batch)
+    b.db, b._h, b._hold = db, None, []
+    names = ("query", "left", "right", "v_folding")
+    with pytest.raises(ValueError, match=names[which]):
+        q.load(*arrs)
+    with pytest.raises(ValueError, match=names[which]):
+        b.load(0, *arrs)
+    with pytest.raises(ValueError, match=names[which]):
+        answer(db, *arrs)
+    if which in (1, 2):
+        with pytest.raises(ValueError, match=names[which]):
+            Keys(db, arrs[1], arrs[2])
+    if which == 0:
+        with pytest.raises(ValueError, match="query"):
+            q.load_packed(arrs[0], None)
+        with pytest.raises(ValueError, match="query"):
+            b.load_packed(0, [arrs[0]], [None])
+    if which in (0, 3):
+        with pytest.raises(ValueError):
+            b.load_many(0, [arrs[0]], [None], [arrs[3]])
+
+
+def test_bulk_loads_need_open_keys():
+    from spiral_pir import Shape
+    from spiral_pir.server import Batch
+
+    s = Shape(6, 2)
+    b = Batch.__new__(Batch)
+    b.db, b._h, b._hold = _FakeDb(s), None, []
+    q, _, _, v = _arrays(s)
+    with pytest.raises(ValueError, match="Keys"):
+        b.load_many(0, [q], [object()], [v])
+    with pytest.raises(ValueError, match="Keys"):
+        b.load_packed(0, [q], [object()], asynchronous=True)
+    assert b._hold == []  # nothing held for a load that never reached the library

@@ -248,3 +248,45 @@ def test_threads_do_not_change_results(oracle):
         oracle.set_threads(1)
     np.testing.assert_array_equal(out[1][0], out[4][0])
     np.testing.assert_array_equal(out[1][1], out[4][1])
+
+
+def test_expansion_key_set_choice_planted(oracle):
+    """Which key set each expansion step uses (gpu-pir/expansion.cuh:121-130): in
+    round r, element i takes left[r] when r != 0 and i is even, else right[r].
+    Planted mistakes in the oracle's inputs: corrupting right[r] (1 <= r <=
+    stop_round) changes exactly the odd elements of round r and no even one,
+    corrupting left[r] exactly the even ones; in round 0 right[0] serves both
+    elements and left[0] neither; past stop_round left[r] changes the even
+    elements only (odd ones are not updated there).  A swap of the key sets, a
+    wrong parity test or a wrong round index fails one of these."""
+    nu1, nu2, t = 6, 2, 8
+    g_, st = oracle.expansion_g(nu1, nu2, t), oracle.expansion_stop_round(nu2, t)
+    assert (g_, st) == (7, 4)
+    g = rng(4242)
+    q = rand_residues(g, 2).reshape(2, 2, N)
+    def keys(gen, levels):  # key matrices [level][crt][col][row][z], canonical residues
+        k = np.empty((levels, 2, t, 2, N), np.uint32)
+        k[:, 0] = gen.integers(0, Q0, size=(levels, t, 2, N), dtype=np.uint32)
+        k[:, 1] = gen.integers(0, Q1, size=(levels, t, 2, N), dtype=np.uint32)
+        return k
+
+    left, right = keys(g, g_), keys(g, st + 1)
+
+    def changed(r, which):
+        base = oracle.expand(q, nu1, nu2, t, t, left, right, rounds=r + 1)
+        lft, rgt = left.copy(), right.copy()
+        k = lft if which == "left" else rgt
+        k[r] = keys(rng(100 + r), 1)[0]
+        bad = oracle.expand(q, nu1, nu2, t, t, lft, rgt, rounds=r + 1)
+        diff = {i for i in range(1 << g_) if not np.array_equal(base[i], bad[i])}
+        assert all(np.array_equal(base[i], bad[i]) for i in range(1 << (r + 1), 1 << g_))
+        return diff
+
+    live = lambda r: set(range(1 << (r + 1)))  # noqa: E731
+    assert changed(0, "right") == {0, 1}
+    assert changed(0, "left") == set()
+    for r in range(1, st + 1):
+        assert changed(r, "right") == {i for i in live(r) if i % 2}, f"right[{r}]"
+        assert changed(r, "left") == {i for i in live(r) if i % 2 == 0}, f"left[{r}]"
+    for r in range(st + 1, g_):
+        assert changed(r, "left") == {i for i in live(r) if i % 2 == 0}, f"left[{r}] past stop_round"

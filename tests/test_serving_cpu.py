@@ -15,29 +15,36 @@ class FakeStream:
 
 
 class FakeBatch:
-    """Answers each slot with a vector filled with the query's first word."""
+    """Answers each slot with a vector filled with the query's first word.
+    fail_poll: poll() raises from the k-th completed launch on (a device error)."""
 
-    def __init__(self, fail_load: bool = False, fail_after: int | None = None):
+    def __init__(self, fail_load: bool = False, fail_after: int | None = None, fail_poll: int | None = None):
         self.fail_load = fail_load
         self.fail_after = fail_after
+        self.fail_poll = fail_poll
         self.loaded = []
         self.launches = []
         self.tuned = []
         self.forms = []
+        self.asynchronous = []
+        self.polled = 0
+        self.pending_out = None
 
-    def load_many(self, first, queries, keys, v_folding):
+    def load_many(self, first, queries, keys, v_folding, asynchronous=False):
         if self.fail_load or (self.fail_after is not None and len(self.launches) >= self.fail_after):
             raise RuntimeError("injected load failure")
         assert first == 0 and len(queries) == len(keys) == len(v_folding)
         self.loaded = [int(q[0]) for q in queries]
         self.forms.append(("vectors", len(queries)))
+        self.asynchronous.append(asynchronous)
 
-    def load_packed(self, first, queries, keys):
+    def load_packed(self, first, queries, keys, asynchronous=False):
         if self.fail_load:
             raise RuntimeError("injected load failure")
         assert first == 0 and len(queries) == len(keys)
         self.loaded = [int(q[0]) for q in queries]
         self.forms.append(("packed", len(queries)))
+        self.asynchronous.append(asynchronous)
 
     def run(self, n, stages, stream=None):
         pass
@@ -49,11 +56,22 @@ class FakeBatch:
     def graph_launch(self, n, stream=None):
         self.launches.append((n, len(self.loaded)))
 
-    def poll(self):
-        return True
+    def read_responses_async(self, n, out, stream=None):
+        self.pending_out = (out, list(self.loaded[:n]))
 
-    def responses(self, n):
-        return np.stack([np.full(4, v, np.uint32) for v in self.loaded[:n]])
+    def synchronize(self):
+        pass
+
+    def poll(self):
+        if self.fail_poll is not None and self.polled >= self.fail_poll:
+            raise RuntimeError("injected device error")
+        self.polled += 1
+        if self.pending_out is not None:  # the queued copy lands when the run is seen done
+            out, vals = self.pending_out
+            for i, v in enumerate(vals):
+                out[i] = v
+            self.pending_out = None
+        return True
 
 
 def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
@@ -65,6 +83,7 @@ def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
         ln = _Lane.__new__(_Lane)
         ln.name, ln.db, ln.capacity = name, None, capacity
         ln.batches = batches
+        ln.out = [np.zeros((capacity, 4), np.uint32) for _ in batches]
         ln.streams = [FakeStream() for _ in batches]
         ln.free = collections.deque(range(len(batches)))
         ln.inflight, ln.pending, ln.ready = {}, collections.deque(), set()
@@ -179,3 +198,57 @@ def test_packed_request_needs_conversion_key():
     srv = _server({"x": [FakeBatch()]})
     with pytest.raises(ValueError, match="conversion key"):
         srv.submit("x", "c", _q(0), None)
+
+
+def test_loads_are_queued_asynchronously():
+    b = FakeBatch()
+    srv = _server({"x": [b]}, capacity=4)
+    srv.start()
+    assert int(srv.submit("x", "c", _q(5), np.zeros(2, np.uint32)).wait(5)[0]) == 5
+    srv.stop()
+    assert b.asynchronous == [True]
+
+
+def test_device_error_fails_every_request_and_the_server():
+    """A readout that raises fails its own batch, every pending and queued request
+    (none is left waiting forever), and later submissions raise."""
+    bad = FakeBatch(fail_poll=0)
+    srv = _server({"x": [bad]}, capacity=2)
+    reqs = [srv.submit("x", "c", _q(i), np.zeros(2, np.uint32), tag=i) for i in range(5)]
+    srv.start()
+    for r in reqs:
+        with pytest.raises(RuntimeError, match="injected device error"):
+            r.wait(5)
+    with pytest.raises(RuntimeError, match="injected device error"):
+        srv.stop()
+    with pytest.raises(RuntimeError, match="failed"):
+        srv.submit("x", "c", _q(9), np.zeros(2, np.uint32))
+
+
+def test_device_error_after_good_batches():
+    """Batches answered before the error keep their responses; the rest fail."""
+    b = FakeBatch(fail_poll=2)
+    srv = _server({"x": [b]}, capacity=1)
+    srv.start()
+    first = [srv.submit("x", "c", _q(i), np.zeros(2, np.uint32), tag=i) for i in range(2)]
+    for r in first:
+        assert int(r.wait(5)[0]) == r.tag
+    later = [srv.submit("x", "c", _q(10 + i), np.zeros(2, np.uint32)) for i in range(3)]
+    for r in later:
+        with pytest.raises(RuntimeError):
+            r.wait(5)
+    with pytest.raises(RuntimeError):
+        srv.stop()
+
+
+def test_latency_stats_split_queue_and_device_time():
+    from spiral_pir.serving import Request, latency_stats
+
+    reqs = []
+    for k in range(4):
+        r = Request("x", "c", _q(k), None)
+        r.t_submit, r.t_launch, r.t_done, r.batch_size = 1.0 + k, 1.001 + k, 1.003 + k, 2
+        reqs.append(r)
+    st = latency_stats(reqs)
+    assert abs(st["p50_ms"] - 3.0) < 1e-6 and abs(st["queue_ms"]["p50"] - 1.0) < 1e-6
+    assert abs(st["device_ms"]["p99"] - 2.0) < 1e-6 and st["mean_batch"] == 2.0
