@@ -313,38 +313,3 @@ def test_dbmul_mfma_partial_row_tile(gpu_ctx, force_dbmul):
         np.testing.assert_array_equal(b.read(k, READ_DB_MUL), ref[k], err_msg=f"slot {k}")
     b.close()
     db.close()
-
-
-def test_tuned_batch_64_at_9_6(gpu_ctx):
-    """The benchmark's batched configuration: (9, 6), 64 queries from 8 clients, the
-    autotuned database multiply, graph replays.  Every slot decodes, the graph result
-    equals the VALU-forced eager result, and the bulk response readout equals the
-    per-slot readout."""
-    import torch
-    from spiral_pir.server import Batch, READ_RESPONSE, STAGE_EXPAND
-
-    s, recs, db = _setup(gpu_ctx, 9, 6, 1001)
-    B = 64
-    qs = _clients_queries(s, B, 8, 3900)
-    b = Batch(db, B)
-    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
-        b.load(slot, q, left, right, vf)
-    b.run(B, STAGE_EXPAND)
-    info = b.tune(B)
-    stream = torch.cuda.current_stream().cuda_stream
-    for _ in range(2):
-        b.graph_launch(B, stream)
-    b.synchronize()
-    got = b.responses(B)
-    for slot in (0, 17, 63):
-        np.testing.assert_array_equal(b.read(slot, READ_RESPONSE), got[slot], err_msg=f"slot {slot}")
-    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
-        np.testing.assert_array_equal(c.decode(got[slot]), recs[idx], err_msg=f"slot {slot} ({info})")
-    gpu_ctx.set_variant(2, 1)
-    try:
-        b.run(B)
-        np.testing.assert_array_equal(b.responses(B), got)
-    finally:
-        gpu_ctx.set_variant(2, 0)
-    b.close()
-    db.close()

@@ -110,7 +110,7 @@ def test_packing_width_changes_between_graph_launches(gpu_ctx, oracle):
     """q2_bits cycled 11 -> 20 -> 11 -> 32 -> 20 between graph launches: each width
     has its own captured chain and the device buffer of the compact responses
     never moves, so every launch's compact responses equal those of an eager run
-    and the oracle's packing of the raw responses, and decode.  A readout after
+    and the oracle's packing of the raw responses, and decode (q2_bits >= 16).  A readout after
     the width changed without a new run is refused."""
     import torch
     from spiral_pir import SpiralError
@@ -137,7 +137,8 @@ def test_packing_width_changes_between_graph_launches(gpu_ctx, oracle):
         for slot, idx in enumerate(idxs):
             np.testing.assert_array_equal(got[slot], oracle.pack_response(raw[slot], 2, w, 4, q2),
                                           err_msg=f"q2 {q2} slot {slot}")
-            np.testing.assert_array_equal(c.decode_packed(got[slot], q2), recs[idx])
+            if q2 >= 16:  # 11 bits of row 0 keep too little of a * s for the record to decode
+                np.testing.assert_array_equal(c.decode_packed(got[slot], q2), recs[idx])
     assert b.last_run()["graphs"] == 3  # one chain per width
     b.set_packing(11)
     with pytest.raises(SpiralError, match="packing changed"):
