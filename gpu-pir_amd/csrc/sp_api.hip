@@ -123,9 +123,18 @@ static void build_tw(uint32_t *out, int m, int inv) {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// Scratch of sp_fold_noalloc: one array per GPU for the life of the process
+// (the converted fold keys and the split fold's partial sums).  Every call on
+// a GPU shares that GPU's copy, so calls that overlap on one GPU race; nothing
+// in it must start at zero.  Host code uses the address hipGetSymbolAddress
+// gives for the context's device (sp_ctx::d_scratch), never the name.
+constexpr size_t NOALLOC_SCRATCH_WORDS = (size_t)2 << 20;  // 8 MiB
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
 struct sp_ctx {
   int device = 0;
   int variant[3] = {0, 0, 0};  // per stage (SP_STAGE_*): 0 = automatic choice
+  uint32_t *d_scratch = nullptr;  // this device's copy of `scratch`
   hipStream_t stream = nullptr;
   uint32_t *d_tw = nullptr;
   uint32_t *d_neg1 = nullptr;
@@ -149,7 +158,12 @@ extern "C" int sp_ctx_create(int device, sp_ctx **out) {
   HIP_TRY(hipSetDevice(device));
   sp_ctx *c = new sp_ctx();
   c->device = device;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&c->d_scratch), HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !c->d_scratch) {
+    delete c;
+    return fail(SP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(SP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
   std::vector<uint32_t> tw((size_t)4 * TW_WORDS);
   for (int m = 0; m < 2; m++)
@@ -316,37 +330,79 @@ extern "C" int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *de
   return SP_OK;
 }
 
-extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
-                       const uint32_t *dev_v_folding_neg, void *stream) {
-  if (!ctx || !dev_accs || !dev_v_folding) return fail(SP_ERR_ARG, "sp_fold: NULL");
-  if (nu2 < 1 || nu2 > 12 || t_gsw < 2 || t_gsw > 28) return fail(SP_ERR_ARG, "sp_fold: nu2 %d, t_gsw %d", nu2, t_gsw);
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipStream_t s = pick(ctx, stream);
+static int check_fold_args(const sp_ctx *ctx, const uint32_t *dev_accs, int nu2, int t_gsw,
+                           const uint32_t *dev_v_folding, const char *who) {
+  if (!ctx || !dev_accs || !dev_v_folding) return fail(SP_ERR_ARG, "%s: NULL", who);
+  if (nu2 < 1 || nu2 > 12 || t_gsw < 2 || t_gsw > 28) return fail(SP_ERR_ARG, "%s: nu2 %d, t_gsw %d", who, nu2, t_gsw);
+  return SP_OK;
+}
+
+// Device words of the fold's temporaries: keys in layout P (and the given
+// negated keys), then the split fold's partial sums ((2^nu2 / 2) pairs x
+// [m][h][row][N]) when that form runs.
+struct FoldScratch {
+  size_t keys = 0, neg = 0, part = 0;
+  size_t total() const { return keys + neg + part; }
+};
+static FoldScratch fold_scratch(const sp_ctx *ctx, int nu2, int t_gsw, bool fast) {
+  FoldScratch f;
+  f.keys = (size_t)2 * (2 * t_gsw) * 2 * nu2 * N;
+  f.neg = fast ? 0 : f.keys;
+  f.part = fast && t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1 ? (size_t)((1 << nu2) / 2) * 8 * N : 0;
+  return f;
+}
+
+// The fold stages of sp_fold / sp_fold_noalloc on stream s, temporaries in
+// kp (keys), kn (negated keys, general form) and part (partial sums).
+static int fold_enqueue(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
+                        const uint32_t *dev_v_folding_neg, uint32_t *kp, uint32_t *kn, uint32_t *part, hipStream_t s) {
   const int mx = 2 * t_gsw;
   const size_t level_polys = (size_t)2 * mx * 2;
-  DevBuf kp, kn;
-  HIP_TRY(kp.alloc(level_polys * nu2 * N * 4));
-  HIP_TRY(launch_nat_to_p(dev_v_folding, kp.u32(), level_polys * nu2, 2 * mx, 1, ctx->tables, s));
   const bool fast = dev_v_folding_neg == nullptr;
-  if (!fast) {
-    HIP_TRY(kn.alloc(level_polys * nu2 * N * 4));
-    HIP_TRY(launch_nat_to_p(dev_v_folding_neg, kn.u32(), level_polys * nu2, 2 * mx, 1, ctx->tables, s));
-  }
-  DevBuf part;  // split-fold partial sums: (2^nu2 / 2) pairs x [m][h][row][N]
-  if (fast && t_gsw == 8) HIP_TRY(part.alloc((size_t)((1 << nu2) / 2) * 8 * N * 4));
+  HIP_TRY(launch_nat_to_p(dev_v_folding, kp, level_polys * nu2, 2 * mx, 1, ctx->tables, s));
+  if (!fast) HIP_TRY(launch_nat_to_p(dev_v_folding_neg, kn, level_polys * nu2, 2 * mx, 1, ctx->tables, s));
   int num_per = 1 << nu2;
   for (int cur = 0; cur < nu2; cur++) {
     num_per >>= 1;
     const size_t lvl = (size_t)(nu2 - 1 - cur) * level_polys * N;
     if (fast && t_gsw == 8 && ctx->variant[SP_STAGE_FOLD] != 1)
-      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, 0, ctx->tables, ctx->variant[SP_STAGE_FOLD],
-                           part.u32(), 0, s));
+      HIP_TRY(launch_fold8(dev_accs, 0, 1, 1, num_per, kp + lvl, 0, ctx->tables, ctx->variant[SP_STAGE_FOLD], part, 0,
+                           s));
     else
-      HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp.u32() + lvl, fast ? nullptr : kn.u32() + lvl, 0,
-                                t_gsw, gadget_bits(t_gsw), ctx->tables, s));
+      HIP_TRY(launch_fold_level(fast, dev_accs, 0, 1, 1, num_per, kp + lvl, fast ? nullptr : kn + lvl, 0, t_gsw,
+                                gadget_bits(t_gsw), ctx->tables, s));
   }
+  return SP_OK;
+}
+
+extern "C" int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
+                       const uint32_t *dev_v_folding_neg, void *stream) {
+  int rc = check_fold_args(ctx, dev_accs, nu2, t_gsw, dev_v_folding, "sp_fold");
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  const FoldScratch f = fold_scratch(ctx, nu2, t_gsw, dev_v_folding_neg == nullptr);
+  DevBuf kp, kn, part;
+  HIP_TRY(kp.alloc(f.keys * 4));
+  if (f.neg) HIP_TRY(kn.alloc(f.neg * 4));
+  if (f.part) HIP_TRY(part.alloc(f.part * 4));
+  rc = fold_enqueue(ctx, dev_accs, nu2, t_gsw, dev_v_folding, dev_v_folding_neg, kp.u32(), kn.u32(), part.u32(), s);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(s));
   return SP_OK;
+}
+
+extern "C" int sp_fold_noalloc(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
+                               const uint32_t *dev_v_folding_neg, void *stream) {
+  int rc = check_fold_args(ctx, dev_accs, nu2, t_gsw, dev_v_folding, "sp_fold_noalloc");
+  if (rc) return rc;
+  const FoldScratch f = fold_scratch(ctx, nu2, t_gsw, dev_v_folding_neg == nullptr);
+  if (f.total() > NOALLOC_SCRATCH_WORDS)
+    return fail(SP_ERR_ARG, "sp_fold_noalloc: nu2 %d, t_gsw %d needs %zu bytes of scratch, the maximum is %zu", nu2,
+                t_gsw, f.total() * 4, NOALLOC_SCRATCH_WORDS * 4);
+  HIP_TRY(hipSetDevice(ctx->device));
+  uint32_t *kp = ctx->d_scratch, *kn = kp + f.keys, *part = kn + f.neg;
+  return fold_enqueue(ctx, dev_accs, nu2, t_gsw, dev_v_folding, dev_v_folding_neg, kp, kn, part, pick(ctx, stream));
 }
 
 extern "C" int sp_regev_to_gsw(sp_ctx *ctx, const uint32_t *dev_v, size_t n_ct, int nu2, int t_gsw, int t_conv,

@@ -48,6 +48,7 @@ SIGNATURES = {
     "sp_gadget_extprod": [P, P, I, I, P, P, SZ, SZ, SZ, SZ, P],
     "sp_db_mul": [P, P, P, P, I, I, P],
     "sp_fold": [P, P, I, I, P, P, P],
+    "sp_fold_noalloc": [P, P, I, I, P, P, P],
     "sp_expansion_depth": [I, I, I, PI, PI],
     "sp_expand": [P, P, I, I, I, I, P, P, I, P],
     "sp_regev_to_gsw": [P, P, SZ, I, I, I, P, P, P],

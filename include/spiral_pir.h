@@ -133,6 +133,17 @@ int sp_db_mul(sp_ctx *ctx, const uint32_t *dev_db, const uint32_t *dev_first_dim
  * evaluated with half the NTTs. */
 int sp_fold(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
             const uint32_t *dev_v_folding_neg, void *stream);
+/* sp_fold without any allocation: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result as
+ * sp_fold; the converted keys and partial sums live in one static 8 MiB
+ * device array per GPU, held until the process ends, so nu2 and t_gsw must
+ * fit it (fast form, t_gsw = 8: nu2 <= 7; given v_folding_neg: up to 7 levels
+ * of 16 key columns) or the call fails with SP_ERR_ARG.  Asynchronous on
+ * `stream` (no host synchronization).  Calls that overlap on one GPU share
+ * that array and race: serialize them (one stream, or events). */
+int sp_fold_noalloc(sp_ctx *ctx, uint32_t *dev_accs, int nu2, int t_gsw, const uint32_t *dev_v_folding,
+                    const uint32_t *dev_v_folding_neg, void *stream);
 
 /* ---- coefficient expansion -----------------------------------------------
  * coefficient_expansion_r over all rounds (gpu-pir/expansion.cuh:222-260,
