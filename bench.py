@@ -295,31 +295,57 @@ def run_group(torch, mods, ctx, main_db, steps: int, stream) -> tuple[dict, dict
     return out, dbs
 
 
-def run_serving(torch, mods, dbs, rates, seconds: float) -> dict:
-    """Poisson arrivals, 70/30 over the resident (9,6)/(10,7) databases, continuous batching."""
+def run_serving(torch, mods, dbs, rates, seconds: float, form: str = "packed", q2_bits: int = 20) -> dict:
+    """Poisson arrivals, 70/30 over the resident (9,6)/(10,7) databases, continuous batching.
+
+    form "packed": Spiral's whole server path per request, a packed query (32 KiB; the
+    fold vectors are converted on the device) in and a compact response (q2_bits) out;
+    form "vectors": a query with its GSW fold vectors in (3 MiB at (9, 6)), the n2 raw
+    response ciphertexts out.  Every response is compared with the same query
+    answered alone by a single-query object."""
     from spiral_pir.serving import Server, drive, latency_stats, pinned_u32, poisson_times
 
-    Client, Query = mods["Client"], mods["Query"]
+    Client, Query, Keys = mods["Client"], mods["Query"], mods["Keys"]
+    packed = form == "packed"
     lanes = {"9,6": dbs["9,6"], "10,7": dbs["10,7"]}
-    srv = Server({k: v[2] for k, v in lanes.items()}, max_batch=128, buffers=2, weights={"9,6": 0.5, "10,7": 0.5})
+    srv = Server({k: v[2] for k, v in lanes.items()}, max_batch=128, buffers=2, weights={"9,6": 0.5, "10,7": 0.5},
+                 pack_q2_bits=q2_bits if packed else 0)
     pools = {}
     t0 = time.perf_counter()
     for li, (name, (shape, records, db)) in enumerate(lanes.items()):
-        qs = _client_queries(Client, shape, 16, 4, 4000 + 100 * li)
+        clients = [Client(shape, 4000 + 100 * li + k) for k in range(4)]
+        idx = [int(x) for x in np.random.default_rng(4050 + li).integers(0, shape.records, size=16)]
         pool = []
-        for j, (cl, (left, right), idx, q, vf) in enumerate(qs):
-            cid = (name, j % 4)
-            if cid not in srv.lanes[name].keys:
-                srv.register(name, cid, left, right)
-            one = Query(db)
-            one.load(q, left, right, vf)
-            one.run()
-            ref = one.read()
-            one.close()
-            pq, pv = pinned_u32(q.size), pinned_u32(vf.size)
-            pq[:] = q.reshape(-1)
-            pv[:] = vf.reshape(-1)
-            pool.append((cid, pq, pv, ref, bool(np.array_equal(cl.decode(ref), records[idx]))))
+        for k, cl in enumerate(clients):
+            left, right = cl.keygen()
+            conv, w = (cl.conversion_key(), cl.packing_keys()) if packed else (None, None)
+            srv.register(name, (name, k), left, right, conv, w)
+            keys = Keys(db, left, right, conv, w) if packed else None
+            for j in range(k, 16, 4):
+                one = Query(db)
+                if packed:
+                    q, vf = cl.query_packed(idx[j]), None
+                    one.set_packing(q2_bits)
+                    one.load_packed(q, keys)
+                    one.run()
+                    ref = one.read_packed()
+                    ok = bool(np.array_equal(cl.decode_packed(ref, q2_bits), records[idx[j]]))
+                else:
+                    q, vf = cl.query(idx[j])
+                    one.load(q, left, right, vf)
+                    one.run()
+                    ref = one.read()
+                    ok = bool(np.array_equal(cl.decode(ref), records[idx[j]]))
+                one.close()
+                pq = pinned_u32(q.size)
+                pq[:] = q.reshape(-1)
+                pv = None
+                if vf is not None:
+                    pv = pinned_u32(vf.size)
+                    pv[:] = vf.reshape(-1)
+                pool.append(((name, k), pq, pv, ref, ok))
+            if keys is not None:
+                keys.close()
         pools[name] = pool
         srv.warmup(name, pool[0][0], pool[0][1], pool[0][2])
     t_setup = time.perf_counter() - t0
@@ -346,12 +372,18 @@ def run_serving(torch, mods, dbs, rates, seconds: float) -> dict:
             sub = [r for r in reqs if r.lane == name]
             if sub:
                 ls = latency_stats(sub)
-                by_lane[name] = {"completed": ls["completed"], "p50_ms": ls["p50_ms"], "p99_ms": ls["p99_ms"]}
+                by_lane[name] = {"completed": ls["completed"], "p50_ms": ls["p50_ms"], "p99_ms": ls["p99_ms"],
+                                 "queue_ms": ls["queue_ms"], "device_ms": ls["device_ms"],
+                                 "mean_batch": ls["mean_batch"]}
         results.append({"offered_qps": rate, **st, "by_database": by_lane, "responses_equal_alone": bool(same)})
     srv.stop()
+    what = (f"packed queries (32 KiB in, fold vectors converted on the device), compact responses "
+            f"({shape.packed_response_bytes(q2_bits)} B at q2_bits {q2_bits})" if packed else
+            "queries with GSW fold vectors, raw response ciphertexts")
     out = {"workload": "Poisson arrivals, 70/30 over resident (9,6)/(10,7) databases, resident client keys, "
-                       "continuous batching (<=128 per shape, 2 buffers per shape, graph replay per size class)",
-           "seconds_per_rate": seconds, "rates": results, "admission": srv.admission,
+                       f"{what}, continuous batching (<=128 per shape, 2 buffers per shape, graph replay per size "
+                       "class, uploads and response copies queued without host waits)",
+           "form": form, "seconds_per_rate": seconds, "rates": results, "admission": srv.admission,
            "batches_launched": {k: ln.launched_batches for k, ln in srv.lanes.items()},
            "setup_s": t_setup, "all_responses_exact_and_decoded": bool(exact)}
     srv.close()
@@ -376,6 +408,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-rates", default="200,1000,2000", help="offered queries/s, comma separated")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-form", choices=("packed", "vectors"), default="packed",
+                    help="serving requests: packed queries with compact responses, or fold-vector queries")
     ap.add_argument("--no-full-path", action="store_true",
                     help="skip the packed-query / compact-response leg (conversion and response packing)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -484,7 +518,7 @@ def main() -> int:
             group = None
         if not args.no_serve:
             rates = [float(x) for x in args.serve_rates.split(",") if x]
-            serving = run_serving(torch, mods, dbs, rates, args.serve_seconds)
+            serving = run_serving(torch, mods, dbs, rates, args.serve_seconds, args.serve_form)
             correct = correct and serving["all_responses_exact_and_decoded"]
         for name, (_, _, d) in dbs.items():
             if d is not db:
