@@ -224,7 +224,7 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
 }
 
 extern "C" int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant) {
-  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > (stage == SP_STAGE_DB_MUL ? 2 : 4))
+  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > 4)
     return fail(SP_ERR_ARG, "sp_ctx_set_variant: stage %d variant %d", stage, variant);
   ctx->variant[stage] = variant;
   return SP_OK;
@@ -765,7 +765,8 @@ struct sp_batch {
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
   uint32_t *mfma_out = nullptr;   // p-major products workspace
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
-  std::vector<std::pair<int, float>> dbmul_ms[3];  // measurements per variant, for reporting
+  std::vector<std::pair<int, float>> dbmul_ms[5];  // measurements per variant, for reporting
+  float tune_ms[2] = {-1, -1};                     // last tuning: multi-query VALU forms (2, 4 queries per read)
 };
 
 // Queries a batch keeps expansion snapshots for (the latency form runs only
@@ -1264,16 +1265,27 @@ static int mfma_workspace(sp_batch *b) {
   return SP_OK;
 }
 
-// Database-multiply variant for n queries: forced by the context, else the
-// autotuner's cached choice, else VALU (the autotuner runs in sp_batch_tune).
+// Whether the multi-query VALU multiply takes this shape (whole packed groups of 8 columns, 8+ rows).
+static bool multi_shape_ok(const sp_db *db) { return db->nu2 >= 3 && db->nu1 >= 3; }
+
+// Whether variant v of the database multiply can run n queries on this database.
+static bool dbmul_applies(const sp_db *db, int v, int n) {
+  if (v == 1) return true;
+  if (v == 2) return mfma_shape_ok(db);
+  return (v == 3 || v == 4) && n >= 2 && multi_shape_ok(db);
+}
+
+// Database-multiply variant for n queries: 1 VALU one query per database read,
+// 2 int8-limb MFMA, 3 / 4 VALU 2 / 4 queries per database read.  Forced by the
+// context where it applies, else the autotuner's cached choice, else the
+// multi-query VALU form for n >= 2 and the one-query form for n = 1.
 static int dbmul_variant(sp_batch *b, int n) {
   const int forced = b->db->ctx->variant[SP_STAGE_DB_MUL];
-  const bool mfma_ok = mfma_shape_ok(b->db);
-  if (forced == 2 && mfma_ok) return 2;
-  if (forced == 1 || !mfma_ok) return 1;
-  for (auto &c : b->dbmul_choice)
-    if (c.first == n) return c.second;
-  return 1;
+  if (forced && dbmul_applies(b->db, forced, n)) return forced;
+  if (!forced)
+    for (auto &c : b->dbmul_choice)
+      if (c.first == n) return c.second;
+  return dbmul_applies(b->db, 4, n) ? 4 : 1;
 }
 
 // Enqueues the stages for slots [0, n) on stream s (no synchronization).
@@ -1304,7 +1316,10 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
       rc = mfma_workspace(b);
       if (rc) return rc;
       HIP_TRY(launch_dbmul_mfma(b->v, b->v_q, n, db->d_limbs, db->n2, R, C, mfma_rows(n), b->mfma_a, b->mfma_out,
-                                b->acc, b->acc_q, ctx->d_pw, ctx->tables, mfma_rows(n) > 64 ? 4 : 0, s));
+                                b->acc, b->acc_q, ctx->d_pw, ctx->tables, mfma_rows(n) > 64 ? 4 : -1, s));
+    } else if (var == 3 || var == 4) {
+      HIP_TRY(launch_db_mul_multi(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, var - 2,
+                                  ctx->tables, s));
     } else {
       HIP_TRY(launch_db_mul(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, ctx->tables, s));
     }
@@ -1565,26 +1580,37 @@ extern "C" int sp_batch_tune(sp_batch *b, int n, float *ms_valu, float *ms_mfma,
   const int saved = ctx->variant[SP_STAGE_DB_MUL];
   float best = 0;
   int pick = 1;
-  float ms[3] = {-1, -1, -1};
+  float ms[5] = {-1, -1, -1, -1, -1};
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   int rc = SP_OK;
-  for (int var = 1; var <= 2 && !rc; var++) {
-    if (var == 2 && !mfma_shape_ok(b->db)) break;
+  // MFMA first (the usual winner for larger runs), then the multi-query and
+  // one-query VALU forms; a form whose first timed run is over twice the best
+  // so far is not repeated.
+  static const int order[4] = {2, 4, 3, 1};
+  for (int oi = 0; oi < 4 && !rc; oi++) {
+    const int var = order[oi];
+    if (!dbmul_applies(b->db, var, n)) continue;
     ctx->variant[SP_STAGE_DB_MUL] = var;
-    rc = batch_enqueue(b, n, 2, b->io);  // warm-up (builds limbs / workspaces)
-    const int reps = 3;
-    if (!rc && hipEventRecord(e0, b->io) != hipSuccess) rc = fail(SP_ERR_HIP, "event");
-    for (int k = 0; k < reps && !rc; k++) rc = batch_enqueue(b, n, 2, b->io);
-    if (!rc && hipEventRecord(e1, b->io) != hipSuccess) rc = fail(SP_ERR_HIP, "event");
-    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(SP_ERR_HIP, "event sync");
+    if (var == 2) rc = batch_enqueue(b, n, 2, b->io);  // warm-up: builds the limbs and workspaces
     float t = 0;
+    int reps = 0;
+    for (int k = 0; k < 3 && !rc; k++) {
+      if (hipEventRecord(e0, b->io) != hipSuccess) rc = fail(SP_ERR_HIP, "event");
+      if (!rc) rc = batch_enqueue(b, n, 2, b->io);
+      if (!rc && hipEventRecord(e1, b->io) != hipSuccess) rc = fail(SP_ERR_HIP, "event");
+      if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(SP_ERR_HIP, "event sync");
+      float dt = 0;
+      if (!rc) (void)hipEventElapsedTime(&dt, e0, e1);
+      t += dt;
+      reps++;
+      if (best > 0 && dt > 2 * best) break;
+    }
     if (!rc) {
-      (void)hipEventElapsedTime(&t, e0, e1);
       ms[var] = t / reps;
       b->dbmul_ms[var].push_back({n, ms[var]});
-      if (var == 1 || ms[var] < best) { best = ms[var]; pick = var; }
+      if (best == 0 || ms[var] < best) { best = ms[var]; pick = var; }
     }
   }
   ctx->variant[SP_STAGE_DB_MUL] = saved;
@@ -1598,6 +1624,26 @@ extern "C" int sp_batch_tune(sp_batch *b, int n, float *ms_valu, float *ms_mfma,
   if (ms_valu) *ms_valu = ms[1];
   if (ms_mfma) *ms_mfma = ms[2];
   if (chosen) *chosen = pick;
+  b->tune_ms[0] = ms[3];
+  b->tune_ms[1] = ms[4];
+  return SP_OK;
+}
+
+extern "C" int sp_batch_set_dbmul(sp_batch *b, int n, int variant) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_set_dbmul: NULL");
+  if (n < 1 || n > b->cap) return fail(SP_ERR_ARG, "sp_batch_set_dbmul: n = %d", n);
+  if (variant < 1 || variant > 4 || !dbmul_applies(b->db, variant, n))
+    return fail(SP_ERR_ARG, "sp_batch_set_dbmul: variant %d does not apply to %d queries of this database", variant, n);
+  for (auto &c : b->dbmul_choice)
+    if (c.first == n) { c.second = variant; return SP_OK; }
+  b->dbmul_choice.push_back({n, variant});
+  return SP_OK;
+}
+
+extern "C" int sp_batch_tune_multi_ms(const sp_batch *b, float *ms_q2, float *ms_q4) {
+  if (!b) return fail(SP_ERR_ARG, "sp_batch_tune_multi_ms: NULL");
+  if (ms_q2) *ms_q2 = b->tune_ms[0];
+  if (ms_q4) *ms_q4 = b->tune_ms[1];
   return SP_OK;
 }
 

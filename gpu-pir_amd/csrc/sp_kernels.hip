@@ -814,6 +814,146 @@ hipError_t launch_db_pack(const uint32_t *in, uint32_t *out, int n2, int rows, i
   return hipGetLastError();
 }
 
+// Database multiply of several queries per database read (VALU).  k_db_mul
+// reads the whole database once per query; this kernel reads it once per Q
+// queries: a lane owns PPL positions and the 8 columns of one packed group
+// (CG = 8 in W = 7 words) and keeps Q x 8 x PPL x 2 lazy u64 sums; the WPB
+// waves of a block split the rows (UNR rows per loop pass) and reduce through
+// LDS, one query at a time.  Block = (residue, block of 64 PPL positions,
+// trial, column group, query group); each XCD owns a contiguous range of
+// (residue, position block) groups so that the re-reads of the selectors stay
+// in its L2, and the query groups of one database tile are consecutive on an
+// XCD.  Needs C >= 8 and R >= WPB; queries of the last group past nq repeat
+// query nq - 1 and are not written.
+template <int Q, int PPL, int WPB, int UNR>
+__global__ __launch_bounds__(WPB * 64) void k_db_mul_q(const uint32_t *__restrict__ db,
+                                                       const uint32_t *__restrict__ sel_all, size_t sel_stride,
+                                                       size_t sel_qstride, uint32_t *__restrict__ out_all,
+                                                       size_t out_qstride, int nq, int T, int R, int C, ModConst mc0,
+                                                       ModConst mc1) {
+  constexpr int CG = 8, W = 7, PB = 64 * PPL, NPB = N / PB, SGX = 2 * NPB / 8;
+  typedef unsigned int vu __attribute__((ext_vector_type(PPL)));
+  __shared__ uint32_t red[WPB][CG][2][PB];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ncg = C / CG;
+  const int nqg = (nq + Q - 1) / Q;
+  const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+  const int qg = loc % nqg;
+  const int rest0 = loc / nqg;
+  const int sg = xcd * SGX + rest0 % SGX;
+  const int rest = rest0 / SGX;
+  const int tt = rest % T, cg = rest / T;
+  if (cg >= ncg) return;
+  const int m = sg / NPB, pb = sg % NPB;
+  const ModConst mc = m ? mc1 : mc0;
+  const int p0 = pb * PB + lane * PPL;
+  const uint32_t *sel[Q];
+#pragma unroll
+  for (int qi = 0; qi < Q; qi++)
+    sel[qi] = sel_all + (size_t)min(qg * Q + qi, nq - 1) * sel_qstride + (size_t)m * N + p0;
+  const int rows_per_wave = R / WPB;
+  const int j0 = wave * rows_per_wave;
+  uint64_t acc[Q][CG][PPL][2];
+#pragma unroll
+  for (int qi = 0; qi < Q; qi++)
+#pragma unroll
+    for (int c = 0; c < CG; c++)
+#pragma unroll
+      for (int pp = 0; pp < PPL; pp++) { acc[qi][c][pp][0] = 0; acc[qi][c][pp][1] = 0; }
+  // Products are < q^2 < 2^56; every 128 rows the sums are reduced mod q, so a
+  // block of rows starts below q and ends below q + 128 q^2 < 2^63.
+  const uint32_t *dbase = db + ((((size_t)tt * 2 + m) * R) * ncg + cg) * W * N + p0;
+  for (int jb = 0; jb < rows_per_wave; jb += 128) {
+    const int jend = min(rows_per_wave, jb + 128);
+#pragma unroll UNR
+    for (int jj = jb; jj < jend; jj++) {
+      const int j = j0 + jj;
+      const uint32_t *dp = dbase + (size_t)j * ncg * W * N;
+      uint32_t w[PPL][W];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        const vu d = __builtin_nontemporal_load(reinterpret_cast<const vu *>(dp + (size_t)k * N));
+#pragma unroll
+        for (int pp = 0; pp < PPL; pp++) w[pp][k] = d[pp];
+      }
+      uint32_t sv[Q][2][PPL];
+#pragma unroll
+      for (int qi = 0; qi < Q; qi++)
+#pragma unroll
+        for (int row = 0; row < 2; row++) {
+          const vu x = *reinterpret_cast<const vu *>(sel[qi] + (size_t)j * sel_stride + (size_t)row * 2 * N);
+#pragma unroll
+          for (int pp = 0; pp < PPL; pp++) sv[qi][row][pp] = x[pp];
+        }
+      static_for<CG>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+#pragma unroll
+        for (int pp = 0; pp < PPL; pp++) {
+          const uint32_t d = unpack28<c, W>(w[pp]);
+#pragma unroll
+          for (int qi = 0; qi < Q; qi++) {
+            acc[qi][c][pp][0] += (uint64_t)sv[qi][0][pp] * d;
+            acc[qi][c][pp][1] += (uint64_t)sv[qi][1][pp] * d;
+          }
+        }
+      });
+    }
+    if (jend < rows_per_wave) {
+#pragma unroll
+      for (int qi = 0; qi < Q; qi++)
+#pragma unroll
+        for (int c = 0; c < CG; c++)
+#pragma unroll
+          for (int pp = 0; pp < PPL; pp++) {
+            acc[qi][c][pp][0] = mod_u64(acc[qi][c][pp][0], mc);
+            acc[qi][c][pp][1] = mod_u64(acc[qi][c][pp][1], mc);
+          }
+    }
+  }
+#pragma unroll
+  for (int qi = 0; qi < Q; qi++) {
+    const int q = qg * Q + qi;
+    if (q >= nq) break;  // uniform over the block
+#pragma unroll
+    for (int c = 0; c < CG; c++)
+#pragma unroll
+      for (int row = 0; row < 2; row++)
+#pragma unroll
+        for (int pp = 0; pp < PPL; pp++) red[wave][c][row][lane * PPL + pp] = mod_u64(acc[qi][c][pp][row], mc);
+    __syncthreads();
+    uint32_t *out = out_all + (size_t)q * out_qstride;
+    for (int idx = threadIdx.x; idx < CG * 2 * PB; idx += WPB * 64) {
+      const int c = idx / (2 * PB), row = (idx / PB) & 1, pos = idx % PB;
+      uint32_t v = red[0][c][row][pos];
+#pragma unroll
+      for (int w = 1; w < WPB; w++) v = canon(v + red[w][c][row][pos], mc.q);
+      out[((((size_t)tt * C + cg * CG + c) * 2 + row) * 2 + m) * N + pb * PB + pos] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Launch of the multi-query multiply.  form 1: 2 queries per read, 2 positions
+// per lane; form 2: 4 queries per read, 1 position per lane.  4-wave blocks
+// (3 per CU at 161-166 VGPRs): at (10, 7), 2 queries 1.36 ms (one database
+// read at HBM speed is 1.17 ms) and 4 queries 1.85 ms, against 1.43 / 2.56 ms
+// in 8-wave blocks and 2.25 / 4.37 ms for one read per query; deeper unrolling
+// and half-group lanes (4 columns x 2 positions) were slower.
+hipError_t launch_db_mul_multi(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
+                               uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, int form,
+                               const DevTables &t, hipStream_t s) {
+  const int ncg = C / 8;
+#define SP_DBQ(QV, PPLV)                                                                                        \
+  k_db_mul_q<QV, PPLV, 4, 2><<<(unsigned)(2 * (N / (64 * PPLV)) * T * ncg * ((nq + QV - 1) / QV)), 256, 0, s>>>( \
+      db, sel, sel_stride, sel_qstride, out, out_qstride, nq, T, R, C, t.mc[0], t.mc[1])
+  if (form == 2)
+    SP_DBQ(4, 1);
+  else
+    SP_DBQ(2, 2);
+#undef SP_DBQ
+  return hipGetLastError();
+}
+
 hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
                          uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,
                          hipStream_t s) {
@@ -2647,6 +2787,80 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
   }
 }
 
+// Streaming form for small batches (I <= 64 selector rows, 32 queries): the
+// same products as k_dbmul_mfma, but a block holds only the waves that have
+// rows (RT row tiles of 32 x 2 column tiles of 32 = one 64-column group), and
+// each wave streams its operands straight from global memory through a ring
+// of PD register slots, PD row blocks ahead.  k_dbmul_mfma keeps 8-wave
+// blocks whose row waves past I return at once, so at I = 32 a CU streams
+// the database limbs with 2 waves and one block of loads in flight each
+// (about 2-3 TB/s); here 8 waves per CU keep PD blocks in flight each.
+// JBC: row blocks known at compile time (whole loop unrolled) or 0.
+template <int RT, int PD, int JBC>
+__global__ __launch_bounds__(RT * 128) void k_dbmul_mfma_stream(const uint8_t *__restrict__ A,
+                                                                const uint8_t *__restrict__ D,
+                                                                uint32_t *__restrict__ outp, int T, int R, int C,
+                                                                int I, ModConst mc0, ModConst mc1,
+                                                                const uint32_t *__restrict__ pw) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  int bid = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+  const int n_cg = (C + 63) / 64;
+  const int t = bid % T; bid /= T;
+  const int cg = bid % n_cg; bid /= n_cg;
+  const int p = bid % N;
+  const int m = bid / N;
+  const int row0 = (wave >> 1) * 32;
+  const int col0 = cg * 64 + (wave & 1) * 32;
+  if (row0 >= I || col0 >= C) return;
+  const int JB = JBC ? JBC : R / 32;
+  const int h = lane >> 5, l32 = lane & 31;
+  const uint8_t *a_base = A + ((((size_t)m * N + p) * JB) * 4 * I + row0 + l32) * 32 + 16 * h;
+  const uint8_t *d_base = D + (((((size_t)t * 2 + m) * N + p) * JB) * 4 * C + col0 + l32) * 32 + 16 * h;
+  const size_t a_limb = (size_t)I * 32, a_jb = 4 * a_limb;
+  const size_t d_limb = (size_t)C * 32, d_jb = 4 * d_limb;
+  v4i fa[PD][4], fd[PD][4];
+  auto load = [&](v4i (&xa)[4], v4i (&xd)[4], int jb) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      xa[a] = *reinterpret_cast<const v4i *>(a_base + jb * a_jb + a * a_limb);
+      xd[a] = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(d_base + jb * d_jb + a * d_limb));
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < PD; k++) load(fa[k], fd[k], min(k, JB - 1));
+  v16i acc[7];
+#pragma unroll
+  for (int w = 0; w < 7; w++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[w][v] = 0;
+  // PD divides JB (both powers of two, PD <= JB): no partial ring pass.
+#pragma unroll
+  for (int jb0 = 0; jb0 < JB; jb0 += PD) {
+#pragma unroll
+    for (int k = 0; k < PD; k++) {
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[k][a], fd[k][b], acc[a + b], 0, 0, 0);
+      load(fa[k], fd[k], min(jb0 + k + PD, JB - 1));
+      // keep the refill here: sunk toward its use it would leave too few loads in flight
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  const ModConst &mc = m ? mc1 : mc0;
+  const uint32_t *pwm = pw + m * 8;
+  const int c = col0 + l32;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
+    uint64_t sum = 0;
+#pragma unroll
+    for (int w = 0; w < 7; w++) sum += (uint64_t)(uint32_t)acc[w][v] * pwm[w];
+    outp[((((size_t)m * N + p) * T + t) * I + i) * C + c] = mod_u64(sum, mc);
+  }
+}
+
 // p-major products [m][p][t][I][C] -> per-query accumulators [b][t][c][row][m][P].
 __global__ __launch_bounds__(256) void k_outp_to_acc(const uint32_t *__restrict__ outp, int T, int I, int C, int nq,
                                                      uint32_t *__restrict__ acc, size_t acc_q) {
@@ -2703,7 +2917,28 @@ hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_
   const int JB = R / 32;  // ring depth: 4 j blocks ahead, or all of them when there are fewer (a power of two)
   if (ring == 0)
     k_dbmul_mfma<<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw);
-  else {
+  else if (ring < 0) {  // streaming form, I <= 64
+    const unsigned sblocks = (unsigned)(2 * N * n_cg * T);
+#define SP_STREAM(RTV, PDV, JBV)                                                                      \
+  k_dbmul_mfma_stream<RTV, PDV, JBV><<<sblocks, RTV * 128, 0, s>>>(A, D, outp, T, R, C, I, t.mc[0], t.mc[1], pw)
+#define SP_STREAM_JB(RTV)                  \
+  switch (JB) {                            \
+    case 1: SP_STREAM(RTV, 1, 1); break;   \
+    case 2: SP_STREAM(RTV, 2, 2); break;   \
+    case 4: SP_STREAM(RTV, 4, 4); break;   \
+    case 8: SP_STREAM(RTV, 4, 8); break;   \
+    case 16: SP_STREAM(RTV, 4, 16); break; \
+    case 32: SP_STREAM(RTV, 4, 32); break; \
+    default: SP_STREAM(RTV, 4, 0); break;  \
+  }
+    if (I <= 32) {
+      SP_STREAM_JB(1)
+    } else {
+      SP_STREAM_JB(2)
+    }
+#undef SP_STREAM_JB
+#undef SP_STREAM
+  } else {
 #define SP_RING(PDV, JBV) \
   k_dbmul_mfma_ring<PDV, JBV><<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw)
     switch (JB) {

@@ -50,6 +50,11 @@ hipError_t launch_db_pack(const uint32_t *in, uint32_t *out, int n2, int rows, i
 hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
                          uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,
                          hipStream_t s);
+// Several queries per database read (C >= 8, R >= 8): form 1 = 2 queries x 2
+// positions per lane, 2 = 4 queries x 1 position per lane.
+hipError_t launch_db_mul_multi(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
+                               uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, int form,
+                               const DevTables &t, hipStream_t s);
 hipError_t launch_fold_level(bool fast, uint32_t *accs, size_t set_stride, int n_trials, int n_queries,
                              int num_per, const uint32_t *cpos, const uint32_t *cneg, size_t key_query_stride,
                              int t_gsw, int bits, const DevTables &t, hipStream_t s);
@@ -130,8 +135,10 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
 //   A: workspace for selector limbs, 2 * N * R * 4 * I bytes (I = 2*nq padded to 32)
 //   outp: workspace, 2 * N * T * I * C words;  pw: [m][8] = 2^(7w) mod q_m
 hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *D, hipStream_t s);
-// ring: 0 = operands straight from global memory (small I: no padded rows are
-// multiplied), 4 = staged through LDS with a 4-deep prefetch ring (I > 64).
+// ring: 4 = staged through LDS with a 4-deep prefetch ring (I > 64); -1 =
+// streaming form for I <= 64 (blocks of the row waves only, a register
+// prefetch ring per wave); 0 = operands straight from global memory one block
+// ahead (the first small-batch form, kept for comparison).
 hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
                              uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
                              const DevTables &t, int ring, hipStream_t s);

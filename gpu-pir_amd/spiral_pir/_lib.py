@@ -77,6 +77,8 @@ SIGNATURES = {
     "sp_batch_synchronize": [P],
     "sp_batch_poll": [P, PI],
     "sp_batch_tune": [P, I, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), PI],
+    "sp_batch_tune_multi_ms": [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "sp_batch_set_dbmul": [P, I, I],
     "sp_batch_destroy": [P],
     "sp_keys_create": [P, I, I, P, P, PP],
     "sp_keys_destroy": [P],

@@ -24,6 +24,9 @@ VARIANT_EXPAND = 0
 VARIANT_FOLD = 1
 VARIANT_DB_MUL = 2
 
+# database-multiply variants (Context.set_variant(VARIANT_DB_MUL, v))
+DBMUL_NAMES = {0: None, 1: "valu", 2: "mfma", 3: "valu_q2", 4: "valu_q4"}
+
 READ_RESPONSE = 0
 READ_EXPANDED = 1
 READ_DB_MUL = 2
@@ -266,14 +269,22 @@ class Batch:
         the number of captured stage-chain graphs."""
         v, g = ctypes.c_int(), ctypes.c_int()
         _lib.call("sp_batch_last_run", self._h, ctypes.byref(v), ctypes.byref(g))
-        return {"dbmul": {0: None, 1: "valu", 2: "mfma"}[v.value], "graphs": g.value}
+        return {"dbmul": DBMUL_NAMES[v.value], "graphs": g.value}
 
     def tune(self, n: int) -> dict:
         """Autotune the database multiply for n queries (after one expansion has run)."""
         a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
         _lib.call("sp_batch_tune", self._h, n, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
-        return {"valu_ms": a.value, "mfma_ms": b.value if b.value >= 0 else None,
-                "chosen": {1: "valu", 2: "mfma"}[c.value]}
+        m2, m4 = ctypes.c_float(), ctypes.c_float()
+        _lib.call("sp_batch_tune_multi_ms", self._h, ctypes.byref(m2), ctypes.byref(m4))
+        opt = lambda x: x.value if x.value >= 0 else None  # noqa: E731
+        return {"valu_ms": a.value, "mfma_ms": opt(b), "valu_q2_ms": opt(m2), "valu_q4_ms": opt(m4),
+                "chosen": DBMUL_NAMES[c.value]}
+
+    def set_dbmul(self, n: int, variant) -> None:
+        """Database-multiply variant for later runs of n queries (a DBMUL_NAMES name or number)."""
+        v = variant if isinstance(variant, int) else {name: k for k, name in DBMUL_NAMES.items()}[variant]
+        _lib.call("sp_batch_set_dbmul", self._h, n, v)
 
     def load_many(self, first: int, queries, keys: "list[Keys]", v_folding, asynchronous: bool = False) -> None:
         """Slots [first, first + len(keys)): queries[i] ([2][2][2048] u32), resident keys[i],

@@ -68,7 +68,10 @@ int sp_ctx_synchronize(sp_ctx *ctx);
  *     launches (expansion: digits, then per-residue transforms and update;
  *     fold: partial products per residue and key-column half, then finish).
  *     Automatic: 4 for both
- *   SP_STAGE_DB_MUL: 1 = VALU u64 multiply-accumulate, 2 = int8-limb MFMA */
+ *   SP_STAGE_DB_MUL: 1 = VALU u64 multiply-accumulate (one query per database
+ *     read), 2 = int8-limb MFMA, 3 / 4 = VALU with 2 / 4 queries per database
+ *     read (runs of 2+ queries); a forced variant that does not apply to a run
+ *     falls back to the automatic choice */
 #define SP_STAGE_EXPAND 0
 #define SP_STAGE_FOLD 1
 #define SP_STAGE_DB_MUL 2
@@ -258,12 +261,21 @@ int sp_batch_read_responses(sp_batch *b, int n, uint32_t *host_out);
 int sp_batch_synchronize(sp_batch *b);
 /* *done = 1 when the batch's last run has finished, 0 while it runs (no wait). */
 int sp_batch_poll(sp_batch *b, int *done);
-/* Autotuner: times the database multiply of n queries with the VALU u64 and
- * the int8-limb MFMA kernels (the latter needs 2^nu1 and 2^nu2 >= 32), keeps
- * the faster for later runs of n queries and reports both times (ms; -1 when
- * a variant does not apply) and the choice (1 VALU, 2 MFMA).  Run it after the
- * batch's expansion has run once; it overwrites the accumulators. */
+/* Autotuner: times the database multiply of n queries with every variant
+ * that applies (SP_STAGE_DB_MUL: 1 VALU one query per database read; 2
+ * int8-limb MFMA, needs 2^nu1 and 2^nu2 >= 32; 3 / 4 VALU reading the
+ * database once for 2 / 4 queries, needs n >= 2 and 2^nu1, 2^nu2 >= 8), keeps
+ * the fastest for later runs of n queries and reports the times of variants 1
+ * and 2 (ms; -1 when a variant does not apply) and the choice.
+ * sp_batch_tune_multi_ms gives the last tuning's times of variants 3 and 4.
+ * Run it after the batch's expansion has run once; it overwrites the
+ * accumulators. */
 int sp_batch_tune(sp_batch *b, int n, float *ms_valu, float *ms_mfma, int *chosen);
+int sp_batch_tune_multi_ms(const sp_batch *b, float *ms_q2, float *ms_q4);
+/* Sets the database-multiply variant later runs of n queries use (as the
+ * autotuner would), e.g. a choice tuned on another batch of the same
+ * database; SP_ERR_ARG if the variant does not apply. */
+int sp_batch_set_dbmul(sp_batch *b, int n, int variant);
 int sp_batch_destroy(sp_batch *b);
 
 /* Client expansion keys resident on the device, for one database shape and

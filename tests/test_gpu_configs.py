@@ -5,7 +5,7 @@ oracle on the GPU:
   1024-row database multiply, a 7-level fold in the latency form) and the
   split fold form; the expansion with random keys, element by element;
 * 2^nu1 = 1024 rows through the int8-limb MFMA multiply (the 32-block ring
-  kernel and the global-memory kernel) and the VALU kernel, random and
+  kernel and the streaming kernel) and the VALU kernels, random and
   all-(q-1) databases; the ring kernels for 1, 2, 4 and 8 row blocks;
 * (8, 4), 16 queries: the VALU batched path (no MFMA at 2^nu2 = 16);
 * (9, 6), 64 queries: the autotuned graph batch, slots against the oracle;
@@ -205,8 +205,8 @@ def _dbmul_check(oracle, b, s, db_ntt, slots, what):
 
 def test_dbmul_nu1_10_mfma_ring_and_valu(gpu_ctx, oracle, dbs, force_dbmul):
     """(10, 5), 40 queries = 80 selector rows: the LDS-staged MFMA kernel with 32 row
-    blocks, then 2 queries through the global-memory MFMA kernel, and the VALU
-    kernel; accumulators of the first and last slot against the oracle."""
+    blocks, then 2 queries through the streaming MFMA kernel (I <= 64), and the
+    VALU kernel; accumulators of the first and last slot against the oracle."""
     from spiral_pir.server import STAGE_DB_MUL, STAGE_EXPAND
 
     s, recs, db, db_ntt = dbs.get(10, 5)
@@ -223,15 +223,15 @@ def test_dbmul_nu1_10_mfma_ring_and_valu(gpu_ctx, oracle, dbs, force_dbmul):
     b.run(2, STAGE_EXPAND)
     force_dbmul(2)
     b.run(2, STAGE_DB_MUL)
-    _dbmul_check(oracle, b, s, db_ntt, (0, 1), "mfma global")
+    _dbmul_check(oracle, b, s, db_ntt, (0, 1), "mfma streaming")
     b.close()
     dbs.drop(10, 5)
 
 
 def test_dbmul_nu1_10_extreme_residues(gpu_ctx, oracle, force_dbmul):
     """A (10, 5) database of all-(q-1) residues (every limb 127 over 1024 rows: the
-    int32 limb sums reach 4 x 1024 x 127^2) through the ring and global MFMA
-    kernels and the VALU kernel, against the oracle."""
+    int32 limb sums reach 4 x 1024 x 127^2) through the ring and streaming MFMA
+    kernels and the VALU kernels (one and four queries per read), against the oracle."""
     from spiral_pir import Shape
     from spiral_pir.server import Database, STAGE_DB_MUL, STAGE_EXPAND
 
@@ -241,7 +241,7 @@ def test_dbmul_nu1_10_extreme_residues(gpu_ctx, oracle, force_dbmul):
     db_ntt[:, :, :, 1] = Q1 - 1
     db = Database(gpu_ctx, s)
     db.load_ntt(db_ntt)
-    for B, variants in ((34, (2, 1)), (2, (2,))):
+    for B, variants in ((34, (2, 1, 4)), (2, (2, 3))):
         entries = _clients_queries(s, B, 2, 4200 + B)
         b = _batch(db, entries)
         b.run(B, STAGE_EXPAND)
@@ -284,17 +284,53 @@ def test_dbmul_mfma_ring_row_blocks(gpu_ctx, oracle, force_dbmul, nu1):
     db.close()
 
 
+@pytest.mark.parametrize("shape,B", [((9, 6), 5), ((10, 7), 3), ((3, 3), 6)])
+def test_dbmul_multi_query_valu(gpu_ctx, oracle, dbs, force_dbmul, shape, B):
+    """The VALU multiply that reads the database once for 2 (variant 3) or 4
+    (variant 4) queries, with the last query group partial: accumulators equal
+    the one-query kernel's for every slot and the oracle's for the first and
+    last slot; at (3, 3) one packed group of 8 columns and 8 rows (one per wave)."""
+    from spiral_pir import Shape
+    from spiral_pir.server import Database, READ_DB_MUL, STAGE_DB_MUL, STAGE_EXPAND
+
+    if shape == (3, 3):
+        s = Shape(3, 3)
+        recs = rng(4800).integers(0, 256, size=(s.records, s.n2, N), dtype=np.uint8)
+        db = Database(gpu_ctx, s)
+        db.load_bytes(recs)
+        db_ntt = oracle.db_encode(recs, s.n2, 3, 3)
+    else:
+        s, recs, db, db_ntt = dbs.get(*shape)
+    entries = _clients_queries(s, B, 3, 4900 + B)
+    b = _batch(db, entries)
+    b.run(B, STAGE_EXPAND)
+    force_dbmul(1)
+    b.run(B, STAGE_DB_MUL)
+    ref = [b.read(k, READ_DB_MUL) for k in range(B)]
+    for var, name in ((3, "valu_q2"), (4, "valu_q4")):
+        force_dbmul(var)
+        b.run(B, STAGE_DB_MUL)
+        assert b.last_run()["dbmul"] == name
+        for k in range(B):
+            np.testing.assert_array_equal(b.read(k, READ_DB_MUL), ref[k], err_msg=f"{name} slot {k}")
+    _dbmul_check(oracle, b, s, db_ntt, (0, B - 1), "valu_q4")
+    b.close()
+    if shape == (3, 3):
+        db.close()
+
+
 # ---- (8, 4): the VALU batched path ------------------------------------------------
 
 def test_batch_nu8_4_valu_16_queries(gpu_ctx, oracle, dbs):
     """(8, 4), 16 queries by graph replay: 2^nu2 = 16 leaves the VALU multiply as the
-    only batched form; every slot equals the oracle's answer and decodes."""
+    only batched form (here the one reading the database once per 4 queries);
+    every slot equals the oracle's answer and decodes."""
     s, recs, db, db_ntt = dbs.get(8, 4)
     entries = _clients_queries(s, 16, 4, 4500)
     b = _batch(db, entries)
     b.graph_launch(16)
     b.synchronize()
-    assert b.last_run()["dbmul"] == "valu"
+    assert b.last_run()["dbmul"] == "valu_q4"
     got = b.responses(16)
     for slot, (cl, idx, q, left, right, vf) in enumerate(entries):
         np.testing.assert_array_equal(got[slot], _oracle_answer(oracle, s, db_ntt, q, left, right, vf),

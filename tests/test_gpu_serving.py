@@ -244,7 +244,7 @@ def test_serving_warmup_and_size_classes(gpu_ctx):
             exp[(cid, k)] = (cl, idx, _alone(db, left, right, q, vf))
     cid0, _, _, qs0 = pool[0]
     srv.warmup("x", cid0, qs0[0][1], qs0[0][2])
-    assert {c for (_, c) in srv.lanes["x"].ready} == {1, 2, 4, 6}
+    assert {c for (_, c) in srv.lanes["x"].ready} == {1, 2, 3, 4, 5, 6}
     srv.start()
     for n in (1, 3, 6, 5, 2):
         batch = []

@@ -51,7 +51,10 @@ class FakeBatch:
 
     def tune(self, n):
         self.tuned.append(n)
-        return {}
+        return {"chosen": "valu"}
+
+    def set_dbmul(self, n, variant):
+        self.tuned.append(("set", n, variant))
 
     def graph_launch(self, n, stream=None):
         self.launches.append((n, len(self.loaded)))
@@ -86,7 +89,8 @@ def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
         ln.out = [np.zeros((capacity, 4), np.uint32) for _ in batches]
         ln.streams = [FakeStream() for _ in batches]
         ln.free = collections.deque(range(len(batches)))
-        ln.inflight, ln.pending, ln.ready = {}, collections.deque(), set()
+        ln.inflight, ln.pending, ln.ready, ln.choice = {}, collections.deque(), set(), {}
+        ln.max_inflight = len(batches)
         ln.keys = {"c": object(), "p": type("PackedKeys", (), {"packed": True})()}
         ln.launched_batches = ln.launched_queries = 0
         srv.lanes[name] = ln
@@ -109,8 +113,13 @@ def _q(v):
 def test_size_classes():
     from spiral_pir.serving import size_class
 
-    assert [size_class(n, 128) for n in (1, 2, 3, 5, 9, 33, 100, 128)] == [1, 2, 4, 8, 16, 64, 128, 128]
-    assert size_class(5, 6) == 6 and size_class(6, 6) == 6 and size_class(3, 6) == 4
+    assert [size_class(n, 128) for n in (1, 2, 3, 5, 9, 33, 100, 128)] == [1, 2, 3, 5, 10, 40, 112, 128]
+    assert size_class(5, 6) == 5 and size_class(6, 6) == 6 and size_class(7, 6) == 6
+    # classes past 8 are at most 1.25x apart, so a run wastes at most a fifth of its slots
+    from spiral_pir.serving import SIZE_CLASSES
+
+    big = [c for c in SIZE_CLASSES if c >= 8]
+    assert all(b <= 1.25 * a for a, b in zip(big, big[1:]))
 
 
 def test_slot_bytes_grow_with_shape():
@@ -252,3 +261,26 @@ def test_latency_stats_split_queue_and_device_time():
     st = latency_stats(reqs)
     assert abs(st["p50_ms"] - 3.0) < 1e-6 and abs(st["queue_ms"]["p50"] - 1.0) < 1e-6
     assert abs(st["device_ms"]["p99"] - 2.0) < 1e-6 and st["mean_batch"] == 2.0
+
+
+def test_tuning_is_shared_between_buffers():
+    """A size class is tuned once per lane; the other buffers get the same choice."""
+    a, b = FakeBatch(), FakeBatch()
+    srv = _server({"x": [a, b]}, capacity=4)
+    ln = srv.lanes["x"]
+    srv._prepare(ln, 0, 4)
+    srv._prepare(ln, 1, 4)
+    srv._prepare(ln, 1, 4)
+    assert a.tuned == [4] and b.tuned == [("set", 4, "valu")]
+
+
+def test_max_inflight_limits_concurrent_batches():
+    b0, b1 = FakeBatch(), FakeBatch()
+    srv = _server({"x": [b0, b1]}, capacity=2)
+    srv.lanes["x"].max_inflight = 1
+    ln = srv.lanes["x"]
+    for i in range(5):
+        ln.pending.append(__import__("spiral_pir.serving", fromlist=["Request"]).Request("x", "c", _q(i), np.zeros(2, np.uint32)))
+    while ln.free and ln.pending and len(ln.inflight) < ln.max_inflight:
+        srv._launch(ln)
+    assert len(ln.inflight) == 1 and len(ln.pending) == 3
