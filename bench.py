@@ -32,13 +32,41 @@ HBM_PEAK_GBPS = 8000.0
 DB_MUL_TRAFFIC_BYTES = {(9, 6): (1852411 + 8192) * 1024}
 
 
-def db_mul_algorithmic_bytes(shape) -> int:
-    """Resident DB (28-bit residues, packed) + selectors + accumulator outputs
-    (u32 words) for one query."""
-    db = shape.db_device_bytes
+def db_mul_algorithmic_bytes(shape, packed: bool = False) -> int:
+    """Bytes one query's database multiply must move: the database (u32 NTT
+    residues, 4 bytes each, the survey's definition; or, packed=True, the 28-bit
+    packed resident form this library reads) + selectors + accumulator outputs."""
+    db = shape.db_device_bytes if packed else shape.db_ntt_bytes
     sel = shape.rows * 2 * 2 * 2048 * 4
     out = shape.n2 * shape.cols * 2 * 2 * 2048 * 4
     return db + sel + out
+
+
+BUTTERFLIES_PER_TRANSFORM = 11 * 1024  # 2048-point radix-2 NTT
+# VALU issue rate of the part (256 CUs x 64 lanes x 2.4 GHz) over the 5 VALU
+# instructions of a lazy Harvey butterfly with a negated Shoup twiddle (one
+# v_mul_lo_u32, one v_mad_u64_u32, one v_add3_u32, one v_sub, one v_min): the
+# ceiling for transform-bound kernels, counting butterflies only (the key
+# multiply-accumulates of the same kernels come on top, uncounted).
+VALU_LANE_OPS = 256 * 64 * 2.4e9
+BUTTERFLY_PEAK = VALU_LANE_OPS / 5
+
+
+def expansion_steps(shape) -> int:
+    """Working steps of one query's expansion (gpu-pir/expansion.cuh:87-97):
+    every element below stop_round, then the even elements, and in round
+    stop_round the odd ones of the t_gsw nu2 GSW slots."""
+    g, st, mb = shape.g, shape.stop_round, shape.t_gsw * shape.nu2
+    return sum((1 << (r + 1)) if r < st else (1 << r) + (min(1 << r, mb) if r == st else 0) for r in range(g))
+
+
+def transforms_per_query(shape) -> dict:
+    """2048-point transforms of one answer: expansion 18 per step (2 inverse of
+    v_i.a, 2 x 8 digit transforms; tau(as_e) is a permutation of NTT positions);
+    from_ntt 2 x 2 per accumulator ciphertext; fold 36 per pair (2 x 16 digits
+    of X - Y in the G - C form, 4 inverse of the product)."""
+    pairs = shape.n2 * (shape.cols - 1)
+    return {"expand": 18 * expansion_steps(shape), "from_ntt": 4 * shape.n2 * shape.cols, "fold": 36 * pairs}
 
 
 def _cpu_query_seconds(O, shape, q, left, right, vf, sample_cols: int, threads: int) -> dict:
@@ -143,6 +171,33 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
     launch()  # first graph launch captures the graph
     launch()
     torch.cuda.synchronize()
+    # stage times of one eager run (HIP events on the launch stream), for the stage rooflines
+    st = {"expand": [], "db_mul": [], "fold": []}
+    for _ in range(3):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record(stream)
+        batch.run(B, 1, sptr)
+        e[1].record(stream)
+        batch.run(B, 2, sptr)
+        e[2].record(stream)
+        batch.run(B, 4, sptr)
+        e[3].record(stream)
+        torch.cuda.synchronize()
+        for k, (a, b) in zip(st, ((0, 1), (1, 2), (2, 3))):
+            st[k].append(e[a].elapsed_time(e[b]))
+    stage_ms = {k: float(np.median(v)) for k, v in st.items()}
+    tr = transforms_per_query(shape)
+    rooflines = {}
+    for stage, n_tr in (("expand", tr["expand"]), ("fold", tr["fold"] + tr["from_ntt"])):
+        a = B * n_tr * BUTTERFLIES_PER_TRANSFORM / (stage_ms[stage] * 1e-3)
+        rooflines[stage] = {"bound": "valu", "achieved": a / 1e12, "peak": BUTTERFLY_PEAK / 1e12,
+                            "unit": "T butterflies/s", "frac": a / BUTTERFLY_PEAK,
+                            "work": f"{n_tr} transforms x {BUTTERFLIES_PER_TRANSFORM} butterflies per query"}
+    macs = B * shape.rows * shape.cols * shape.n2 * 2048 * 2 * 2
+    ops = macs * 32 / (stage_ms["db_mul"] * 1e-3)
+    rooflines["db_mul"] = {"bound": "mfma" if tuned["chosen"] == "mfma" else "valu", "kernel": tuned["chosen"],
+                           "achieved": ops / 1e12, "peak": 5000.0, "unit": "int8 TOP/s (16 limb products per MAC)",
+                           "frac": ops / 5e15}
     resp = batch.responses(B)
     bad = [s for s in range(B) if not np.array_equal(clients[s % n_clients].decode(resp[s]), records[idx[s]])]
     ok = not bad
@@ -157,6 +212,7 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
     batch.close()
     return {"B": B, "ms_per_batch": ms, "queries_per_s": B * 1000.0 / ms, "steps": steps,
             "launch": "eager (one launch per kernel)" if eager else "hipGraph replay of the whole stage chain",
+            "stage_ms_eager": stage_ms, "stage_roofline": rooflines,
             "db_mul_autotune": tuned, "decoded_all_correct": ok, "slots_decoded_wrong": bad}
 
 
@@ -526,6 +582,8 @@ def main() -> int:
 
     alg_bytes = db_mul_algorithmic_bytes(shape)
     achieved = alg_bytes / (dbmul_ms * 1e-3) / 1e9
+    packed_bytes = db_mul_algorithmic_bytes(shape, packed=True)
+    achieved_packed = packed_bytes / (dbmul_ms * 1e-3) / 1e9
     # effective scan rate: the database's NTT form as u32 words (2^31 B at (9, 6))
     db_scan_gbps = shape.db_ntt_bytes / (dbmul_ms * 1e-3) / 1e9
     line = {
@@ -563,6 +621,12 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": DB_MUL_TRAFFIC_BYTES.get((shape.nu1, shape.nu2)),
             "traffic_unit": "bytes per launch",
+            "algorithmic_bytes": alg_bytes,
+            "definition": "database as u32 NTT residues (2^(nu1+nu2) n^2 2 2048 4 B) + selectors + outputs, "
+                          "per launch, over the launch's HIP-event time",
+            "packed_layout": {"bytes": packed_bytes, "achieved": achieved_packed, "frac": achieved_packed / HBM_PEAK_GBPS,
+                              "definition": "the bytes this kernel reads: the resident database packed 8 residues "
+                                            "in 7 words + selectors + outputs (what the traffic counters measure)"},
         },
         "batched": None if batched is None else {k: v for k, v in batched.items() if k != "decoded_all_correct"},
         "group": group,

@@ -413,16 +413,17 @@ def poisson_times(rate: float, duration: float, rng: np.random.Generator) -> np.
 
 def drive(server: Server, arrivals: list[tuple[float, str, object, np.ndarray, np.ndarray, object]]) -> list[Request]:
     """Submit (offset_s, lane, client, query, v_folding, tag) at their offsets from now
-    (sleeping, then spinning for the last 200 us) and return the requests."""
+    and return the requests.  The wait is a sleep, never a spin: a spinning
+    Python thread holds the interpreter lock, and the scheduler thread would
+    then wait up to a switch interval (5 ms) after each library call to run
+    again.  Requests are submitted at most ~0.1 ms late (sleep granularity);
+    their latency is counted from the actual submission."""
     t0 = time.perf_counter()
     out = []
     for off, lane, client, q, vf, tag in arrivals:
-        while True:
-            dt = t0 + off - time.perf_counter()
-            if dt <= 0:
-                break
-            if dt > 2e-4:
-                time.sleep(dt - 2e-4)
+        dt = t0 + off - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
         out.append(server.submit(lane, client, q, vf, tag))
     return out
 
