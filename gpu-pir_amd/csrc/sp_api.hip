@@ -1245,6 +1245,16 @@ extern "C" int sp_batch_load_packed_async(sp_batch *b, int first, int n, const u
   return batch_load_packed(b, first, n, host_queries, keys, true);
 }
 
+// Makes stream s wait on the device for the batch's uploads, unless they are
+// known to be done (a wait on a completed event still costs the stream a
+// barrier packet, a few microseconds per run).
+static hipError_t wait_loaded(sp_batch *b, hipStream_t s) {
+  const hipError_t q = hipEventQuery(b->loaded);
+  if (q == hipSuccess) return hipSuccess;
+  if (q == hipErrorNotReady) (void)hipGetLastError();  // a pending event is not an error for later launch checks
+  return hipStreamWaitEvent(s, b->loaded, 0);
+}
+
 // Whether any of slots [0, n) holds a packed query (the run then converts).
 static int slots_packed(const sp_batch *b, int n) {
   for (int i = 0; i < n; i++)
@@ -1363,7 +1373,7 @@ extern "C" int sp_batch_run(sp_batch *b, int n, int stages, void *stream) {
   if (b->fold_mode < 0) return fail(SP_ERR_STATE, "sp_batch_run: nothing loaded");
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  HIP_TRY(hipStreamWaitEvent(s, b->loaded, 0));  // the slots' uploads, on the device
+  HIP_TRY(wait_loaded(b, s));  // the slots' uploads, on the device
   int rc = batch_enqueue(b, n, stages, s);
   if (rc) return rc;
   if (stages & 4) b->pack_q2_run = b->pack_q2;
@@ -1413,7 +1423,7 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
     b->graph_defs.push_back(graph);
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  HIP_TRY(hipStreamWaitEvent(s, b->loaded, 0));  // the slots' uploads, on the device
+  HIP_TRY(wait_loaded(b, s));  // the slots' uploads, on the device
   HIP_TRY(hipGraphLaunch(exec, s));
   b->last_dbmul = gkey.dbmul;
   b->pack_q2_run = gkey.pack_q2;
@@ -1548,7 +1558,7 @@ extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   for (int i = 0; i < count; i++)
-    if (key[i] > 0) HIP_TRY(hipStreamWaitEvent(s, g->b[i]->loaded, 0));  // uploads of every active batch
+    if (key[i] > 0) HIP_TRY(wait_loaded(g->b[i], s));  // uploads of every active batch
   HIP_TRY(hipGraphLaunch(exec, s));
   HIP_TRY(hipEventRecord(g->done, s));
   for (int i = 0; i < count; i++)
@@ -1745,7 +1755,7 @@ extern "C" int sp_batch_poll(sp_batch *b, int *done) {
   HIP_TRY(hipSetDevice(b->db->ctx->device));
   const hipError_t e = hipEventQuery(b->done);
   if (e == hipSuccess) { *done = 1; return SP_OK; }
-  if (e == hipErrorNotReady) { *done = 0; return SP_OK; }
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); *done = 0; return SP_OK; }
   return fail(SP_ERR_HIP, "sp_batch_poll: %s", hipGetErrorString(e));
 }
 

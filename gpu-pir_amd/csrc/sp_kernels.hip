@@ -2794,7 +2794,11 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
 // of PD register slots, PD row blocks ahead.  k_dbmul_mfma keeps 8-wave
 // blocks whose row waves past I return at once, so at I = 32 a CU streams
 // the database limbs with 2 waves and one block of loads in flight each
-// (about 2-3 TB/s); here 8 waves per CU keep PD blocks in flight each.
+// (about 2-3 TB/s); here 8 waves per CU keep PD blocks in flight each and
+// reach 4.4-4.6 TB/s.  Ring depths 2, 4 and 8 measured the same (within 2%)
+// and plain loads of the database limbs were 2-5% faster than non-temporal
+// ones at (10, 7), so the bound is the 8 waves per CU that 112 accumulator
+// registers per wave allow, not the latency each wave covers.
 // JBC: row blocks known at compile time (whole loop unrolled) or 0.
 template <int RT, int PD, int JBC>
 __global__ __launch_bounds__(RT * 128) void k_dbmul_mfma_stream(const uint8_t *__restrict__ A,
@@ -2823,7 +2827,7 @@ __global__ __launch_bounds__(RT * 128) void k_dbmul_mfma_stream(const uint8_t *_
 #pragma unroll
     for (int a = 0; a < 4; a++) {
       xa[a] = *reinterpret_cast<const v4i *>(a_base + jb * a_jb + a * a_limb);
-      xd[a] = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(d_base + jb * d_jb + a * d_limb));
+      xd[a] = *reinterpret_cast<const v4i *>(d_base + jb * d_jb + a * d_limb);
     }
   };
 #pragma unroll

@@ -30,6 +30,7 @@ entry points.
 from __future__ import annotations
 
 import collections
+import gc
 import threading
 import time
 
@@ -253,8 +254,15 @@ class Server:
 
     # ---- the loop -------------------------------------------------------------
     def start(self) -> None:
+        """Start the scheduler thread.  Everything allocated so far (torch, the
+        databases, the keys, the warm-up) is moved out of the garbage
+        collector's reach first (gc.freeze): a full collection over a process
+        with torch loaded walks about a million objects and stopped the
+        scheduler for 40-55 ms at a time, which showed as the latency tail."""
         if self._thread is not None:
             raise RuntimeError("server already running")
+        gc.collect()
+        gc.freeze()
         self._stop = False
         self._thread = threading.Thread(target=self._run, name="pir-serving", daemon=True)
         self._thread.start()
