@@ -97,6 +97,7 @@ def _server(batches_per_lane: dict[str, list[FakeBatch]], capacity: int = 4):
     srv.admission = {}
     srv.poll_interval = 1e-4
     srv.pack_q2_bits = 0
+    srv.max_inflight_total = None
     srv._queue = collections.deque()
     srv._lock = threading.Lock()
     srv._wake = threading.Event()
@@ -284,3 +285,25 @@ def test_max_inflight_limits_concurrent_batches():
     while ln.free and ln.pending and len(ln.inflight) < ln.max_inflight:
         srv._launch(ln)
     assert len(ln.inflight) == 1 and len(ln.pending) == 3
+
+
+def test_global_limit_runs_oldest_lane_first():
+    """With one batch on the GPU over all lanes, the lane holding the oldest
+    waiting request launches first."""
+    from spiral_pir.serving import Request
+
+    a, b = FakeBatch(), FakeBatch()
+    srv = _server({"a": [a], "b": [b]}, capacity=4)
+    srv.max_inflight_total = 1
+    old = Request("b", "c", _q(1), np.zeros(2, np.uint32))
+    old.t_submit = 1.0
+    new = Request("a", "c", _q(2), np.zeros(2, np.uint32))
+    new.t_submit = 2.0
+    srv.lanes["a"].pending.append(new)
+    srv.lanes["b"].pending.append(old)
+    srv._stop = True
+    srv._thread = None
+    srv.start()
+    srv.stop()
+    assert b.launches and a.launches
+    assert old.t_launch <= new.t_launch
