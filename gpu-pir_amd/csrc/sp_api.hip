@@ -88,6 +88,8 @@ static ModConst make_modconst(int m) {
   c.two32 = (uint32_t)((1ull << 32) % q);
   c.two32_p = shoup_companion(c.two32, q);
   c.mu1 = (uint32_t)((1ull << 32) / q);
+  c.crt_c32 = (uint32_t)((uint64_t)c.crt_c * c.two32 % q);
+  c.crt_c32p = shoup_companion(c.crt_c32, q);
   return c;
 }
 
@@ -743,6 +745,7 @@ struct sp_batch {
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   std::vector<hipGraph_t> graph_defs;
   int last_dbmul = 0;             // database-multiply variant of the last run (1 VALU, 2 MFMA)
+  int pm_n = 0;                   // queries whose products wait in mfma_out (0: they are in acc)
   uint32_t *bulk = nullptr;       // bulk-load staging: [cap][4N] queries then [cap][f_q] v_folding
   uint32_t *dig = nullptr;        // split-expansion digit scratch [cap][dig_q]
   uint32_t *snap = nullptr;       // latency-form expansion snapshots, 2 sets x snap_q x SNAP_QUERIES
@@ -763,7 +766,7 @@ struct sp_batch {
   uint8_t *pout = nullptr;        // packed responses [cap][bytes(q2)]: sized once for q2 = 32, never moved
   uint32_t *rstage = nullptr;     // asynchronous raw readout: responses gathered [cap][n2][4N]
   uint8_t *mfma_a = nullptr;      // selector limbs workspace (MFMA multiply)
-  uint32_t *mfma_out = nullptr;   // p-major products workspace
+  uint32_t *mfma_out = nullptr;   // products workspace [m][t][I][N/4][C][4]
   std::vector<std::pair<int, int>> dbmul_choice;  // n -> variant picked by the autotuner
   std::vector<std::pair<int, float>> dbmul_ms[5];  // measurements per variant, for reporting
   float tune_ms[2] = {-1, -1};                     // last tuning: multi-query VALU forms (2, 4 queries per read)
@@ -1298,6 +1301,22 @@ static int dbmul_variant(sp_batch *b, int n) {
   return dbmul_applies(b->db, 4, n) ? 4 : 1;
 }
 
+// Columns per block of the from_ntt that reads the MFMA products (1, 2 or 4).
+#ifndef SP_PM_NC
+#define SP_PM_NC 1
+#endif
+
+// Products left in mfma_out by the MFMA multiply -> the NTT accumulators (what the
+// VALU variants write), for runs and readouts that need them there.
+static hipError_t pm_to_acc(sp_batch *b, hipStream_t s) {
+  if (b->pm_n == 0) return hipSuccess;
+  const sp_db *db = b->db;
+  const hipError_t e = launch_outp_to_acc(b->mfma_out, db->n2, mfma_rows(b->pm_n), 1 << db->nu2, b->pm_n, b->acc,
+                                          b->acc_q, db->ctx->tables, s);
+  b->pm_n = 0;
+  return e;
+}
+
 // Enqueues the stages for slots [0, n) on stream s (no synchronization).
 static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
   sp_db *db = b->db;
@@ -1325,17 +1344,29 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
       if (rc) return rc;
       rc = mfma_workspace(b);
       if (rc) return rc;
+      // The products stay in mfma_out: stage 4 takes them out of the NTT domain
+      // straight from there (or a readout of the accumulators transposes them).
       HIP_TRY(launch_dbmul_mfma(b->v, b->v_q, n, db->d_limbs, db->n2, R, C, mfma_rows(n), b->mfma_a, b->mfma_out,
-                                b->acc, b->acc_q, ctx->d_pw, ctx->tables, mfma_rows(n) > 64 ? 4 : -1, s));
+                                nullptr, b->acc_q, ctx->d_pw, ctx->tables, mfma_rows(n) > 64 ? 4 : -1, s));
+      b->pm_n = n;
     } else if (var == 3 || var == 4) {
       HIP_TRY(launch_db_mul_multi(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, var - 2,
                                   ctx->tables, s));
+      b->pm_n = 0;
     } else {
       HIP_TRY(launch_db_mul(db->d_db, b->v, 2 * 4 * N, b->v_q, b->acc, b->acc_q, n, db->n2, R, C, ctx->tables, s));
+      b->pm_n = 0;
     }
   }
   if (stages & 4) {
-    HIP_TRY(launch_from_ntt(true, b->acc, b->acc, (size_t)n * db->n2 * C * 2, 2 * N, 2 * N, ctx->tables, s));
+    if (b->pm_n >= n) {
+      HIP_TRY(launch_from_ntt_pm(b->mfma_out, db->n2, mfma_rows(b->pm_n), C, n, b->acc, b->acc_q, SP_PM_NC,
+                                 ctx->tables, s));
+    } else {
+      HIP_TRY(pm_to_acc(b, s));
+      HIP_TRY(launch_from_ntt(true, b->acc, b->acc, (size_t)n * db->n2 * C * 2, 2 * N, 2 * N, ctx->tables, s));
+    }
+    b->pm_n = 0;
     const bool fast = b->fold_mode != 0;
     const int mx = 2 * b->t_gsw;
     const size_t level = (size_t)2 * mx * 2 * N;
@@ -1426,6 +1457,7 @@ extern "C" int sp_batch_graph_launch(sp_batch *b, int n, void *stream) {
   HIP_TRY(wait_loaded(b, s));  // the slots' uploads, on the device
   HIP_TRY(hipGraphLaunch(exec, s));
   b->last_dbmul = gkey.dbmul;
+  b->pm_n = 0;  // the replay ran every stage
   b->pack_q2_run = gkey.pack_q2;
   HIP_TRY(hipEventRecord(b->done, s));
   return SP_OK;
@@ -1565,6 +1597,7 @@ extern "C" int sp_group_launch(sp_group *g, const int *ns, void *stream) {
     if (key[i] > 0) {
       sp_batch *b = g->b[i];
       b->last_dbmul = dbmul_variant(b, key[i]);
+      b->pm_n = 0;  // the replay ran every stage
       b->pack_q2_run = b->pack_q2;
       HIP_TRY(hipEventRecord(b->done, s));
     }
@@ -1678,6 +1711,7 @@ static int batch_read(sp_batch *b, int slot, int what, uint32_t *host_out) {
     HIP_TRY(hipMemcpyAsync(host_out, b->stage, n * N * 4, hipMemcpyDeviceToHost, b->io));
   } else if (what == 2) {
     const size_t n = (size_t)db->n2 * C * 4;
+    HIP_TRY(pm_to_acc(b, b->io));
     HIP_TRY(launch_p_to_nat(acc, b->stage, n, b->io));
     HIP_TRY(hipMemcpyAsync(host_out, b->stage, n * N * 4, hipMemcpyDeviceToHost, b->io));
   } else if (what == 3) {

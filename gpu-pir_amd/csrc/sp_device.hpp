@@ -59,6 +59,8 @@ struct ModConst {
   uint32_t two32;       // 2^32 mod q
   uint32_t two32_p;     // Shoup companion of two32
   uint32_t mu1;         // floor(2^32 / q)  (Shoup companion of 1)
+  uint32_t crt_c32;     // crt_c * 2^32 mod q (CRT constant for Montgomery-form inputs)
+  uint32_t crt_c32p;    // Shoup companion of crt_c32
 };
 
 __device__ __forceinline__ uint32_t shoup_mul(uint32_t y, uint32_t w, uint32_t wp, uint32_t q) {

@@ -2488,20 +2488,69 @@ hipError_t launch_expand8(uint32_t *v, size_t q_stride, int n_queries, int r, in
 // Residues (< 2^28) split into four 7-bit limbs, so with A = sum_a 2^7a A_a
 // and D = sum_b 2^7b D_b,  O = sum_w 2^7w P_w,  P_w = sum_{a+b=w} A_a D_b:
 // 16 exact int8 products accumulated in 7 int32 weights (each below
-// 4 * 1024 * 127^2 < 2^31 for up to 1024 rows), combined mod q at the end.
+// 4 * 1024 * 127^2 < 2^31 for up to 1024 rows), combined mod q at the end
+// (limb_combine, which leaves O * 2^-32 mod q).
 // Operand layouts (bytes, K = j contiguous in blocks of 32):
 //   A limbs  [m][p][jb][limb][I][32]      I = padded 2*batch rows
 //   D limbs  [t][m][p][jb][limb][C][32]
-// Output (u32, p-major)  [m][p][t][I][C], transposed afterwards into the
-// per-query accumulator layout [b][t][c][row][m][P].
+// Output (u32)  [m][t][I][N/4][C][4] (outp_word), read by k_from_ntt_pm (or
+// transposed into the per-query accumulator layout [b][t][c][row][m][P] by
+// k_outp_to_acc for readouts).
 // A lane of the 32x32x32 instruction holds 16 K-bytes of one row of A (and
 // of one column of B); both operands are read with the same byte->K mapping,
 // so the K order inside the instruction does not matter.
 // ===========================================================================
 namespace sp {
 
+// Row tiles per block of the LDS-staged kernel (4: 8-wave blocks of 128 x 64
+// outputs, 2: 4-wave blocks of 64 x 64) and its prefetch ring depth.
+#ifndef SP_RING_RB
+#define SP_RING_RB 4
+#endif
+#ifndef SP_RING_PD
+#define SP_RING_PD 4
+#endif
+// Row tiles per block of the persistent kernel (2 or 4; 0: the one-tile-per-block
+// kernel), its ring depth, and the limbs multiplied before its staging step.
+#ifndef SP_PERS_RB
+#define SP_PERS_RB 4
+#endif
+#ifndef SP_PERS_PD
+#define SP_PERS_PD 2
+#endif
+#ifndef SP_PERS_SPLIT
+#define SP_PERS_SPLIT 4
+#endif
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Word of product (residue m, position p, trial t, row i, column c) in the
+// multiply's output [m][t][I][N/4][C][4]: four consecutive positions of one
+// column are 16 contiguous bytes, so a from_ntt wave loads its polynomial in
+// layout-P chunks, and the epilogue's stores of the four position blocks of a
+// 128-byte line (which run together on one XCD) complete it in L2.
+__device__ __forceinline__ size_t outp_word(int m, int p, int t, int i, int c, int T, int I, int C) {
+  return (((((size_t)m * T + t) * I + i) * (N / 4) + (p >> 2)) * C + c) * 4 + (p & 3);
+}
+
+// Product O = sum_w 2^(7w) P_w mod q from the 7 limb-weight sums (each below
+// 2^31), times 2^-32: the Montgomery form, which k_from_ntt_pm undoes in its
+// CRT constant and k_outp_to_acc with a multiplication by 2^32 mod q.
+// 2^(7w) < q for w < 4, so those terms are exact shifts; the three others
+// take their weight mod q (pwm = 2^(7w) mod q).  The sum stays below
+// 2^53 + 3 * 2^59 < 2^61 and its REDC below 2^29 + q < 4q.
+__device__ __forceinline__ uint32_t limb_combine(const v16i (&acc)[7], int v, const uint32_t *__restrict__ pwm,
+                                                 const ModConst &mc) {
+  uint64_t s = (uint64_t)(uint32_t)acc[0][v] + ((uint64_t)(uint32_t)acc[1][v] << 7) +
+               ((uint64_t)(uint32_t)acc[2][v] << 14) + ((uint64_t)(uint32_t)acc[3][v] << 21);
+  s += (uint64_t)(uint32_t)acc[4][v] * pwm[4] + (uint64_t)(uint32_t)acc[5][v] * pwm[5] +
+       (uint64_t)(uint32_t)acc[6][v] * pwm[6];
+  const uint32_t mm = (uint32_t)s * mc.qinv_neg;
+  uint32_t r = (uint32_t)((s + (uint64_t)mm * mc.q) >> 32);
+  r = min(r, r - 2 * mc.q);
+  return min(r, r - mc.q);
+}
 
 // Selector limbs from the expanded ciphertexts of a batch.  Block: 4 rows i x
 // 32 j x 64 positions; 16-byte loads of 4 positions into an LDS tile (rows of
@@ -2649,10 +2698,7 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma(const uint8_t *__restrict__ 
 #pragma unroll
   for (int v = 0; v < 16; v++) {
     const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
-    uint64_t s = 0;
-#pragma unroll
-    for (int w = 0; w < 7; w++) s += (uint64_t)(uint32_t)acc[w][v] * pwm[w];
-    outp[((((size_t)m * N + p) * T + t) * I + i) * C + c] = mod_u64(s, mc);
+    outp[outp_word(m, p, t, i, c, T, I, C)] = limb_combine(acc, v, pwm, mc);
   }
 }
 
@@ -2673,16 +2719,26 @@ struct MfmaPiece {
   uint4 a0, a1, d;
 };
 
+// LDS slot of operand piece idx (16-byte pieces, two per 32-byte row-limb
+// chunk): bit 0 flips with bit 3 ^ bit 4.  A fragment read (lane l32 of half h
+// reads piece 2 (row0 + l32) + h) then touches 8 (16) distinct 16-byte bank
+// groups in every 8 (16) lanes, and the staging writes (piece = thread) stay
+// contiguous per 8 lanes: without it lanes l32 and l32 + 4 of a fragment read
+// hit the same banks.
+__device__ __forceinline__ int mfma_slot(int idx) { return idx ^ (((idx >> 3) ^ (idx >> 4)) & 1); }
+
 // JBC: the number of j blocks when known at compile time (the loop is then
 // unrolled whole, so that the waits for the ring's loads count exactly the
 // loads issued after them; 0 = R / 32 at run time).
-template <int PD, int JBC>
-__global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
-                                                         uint32_t *__restrict__ outp, int T, int R, int C, int I,
-                                                         int n_ig, int n_cg, ModConst mc0, ModConst mc1,
-                                                         const uint32_t *__restrict__ pw) {
-  __shared__ uint4 sA[3][4 * 128 * 2];
-  __shared__ uint4 sD[3][4 * 64 * 2];
+template <int PD, int JBC, int RB>
+__global__ __launch_bounds__(RB * 128) void k_dbmul_mfma_ring(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
+                                                              uint32_t *__restrict__ outp, int T, int R, int C, int I,
+                                                              int n_ig, int n_cg, ModConst mc0, ModConst mc1,
+                                                              const uint32_t *__restrict__ pw) {
+  constexpr int NT = RB * 128, AP = RB * 32 * 4 * 2;  // threads; 16-byte A pieces per j block
+  constexpr int DPT = 512 / NT;                        // D pieces (64 columns x 4 limbs x 2) per thread
+  __shared__ uint4 sA[3][AP];
+  __shared__ uint4 sD[3][512];
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   int bid = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
@@ -2692,46 +2748,62 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
   const int p = bid % N;
   const int m = bid / N;
   const int JB = JBC ? JBC : R / 32;
-  const int ar = (tid >> 1) & 127, al = tid >> 8, hh = tid & 1;
+  // A pieces tid and tid + NT: (limb al and al + 2, row ar, half hh); D pieces
+  // tid + k NT: (limb dl + 2k (RB = 2) or dl, column dc, half hh).  Rows past I
+  // and columns past C load row / column 0 (I and C are multiples of 32).
+  const int ar = (tid >> 1) & (RB * 32 - 1), al = tid / (RB * 64), hh = tid & 1;
   const int dl = tid >> 7, dc = (tid >> 1) & 63;
-  const bool a_ok = ig * 128 + ar < I, d_ok = cg * 64 + dc < C;
-  const uint32_t ma = a_ok ? ~0u : 0u, md = d_ok ? ~0u : 0u;
+  const bool a_ok = ig * RB * 32 + ar < I, d_ok = cg * 64 + dc < C;
   const size_t a_jb = (size_t)4 * I * 32, d_jb = (size_t)4 * C * 32;
-  const uint8_t *a0 = A + (((size_t)m * N + p) * JB * 4 * I + (size_t)al * I + (a_ok ? ig * 128 + ar : 0)) * 32 + 16 * hh;
+  const uint8_t *a0 = A + (((size_t)m * N + p) * JB * 4 * I + (size_t)al * I + (a_ok ? ig * RB * 32 + ar : 0)) * 32 +
+                      16 * hh;
   const uint8_t *a1 = a0 + (size_t)2 * I * 32;
   const uint8_t *d0 = D + ((((size_t)t * 2 + m) * N + p) * JB * 4 * C + (size_t)dl * C + (d_ok ? cg * 64 + dc : 0)) * 32 +
                       16 * hh;
-  auto masked = [](uint4 x, uint32_t mk) { return make_uint4(x.x & mk, x.y & mk, x.z & mk, x.w & mk); };
-  auto load = [&](MfmaPiece &pc, int jb) {
+  struct Piece {
+    uint4 a0, a1, d[DPT];
+  };
+  auto load = [&](Piece &pc, int jb) {
     pc.a0 = *reinterpret_cast<const uint4 *>(a0 + jb * a_jb);
     pc.a1 = *reinterpret_cast<const uint4 *>(a1 + jb * a_jb);
-    pc.d = *reinterpret_cast<const uint4 *>(d0 + jb * d_jb);
+#pragma unroll
+    for (int k = 0; k < DPT; k++) pc.d[k] = *reinterpret_cast<const uint4 *>(d0 + jb * d_jb + (size_t)k * 2 * C * 32);
   };
-  auto stage = [&](const MfmaPiece &pc, int st) {
-    sA[st][tid] = masked(pc.a0, ma);
-    sA[st][tid + 512] = masked(pc.a1, ma);
-    sD[st][tid] = masked(pc.d, md);
+  const int st_slot = mfma_slot(tid);  // tid + NT has the same bits 3-4
+  // Pieces of rows past I / columns past C are zeroed as they are staged.  The
+  // products would be harmless (they land in outputs that are never stored),
+  // but without the masking the compiler sinks the ring's loads toward their
+  // stage (about 3 loads in flight instead of 12: 3x slower, measured).
+  const uint32_t ma = a_ok ? ~0u : 0u, md = d_ok ? ~0u : 0u;
+  auto masked = [](uint4 x, uint32_t mk) { return make_uint4(x.x & mk, x.y & mk, x.z & mk, x.w & mk); };
+  auto stage = [&](const Piece &pc, int st) {
+    sA[st][st_slot] = masked(pc.a0, ma);
+    sA[st][st_slot + NT] = masked(pc.a1, ma);
+#pragma unroll
+    for (int k = 0; k < DPT; k++) sD[st][st_slot + k * NT] = masked(pc.d[k], md);
   };
   const int rw = wave >> 1, cw = wave & 1;
   const int h = lane >> 5, l32 = lane & 31;
+  // per-lane fragment slots: limb a adds a * RB * 64 (A) / a * 128 (D), which keeps bits 3-4
+  const int fa_slot = mfma_slot((rw * 32 + l32) * 2 + h), fd_slot = mfma_slot((cw * 32 + l32) * 2 + h);
   auto frags = [&](v4i (&fa)[4], v4i (&fd)[4], int st) {
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-      const uint4 x = sA[st][(a * 128 + rw * 32 + l32) * 2 + h];
-      const uint4 y = sD[st][(a * 64 + cw * 32 + l32) * 2 + h];
+      const uint4 x = sA[st][a * RB * 64 + fa_slot];
+      const uint4 y = sD[st][a * 128 + fd_slot];
       fa[a] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
       fd[a] = v4i{(int)y.x, (int)y.y, (int)y.z, (int)y.w};
     }
   };
   // Prologue: blocks 0 and 1 staged, blocks 2 .. PD + 1 in the ring (slot b % PD).
   {
-    MfmaPiece p0, p1;
+    Piece p0, p1;
     load(p0, 0);
     load(p1, min(1, JB - 1));
     stage(p0, 0);
     stage(p1, 1);
   }
-  MfmaPiece ring[PD];
+  Piece ring[PD];
 #pragma unroll
   for (int k = 0; k < PD; k++) {
     const int b = 2 + ((k - 2 + 2 * PD) % PD);  // the block of ring slot k among 2 .. PD + 1
@@ -2762,7 +2834,7 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
             acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca[a], cd[b], acc[a + b], 0, 0, 0);
         // block jb + 2 (ring slot (jb + 2) % PD) into its stage, then that slot
         // loads block jb + 2 + PD
-        MfmaPiece &sl = ring[(k + 2) % PD];
+        Piece &sl = ring[(k + 2) % PD];
         stage(sl, (jb + 2) % 3);
         load(sl, min(jb + 2 + PD, JB - 1));
         // keep the loads here: sunk toward their use they would leave too few in flight
@@ -2772,7 +2844,7 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
       }
     }
   }
-  const int row0 = ig * 128 + rw * 32, col0 = cg * 64 + cw * 32;
+  const int row0 = ig * RB * 32 + rw * 32, col0 = cg * 64 + cw * 32;
   if (row0 >= I || col0 >= C) return;
   const ModConst &mc = m ? mc1 : mc0;
   const uint32_t *pwm = pw + m * 8;
@@ -2780,10 +2852,168 @@ __global__ __launch_bounds__(512) void k_dbmul_mfma_ring(const uint8_t *__restri
 #pragma unroll
   for (int v = 0; v < 16; v++) {
     const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
-    uint64_t sum = 0;
+    outp[outp_word(m, p, t, i, c, T, I, C)] = limb_combine(acc, v, pwm, mc);
+  }
+}
+
+// Persistent form of k_dbmul_mfma_ring: a grid of BPC blocks per CU, each
+// looping over the tiles x TPX + y + k ny of its XCD x (y = blockIdx.x / 8,
+// ny = gridDim.x / 8), so every XCD still sweeps one contiguous range of tiles
+// (the tiles of a position share its L2).  The prefetch ring and the LDS
+// stages run on across tile boundaries: the first j blocks of tile k + 1 are
+// loaded and staged during the last ones of tile k, so only the first tile
+// pays the pipeline's fill, and no block waits for a dispatch.  With 4-wave
+// blocks (RB = 2) two blocks share a CU and drift apart, so one block's
+// epilogue overlaps the other's multiplies.  Four LDS stages (so that the
+// stage of j block jb is jb % 4 in every tile, JB a multiple of 4); the
+// sequence is that of k_dbmul_mfma_ring: in step g the waves multiply block g,
+// read the fragments of g + 1, stage g + 2 and load g + 2 + PD (PD + 2 <= JB,
+// so a load runs at most one tile ahead).
+template <int PD, int JB, int RB>
+__global__ __launch_bounds__(RB * 128) __attribute__((amdgpu_waves_per_eu(2))) void k_dbmul_mfma_pers(const uint8_t *__restrict__ A, const uint8_t *__restrict__ D,
+                                                              uint32_t *__restrict__ outp, int T, int C, int I, int n_ig,
+                                                              int n_cg, int TPX, ModConst mc0, ModConst mc1,
+                                                              const uint32_t *__restrict__ pw) {
+  static_assert(JB % 4 == 0 && PD + 2 <= JB && JB % PD == 0, "tile pipeline shape");
+  constexpr int NT = RB * 128, AP = RB * 32 * 4 * 2, DPT = 512 / NT;
+  __shared__ uint4 sA[4][AP];
+  __shared__ uint4 sD[4][512];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int x = (int)(blockIdx.x & 7u), y = (int)(blockIdx.x >> 3), ny = (int)(gridDim.x >> 3);
+  const int K = y < TPX ? (TPX - y + ny - 1) / ny : 0;  // tiles of this block
+  if (K == 0) return;                                   // uniform per block
+  const int ar = (tid >> 1) & (RB * 32 - 1), al = tid / (RB * 64), hh = tid & 1;
+  const int dl = tid >> 7, dc = (tid >> 1) & 63;
+  const size_t a_jb = (size_t)4 * I * 32, d_jb = (size_t)4 * C * 32;
+  struct Tile {
+    const uint8_t *a0, *d0;
+    uint32_t ma, md;
+    int t, ig, cg, p, m;
+  };
+  auto tile_of = [&](int k) {
+    Tile tl;
+    int bid = x * TPX + y + k * ny;
+    tl.t = bid % T; bid /= T;
+    tl.cg = bid % n_cg; bid /= n_cg;
+    tl.ig = bid % n_ig; bid /= n_ig;
+    tl.p = bid % N;
+    tl.m = bid / N;
+    const bool a_ok = tl.ig * RB * 32 + ar < I, d_ok = tl.cg * 64 + dc < C;
+    tl.ma = a_ok ? ~0u : 0u;
+    tl.md = d_ok ? ~0u : 0u;
+    tl.a0 = A + (((size_t)tl.m * N + tl.p) * JB * 4 * I + (size_t)al * I + (a_ok ? tl.ig * RB * 32 + ar : 0)) * 32 +
+            16 * hh;
+    tl.d0 = D + ((((size_t)tl.t * 2 + tl.m) * N + tl.p) * JB * 4 * C + (size_t)dl * C + (d_ok ? tl.cg * 64 + dc : 0)) *
+                    32 + 16 * hh;
+    return tl;
+  };
+  struct Piece {
+    uint4 a0, a1, d[DPT];
+  };
+  auto load = [&](Piece &pc, const Tile &tl, int jb) {
+    const uint8_t *pa = tl.a0 + jb * a_jb;
+    pc.a0 = *reinterpret_cast<const uint4 *>(pa);
+    pc.a1 = *reinterpret_cast<const uint4 *>(pa + (size_t)2 * I * 32);
 #pragma unroll
-    for (int w = 0; w < 7; w++) sum += (uint64_t)(uint32_t)acc[w][v] * pwm[w];
-    outp[((((size_t)m * N + p) * T + t) * I + i) * C + c] = mod_u64(sum, mc);
+    for (int k = 0; k < DPT; k++)
+      pc.d[k] = *reinterpret_cast<const uint4 *>(tl.d0 + jb * d_jb + (size_t)k * 2 * C * 32);
+  };
+  // masking as in k_dbmul_mfma_ring (it also keeps the ring's loads in place)
+  auto masked = [](uint4 v, uint32_t mk) { return make_uint4(v.x & mk, v.y & mk, v.z & mk, v.w & mk); };
+  const int st_slot = mfma_slot(tid);
+  auto stage = [&](const Piece &pc, const Tile &tl, int st) {
+    sA[st][st_slot] = masked(pc.a0, tl.ma);
+    sA[st][st_slot + NT] = masked(pc.a1, tl.ma);
+#pragma unroll
+    for (int k = 0; k < DPT; k++) sD[st][st_slot + k * NT] = masked(pc.d[k], tl.md);
+  };
+  const int rw = wave >> 1, cw = wave & 1;
+  const int h = lane >> 5, l32 = lane & 31;
+  const int fa_slot = mfma_slot((rw * 32 + l32) * 2 + h), fd_slot = mfma_slot((cw * 32 + l32) * 2 + h);
+  auto frags = [&](v4i (&fa)[4], v4i (&fd)[4], int st) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const uint4 u = sA[st][a * RB * 64 + fa_slot];
+      const uint4 v = sD[st][a * 128 + fd_slot];
+      fa[a] = v4i{(int)u.x, (int)u.y, (int)u.z, (int)u.w};
+      fd[a] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+    }
+  };
+  Tile cur = tile_of(0);
+  Tile nxt = K > 1 ? tile_of(1) : cur;
+  const int last_jb_if_none = JB - 1;  // loads past the last tile repeat its last block (never used)
+  {
+    Piece p0, p1;
+    load(p0, cur, 0);
+    load(p1, cur, 1);
+    stage(p0, cur, 0);
+    stage(p1, cur, 1);
+  }
+  Piece ring[PD];
+#pragma unroll
+  for (int k = 0; k < PD; k++) {
+    const int b = 2 + ((k - 2 + 2 * PD) % PD);  // the block of ring slot k among 2 .. PD + 1
+    load(ring[k], cur, b);
+  }
+  __syncthreads();
+  v4i ca[4], cd[4];
+  frags(ca, cd, 0);
+  const ModConst &mcm0 = mc0;
+  for (int k = 0; k < K; k++) {
+    const bool more = k + 1 < K;
+    v16i acc[7];
+#pragma unroll
+    for (int w = 0; w < 7; w++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[w][v] = 0;
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++) {
+      __syncthreads();  // stage (jb + 1) % 4 written; stage (jb + 2) % 4 no longer read
+      v4i na[4], nd[4];
+      // block jb + 1 (the next tile's block 0 is read after the epilogue, so
+      // that its fragments are not held through it)
+      if (jb + 1 < JB) frags(na, nd, (jb + 1) % 4);
+      // The slot's staging and refill go between the first SPLIT limbs' multiplies
+      // and the rest, so that the wave reaches the next barrier while its last
+      // multiplies still run.
+#pragma unroll
+      for (int a = 0; a < SP_PERS_SPLIT; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca[a], cd[b], acc[a + b], 0, 0, 0);
+      Piece &sl = ring[(jb + 2) % PD];
+      // block jb + 2 of this tile, or block jb + 2 - JB of the next one
+      if (jb + 2 < JB) stage(sl, cur, (jb + 2) % 4);
+      else stage(sl, nxt, (jb + 2) % 4);
+      // then the slot loads block jb + 2 + PD (of this tile or the next)
+      if (jb + 2 + PD < JB) load(sl, cur, jb + 2 + PD);
+      else if (more) load(sl, nxt, jb + 2 + PD - JB);
+      else load(sl, cur, last_jb_if_none);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int a = SP_PERS_SPLIT; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca[a], cd[b], acc[a + b], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (jb + 1 < JB)
+#pragma unroll
+        for (int a = 0; a < 4; a++) { ca[a] = na[a]; cd[a] = nd[a]; }
+    }
+    // epilogue of tile k
+    const int row0 = cur.ig * RB * 32 + rw * 32, col0 = cur.cg * 64 + cw * 32;
+    if (row0 < I && col0 < C) {
+      const ModConst &mc = cur.m ? mc1 : mcm0;
+      const uint32_t *pwm = pw + cur.m * 8;
+      const int c = col0 + l32;
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
+        outp[outp_word(cur.m, cur.p, cur.t, i, c, T, I, C)] = limb_combine(acc, v, pwm, mc);
+      }
+    }
+    cur = nxt;
+    if (k + 2 < K) nxt = tile_of(k + 2);
+    frags(ca, cd, 0);  // block 0 of tile k + 1 (stage 0, staged two steps ago; past the last tile: unused)
   }
 }
 
@@ -2858,21 +3088,23 @@ __global__ __launch_bounds__(RT * 128) void k_dbmul_mfma_stream(const uint8_t *_
 #pragma unroll
   for (int v = 0; v < 16; v++) {
     const int i = row0 + 8 * (v >> 2) + 4 * h + (v & 3);
-    uint64_t sum = 0;
-#pragma unroll
-    for (int w = 0; w < 7; w++) sum += (uint64_t)(uint32_t)acc[w][v] * pwm[w];
-    outp[((((size_t)m * N + p) * T + t) * I + i) * C + c] = mod_u64(sum, mc);
+    outp[outp_word(m, p, t, i, c, T, I, C)] = limb_combine(acc, v, pwm, mc);
   }
 }
 
-// p-major products [m][p][t][I][C] -> per-query accumulators [b][t][c][row][m][P].
+// Products (outp_word layout, Montgomery form) -> per-query accumulators
+// [b][t][c][row][m][P] (NTT domain, canonical: times 2^32 mod q), for readouts
+// between the multiply and the fold.  Block: 64 (t, i, c) combinations x 64
+// positions of residue m through an LDS tile.
 __global__ __launch_bounds__(256) void k_outp_to_acc(const uint32_t *__restrict__ outp, int T, int I, int C, int nq,
-                                                     uint32_t *__restrict__ acc, size_t acc_q) {
+                                                     uint32_t *__restrict__ acc, size_t acc_q, ModConst mc0,
+                                                     ModConst mc1) {
   __shared__ uint32_t tile[64][65];
-  __shared__ long long dst_of[64];                 // accumulator word of combination x (-1: none)
-  const int pb = blockIdx.x;                       // 64 positions
-  const int cb = blockIdx.y;                       // 64 (t, i, c) combinations
+  __shared__ long long dst_of[64];  // accumulator word of combination x (-1: none)
+  const int pb = blockIdx.x;        // 64 positions
+  const int cb = blockIdx.y;        // 64 (t, i, c) combinations
   const int m = blockIdx.z;
+  const ModConst &mc = m ? mc1 : mc0;
   const int tid = threadIdx.x;
   const int TIC = T * I * C;
   if (tid < 64) {  // one division per combination, not per element
@@ -2886,22 +3118,96 @@ __global__ __launch_bounds__(256) void k_outp_to_acc(const uint32_t *__restrict_
     dst_of[tid] = o;
   }
 #pragma unroll
-  for (int k = tid; k < 64 * 16; k += 256) {  // 16-byte loads: 4 combinations of one position
-    const int q = k & 15, pp = k >> 4;
-    const int e = cb * 64 + 4 * q;            // TIC is a multiple of 4
-    uint4 x = make_uint4(0, 0, 0, 0);
-    if (e < TIC) x = *reinterpret_cast<const uint4 *>(outp + ((size_t)m * N + pb * 64 + pp) * TIC + e);
-    tile[pp][4 * q] = x.x;
-    tile[pp][4 * q + 1] = x.y;
-    tile[pp][4 * q + 2] = x.z;
-    tile[pp][4 * q + 3] = x.w;
+  for (int k = tid; k < 64 * 16; k += 256) {  // 16-byte loads: 4 positions of one combination
+    const int q = k & 15, x = k >> 4;
+    const int e = cb * 64 + x;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (e < TIC) {
+      const int c = e % C, ti = e / C;  // ti = t * I + i
+      v = *reinterpret_cast<const uint4 *>(outp + ((((size_t)m * T * I + ti) * (N / 4) + pb * 16 + q) * C + c) * 4);
+    }
+    tile[4 * q][x] = v.x;
+    tile[4 * q + 1][x] = v.y;
+    tile[4 * q + 2][x] = v.z;
+    tile[4 * q + 3][x] = v.w;
   }
   __syncthreads();
   for (int k = tid; k < 64 * 64; k += 256) {
     const int pp = k & 63, x = k >> 6;
     const long long o = dst_of[x];
-    if (o >= 0) acc[(size_t)o + pb * 64 + pp] = tile[pp][x];
+    if (o >= 0) acc[(size_t)o + pb * 64 + pp] = canon(shoup_mul(tile[pp][x], mc.two32, mc.two32_p, mc.q), mc.q);
   }
+}
+
+// from_ntt straight from the multiply's products (layout of outp_word, NTT
+// domain, layout P, canonical) into the raw accumulators [b][t][c][row] (hi,
+// lo planes; i = 2 b + row), in place of k_outp_to_acc + k_from_ntt.  One
+// block of NCB consecutive columns of one (trial t, row i), wave 2 cc + m
+// inverting residue m of column c0 + cc: lane l's register chunk k (layout P
+// words 256 k + 4 l .. + 3) is one 16-byte piece of the products, so every
+// wave loads its own polynomial with no staging.  Each transform's own tile
+// then carries its canonical values to the CRT pass.  The columns of one
+// (t, i) run on one XCD (logical block order), so the 128-byte lines that 8
+// neighbouring columns share are fetched from HBM once.
+template <int NCB>
+__global__ __launch_bounds__(NCB * 128) void k_from_ntt_pm(const uint32_t *__restrict__ outp, int T, int I, int C,
+                                                           int nq, uint32_t *__restrict__ acc, size_t acc_q,
+                                                           const uint32_t *__restrict__ tw_base, ModConst mc0,
+                                                           ModConst mc1) {
+  __shared__ uint32_t tiles[2 * NCB][LDS_TILE];
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int cc = w >> 1, m = w & 1;
+  int bid = (int)blockIdx.x;
+  if ((gridDim.x & 7u) == 0) bid = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+  const int ncg = C / NCB;
+  const int c = (bid % ncg) * NCB + cc;
+  bid /= ncg;
+  const int i = bid % (2 * nq), t = bid / (2 * nq);
+  const ModConst &mc = m ? mc1 : mc0;
+  const uint4 *src = reinterpret_cast<const uint4 *>(outp + outp_word(m, 0, t, i, c, T, I, C));
+  uint32_t r[32];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {  // position chunk 64 k + lane: words 4 (64 k + lane) .. + 3
+    const uint4 v = src[(size_t)(64 * k + lane) * C];
+    r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
+  }
+  uint32_t *xm = tiles[w];
+  ntt_inv_wave_lazy<false>(r, tw_base + (m * 2 + 1) * TW_WORDS, xm, mc.q, mc.mu1, lane);
+#pragma unroll
+  for (int ii = 0; ii < 32; ii++) xm[64 * ii + lane] = canon(shoup_mul(r[ii], mc.crt_c32, mc.crt_c32p, mc.q), mc.q);
+  __syncthreads();
+  const uint32_t *x0 = tiles[2 * cc], *x1 = tiles[2 * cc + 1];
+  const int b = i >> 1, row = i & 1;
+  uint32_t *dst = acc + (size_t)b * acc_q + (((size_t)t * C + c) * 2 + row) * 2 * N;
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) {  // wave m combines coefficients 64 (16 m + jj) + lane
+    const int z = 64 * (16 * m + jj) + lane;
+    uint64_t x = (uint64_t)x0[z] * Q1 + (uint64_t)x1[z] * Q0;
+    x = (x >= QQ) ? x - QQ : x;
+    dst[z] = (uint32_t)(x >> 32);
+    dst[N + z] = (uint32_t)x;
+  }
+}
+
+hipError_t launch_outp_to_acc(const uint32_t *outp, int T, int I, int C, int nq, uint32_t *acc, size_t acc_q,
+                              const DevTables &t, hipStream_t s) {
+  const int TIC = T * I * C;
+  k_outp_to_acc<<<dim3(N / 64, (TIC + 63) / 64, 2), 256, 0, s>>>(outp, T, I, C, nq, acc, acc_q, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_from_ntt_pm(const uint32_t *outp, int T, int I, int C, int nq, uint32_t *acc, size_t acc_q,
+                              int nc, const DevTables &t, hipStream_t s) {
+  if (C % 8 || 2 * nq > I || (nc != 1 && nc != 2 && nc != 4)) return hipErrorInvalidValue;
+  const unsigned blocks = (unsigned)(T * 2 * nq * (C / nc));
+  if (nc == 4)
+    k_from_ntt_pm<4><<<blocks, 512, 0, s>>>(outp, T, I, C, nq, acc, acc_q, t.tw, t.mc[0], t.mc[1]);
+  else if (nc == 2)
+    k_from_ntt_pm<2><<<blocks, 256, 0, s>>>(outp, T, I, C, nq, acc, acc_q, t.tw, t.mc[0], t.mc[1]);
+  else
+    k_from_ntt_pm<1><<<blocks, 128, 0, s>>>(outp, T, I, C, nq, acc, acc_q, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
 }
 
 hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *D, hipStream_t s) {
@@ -2909,6 +3215,7 @@ hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *
   k_db_to_limbs<<<grid, 256, 0, s>>>(db, R, C, D);
   return hipGetLastError();
 }
+
 
 hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
                              uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
@@ -2942,25 +3249,41 @@ hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_
     }
 #undef SP_STREAM_JB
 #undef SP_STREAM
+  } else if (SP_PERS_RB && (JB == 4 || JB == 8 || JB == 16 || JB == 32)) {
+    constexpr int RBP = SP_PERS_RB > 0 ? SP_PERS_RB : 2;
+    const int p_ig = (I + RBP * 32 - 1) / (RBP * 32);
+    const int tpx = 2 * N * p_ig * n_cg * T / 8;
+    const unsigned pblocks = (unsigned)(std::max(t.n_cu, 8) / 8 * 8 * (RBP == 2 ? 2 : 1));
+#define SP_PERS(PDV, JBV)                                                                                  \
+  k_dbmul_mfma_pers<PDV, JBV, RBP><<<pblocks, RBP * 128, 0, s>>>(A, D, outp, T, C, I, p_ig, n_cg, tpx, t.mc[0], \
+                                                                 t.mc[1], pw)
+    switch (JB) {
+      case 4: SP_PERS(2, 4); break;
+      case 8: SP_PERS(SP_PERS_PD, 8); break;
+      case 16: SP_PERS(SP_PERS_PD, 16); break;
+      default: SP_PERS(SP_PERS_PD, 32); break;
+    }
+#undef SP_PERS
   } else {
-#define SP_RING(PDV, JBV) \
-  k_dbmul_mfma_ring<PDV, JBV><<<blocks, 512, 0, s>>>(A, D, outp, T, R, C, I, n_ig, n_cg, t.mc[0], t.mc[1], pw)
+    const int r_ig = (I + SP_RING_RB * 32 - 1) / (SP_RING_RB * 32);
+    const unsigned rblocks = (unsigned)(2 * N * r_ig * n_cg * T);
+#define SP_RING(PDV, JBV)                                                                                   \
+  k_dbmul_mfma_ring<PDV, JBV, SP_RING_RB><<<rblocks, SP_RING_RB * 128, 0, s>>>(A, D, outp, T, R, C, I, r_ig, n_cg, \
+                                                                           t.mc[0], t.mc[1], pw)
     switch (JB) {
       case 1: SP_RING(1, 1); break;
       case 2: SP_RING(2, 2); break;
-      case 4: SP_RING(4, 4); break;
-      case 8: SP_RING(4, 8); break;
-      case 16: SP_RING(4, 16); break;
-      case 32: SP_RING(4, 32); break;
-      default: SP_RING(4, 0); break;
+      case 4: SP_RING(SP_RING_PD, 4); break;
+      case 8: SP_RING(SP_RING_PD, 8); break;
+      case 16: SP_RING(SP_RING_PD, 16); break;
+      case 32: SP_RING(SP_RING_PD, 32); break;
+      default: SP_RING(SP_RING_PD, 0); break;
     }
 #undef SP_RING
   }
   e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const int TIC = T * I * C;
-  k_outp_to_acc<<<dim3(N / 64, (TIC + 63) / 64, 2), 256, 0, s>>>(outp, T, I, C, nq, acc, acc_q);
-  return hipGetLastError();
+  if (e != hipSuccess || !acc) return e;
+  return launch_outp_to_acc(outp, T, I, C, nq, acc, acc_q, t, s);
 }
 
 }  // namespace sp

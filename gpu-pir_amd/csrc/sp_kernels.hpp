@@ -133,14 +133,25 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
 // int8-limb MFMA database multiply (C >= 32, R multiple of 32).
 //   D: database limbs [t][m][p][R/32][4][C][32] (k_db_to_limbs, one-time)
 //   A: workspace for selector limbs, 2 * N * R * 4 * I bytes (I = 2*nq padded to 32)
-//   outp: workspace, 2 * N * T * I * C words;  pw: [m][8] = 2^(7w) mod q_m
+//   outp: workspace, 2 * N * T * I * C words ([m][t][I][N/4][C][4]), each product times
+//         2^-32 mod q_m (Montgomery form);  pw: [m][8] = 2^(7w) mod q_m
 hipError_t launch_db_to_limbs(const uint32_t *db, int T, int R, int C, uint8_t *D, hipStream_t s);
 // ring: 4 = staged through LDS with a 4-deep prefetch ring (I > 64); -1 =
 // streaming form for I <= 64 (blocks of the row waves only, a register
 // prefetch ring per wave); 0 = operands straight from global memory one block
 // ahead (the first small-batch form, kept for comparison).
+// acc = nullptr leaves the products in outp (for launch_from_ntt_pm);
+// otherwise they are transposed into the accumulators [b][t][c][row][m][P].
 hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_t *D, int T, int R, int C, int I,
                              uint8_t *A, uint32_t *outp, uint32_t *acc, size_t acc_q, const uint32_t *pw,
                              const DevTables &t, int ring, hipStream_t s);
+// Products of nq queries ([m][t][I][N/4][C][4]) -> NTT accumulators [b][t][c][row][m][P].
+hipError_t launch_outp_to_acc(const uint32_t *outp, int T, int I, int C, int nq, uint32_t *acc, size_t acc_q,
+                              const DevTables &t, hipStream_t s);
+// Products of nq queries -> raw accumulators (the from_ntt of every
+// accumulator, as launch_from_ntt(true, ...) after launch_outp_to_acc);
+// nc = 1, 2 or 4 columns per block, C a multiple of 8, 2 nq <= I.
+hipError_t launch_from_ntt_pm(const uint32_t *outp, int T, int I, int C, int nq, uint32_t *acc, size_t acc_q,
+                              int nc, const DevTables &t, hipStream_t s);
 
 }  // namespace sp

@@ -314,3 +314,34 @@ def test_dbmul_mfma_partial_row_tile(gpu_ctx, force_dbmul):
         np.testing.assert_array_equal(b.read(k, READ_DB_MUL), ref[k], err_msg=f"slot {k}")
     b.close()
     db.close()
+
+
+@pytest.mark.parametrize("nu1,nu2,B", [(9, 5, 48), (6, 5, 5)])
+def test_mfma_products_to_fold_directly(gpu_ctx, force_dbmul, nu1, nu2, B):
+    """After the MFMA multiply the fold stage takes its products out of the NTT domain
+    straight from the multiply's p-major output (no readout in between, so nothing
+    transposes them first): the raw accumulators and the responses equal those of the
+    VALU multiply (B = 48: LDS-staged kernel, partial row tile; B = 5: streaming kernel)."""
+    from spiral_pir.server import Batch, READ_RAW_ACCS, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD
+
+    s, recs, db = _setup(gpu_ctx, nu1, nu2, 940 + B)
+    qs = _clients_queries(s, B, 3, 3900 + B)
+    b = Batch(db, B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    slots = sorted({0, B // 2, B - 1})
+    out = {}
+    for var in (1, 2):
+        force_dbmul(var)
+        b.run(B, STAGE_EXPAND)
+        b.run(B, STAGE_DB_MUL)
+        assert b.last_run()["dbmul"] == ("mfma" if var == 2 else "valu")
+        b.run(B, STAGE_FOLD)
+        out[var] = (b.responses(B), {k: b.read(k, READ_RAW_ACCS) for k in slots})
+    np.testing.assert_array_equal(out[2][0], out[1][0])
+    for k in slots:
+        np.testing.assert_array_equal(out[2][1][k], out[1][1][k], err_msg=f"slot {k}")
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        np.testing.assert_array_equal(c.decode(out[2][0][slot]), recs[idx])
+    b.close()
+    db.close()

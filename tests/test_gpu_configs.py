@@ -4,9 +4,10 @@ oracle on the GPU:
 * (10, 7): the single-query answer stage by stage (expansion g = 11, a
   1024-row database multiply, a 7-level fold in the latency form) and the
   split fold form; the expansion with random keys, element by element;
-* 2^nu1 = 1024 rows through the int8-limb MFMA multiply (the 32-block ring
-  kernel and the streaming kernel) and the VALU kernels, random and
-  all-(q-1) databases; the ring kernels for 1, 2, 4 and 8 row blocks;
+* 2^nu1 = 1024 rows through the int8-limb MFMA multiply (the persistent
+  LDS-staged kernel with 32 row blocks and the streaming kernel) and the VALU
+  kernels, random and all-(q-1) databases; the LDS-staged kernels for 1, 2, 4
+  and 8 row blocks;
 * (8, 4), 16 queries: the VALU batched path (no MFMA at 2^nu2 = 16);
 * (9, 6), 64 queries: the autotuned graph batch, slots against the oracle;
 * 48/32/16 queries over (9, 6)/(10, 7)/(8, 4) in one group launch;
@@ -213,7 +214,7 @@ def test_dbmul_nu1_10_mfma_ring_and_valu(gpu_ctx, oracle, dbs, force_dbmul):
     entries = _clients_queries(s, 40, 4, 4100)
     b = _batch(db, entries)
     b.run(40, STAGE_EXPAND)
-    for var, name in ((2, "mfma ring<4,32>"), (1, "valu")):
+    for var, name in ((2, "mfma persistent, 32 row blocks"), (1, "valu")):
         force_dbmul(var)
         b.run(40, STAGE_DB_MUL)
         assert b.last_run()["dbmul"] == ("mfma" if var == 2 else "valu")
@@ -230,7 +231,7 @@ def test_dbmul_nu1_10_mfma_ring_and_valu(gpu_ctx, oracle, dbs, force_dbmul):
 
 def test_dbmul_nu1_10_extreme_residues(gpu_ctx, oracle, force_dbmul):
     """A (10, 5) database of all-(q-1) residues (every limb 127 over 1024 rows: the
-    int32 limb sums reach 4 x 1024 x 127^2) through the ring and streaming MFMA
+    int32 limb sums reach 4 x 1024 x 127^2) through the LDS-staged and streaming MFMA
     kernels and the VALU kernels (one and four queries per read), against the oracle."""
     from spiral_pir import Shape
     from spiral_pir.server import Database, STAGE_DB_MUL, STAGE_EXPAND
@@ -255,10 +256,10 @@ def test_dbmul_nu1_10_extreme_residues(gpu_ctx, oracle, force_dbmul):
 
 @pytest.mark.parametrize("nu1", [5, 6, 7, 8])
 def test_dbmul_mfma_ring_row_blocks(gpu_ctx, oracle, force_dbmul, nu1):
-    """33 queries (66 selector rows: the LDS-staged ring kernel) at 2^nu1 = 32 ..
-    256 rows, i.e. 1, 2, 4 and 8 row blocks (ring depth 1, 2, 4, 4): the MFMA
-    accumulators equal the VALU kernel's for every slot read and the oracle's
-    for one."""
+    """33 queries (66 selector rows: the LDS-staged kernels) at 2^nu1 = 32 .. 256
+    rows, i.e. 1, 2, 4 and 8 row blocks (one tile per block for 1 and 2, the
+    persistent kernel for 4 and 8): the MFMA accumulators equal the VALU kernel's
+    for every slot read and the oracle's for one."""
     from spiral_pir import Shape
     from spiral_pir.server import Database, READ_DB_MUL, STAGE_DB_MUL, STAGE_EXPAND
 
