@@ -934,11 +934,13 @@ __global__ __launch_bounds__(WPB * 64) void k_db_mul_q(const uint32_t *__restric
 }
 
 // Launch of the multi-query multiply.  form 1: 2 queries per read, 2 positions
-// per lane; form 2: 4 queries per read, 1 position per lane.  4-wave blocks
-// (3 per CU at 161-166 VGPRs): at (10, 7), 2 queries 1.36 ms (one database
-// read at HBM speed is 1.17 ms) and 4 queries 1.85 ms, against 1.43 / 2.56 ms
-// in 8-wave blocks and 2.25 / 4.37 ms for one read per query; deeper unrolling
-// and half-group lanes (4 columns x 2 positions) were slower.
+// per lane, 4-wave blocks (3 per CU at 161-166 VGPRs): at (10, 7), 1.36 ms for
+// 2 queries (one database read at HBM speed is 1.17 ms), against 1.43 ms in
+// 8-wave blocks and 2.25 ms for one read per query.  form 2: 4 queries per
+// read, 1 position per lane, 2-wave blocks without unrolling: at (9, 6), 8
+// queries 0.85 ms against 0.96 ms in 4-wave blocks unrolled twice (at (10, 7)
+// the 4-wave form took 1.85 ms for 4 queries, 8-wave blocks 2.56 ms).  Deeper
+// unrolling and half-group lanes (4 columns x 2 positions) were slower.
 hipError_t launch_db_mul_multi(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
                                uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, int form,
                                const DevTables &t, hipStream_t s) {
@@ -947,7 +949,8 @@ hipError_t launch_db_mul_multi(const uint32_t *db, const uint32_t *sel, size_t s
   k_db_mul_q<QV, PPLV, 4, 2><<<(unsigned)(2 * (N / (64 * PPLV)) * T * ncg * ((nq + QV - 1) / QV)), 256, 0, s>>>( \
       db, sel, sel_stride, sel_qstride, out, out_qstride, nq, T, R, C, t.mc[0], t.mc[1])
   if (form == 2)
-    SP_DBQ(4, 1);
+    k_db_mul_q<4, 1, 2, 1><<<(unsigned)(2 * (N / 64) * T * ncg * ((nq + 3) / 4)), 128, 0, s>>>(
+        db, sel, sel_stride, sel_qstride, out, out_qstride, nq, T, R, C, t.mc[0], t.mc[1]);
   else
     SP_DBQ(2, 2);
 #undef SP_DBQ
@@ -957,6 +960,19 @@ hipError_t launch_db_mul_multi(const uint32_t *db, const uint32_t *sel, size_t s
 hipError_t launch_db_mul(const uint32_t *db, const uint32_t *sel, size_t sel_stride, size_t sel_qstride,
                          uint32_t *out, size_t out_qstride, int nq, int T, int R, int C, const DevTables &t,
                          hipStream_t s) {
+  // One query, whole packed groups: k_db_mul_q in 2-wave blocks, 2 positions
+  // per lane, rows not unrolled (1024 blocks at (9, 6), one per 128 positions,
+  // trial, residue and column group; each wave streams 256 rows).  Measured at
+  // (9, 6) back to back: 271-273 us against 298 us for k_db_mul<8, 8>; 4-wave
+  // blocks 279 us, 1-wave blocks 406 us, 4 positions per lane 275-279 us,
+  // unrolled rows 278-346 us.  For several queries k_db_mul, whose consecutive
+  // blocks on an XCD take one database tile for different queries, stays
+  // ahead (8 queries: 2.09 against 2.15-2.18 ms).
+  if (nq == 1 && C >= 8 && R >= 2) {
+    k_db_mul_q<1, 2, 2, 1><<<(unsigned)(2 * (N / 128) * T * (C / 8)), 128, 0, s>>>(
+        db, sel, sel_stride, sel_qstride, out, out_qstride, 1, T, R, C, t.mc[0], t.mc[1]);
+    return hipGetLastError();
+  }
   const int cg = db_group_cols(C);
   const unsigned blocks = (unsigned)(32 * T * (C / cg) * nq);
   // 8 waves split the rows (R >= 8); the smallest databases (R = 4) use 4.
