@@ -26,10 +26,10 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-pir_amd"))
 
 METRIC = "PIR queries/s + effective DB-scan GB/s, batched vs 1-query; p50/p99 latency"
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per k_db_mul launch for one query, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs scaled by a plain copy kernel's readings (FETCH x2.000, WRITE x1.000);
-# profiles/db_mul_hbm_traffic_nu9_6.md.  Measured shapes only.
-DB_MUL_TRAFFIC_BYTES = {(9, 6): (1852411 + 8192) * 1024}
+# HBM bytes per single-query multiply launch (k_db_mul_q<1, 2, 2, 1>), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy kernel's readings (FETCH
+# x2.000, WRITE x1.000); profiles/db_mul_hbm_traffic_nu9_6.md.  Measured shapes only.
+DB_MUL_TRAFFIC_BYTES = {(9, 6): (1851516 + 8192) * 1024}
 
 
 def db_mul_algorithmic_bytes(shape, packed: bool = False) -> int:
