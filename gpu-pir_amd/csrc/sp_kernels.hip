@@ -1941,16 +1941,48 @@ __device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, u
 // of v_i (< q) and, on row 1, the gathered e term (< q) are added to it (< 7q
 // < 8q), and one reduction (conditional subtractions of 4q, 2q, q) brings the
 // sum to [0, q).  Precondition: v_i, and so the gathered e term, canonical.
+// The e-term gather of expand_update: tau(as_e) at this thread's positions.
 template <int PPT>
-__device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
-                                              uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,
-                                              uint32_t q, uint32_t qinv) {
-  uint32_t ase[PPT];
+__device__ __forceinline__ void expand_gather_e(uint32_t (&ase)[PPT], const uint32_t *__restrict__ vi, int m, int p0,
+                                                uint32_t t) {
 #pragma unroll
   for (int e = 0; e < PPT; e++) {
     const uint32_t z = (uint32_t)z_of_p(p0 + e);
     const uint32_t ev = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
     ase[e] = vi[(size_t)(2 + m) * N + p_of_z((int)brv11((ev - 1) >> 1))];
+  }
+}
+
+// Key words of digits k < KP (rows 0 and 1) at this thread's 4 positions,
+// loaded ahead of the digit transforms by callers that have the registers.
+template <int KP>
+struct ExpandKeys {
+  uint4 k[KP > 0 ? KP : 1][2];
+  uint32_t ase[4];
+};
+template <int KP>
+__device__ __forceinline__ void expand_prefetch(ExpandKeys<KP> &pk, const uint32_t *__restrict__ vi,
+                                                const uint32_t *__restrict__ W, int m, int p0, uint32_t t) {
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + p0;
+    pk.k[k][0] = *reinterpret_cast<const uint4 *>(k0);
+    pk.k[k][1] = *reinterpret_cast<const uint4 *>(k0 + N);
+  }
+  expand_gather_e<4>(pk.ase, vi, m, p0, t);
+}
+
+template <int PPT, int KP = 0>
+__device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
+                                              uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,
+                                              uint32_t q, uint32_t qinv, const ExpandKeys<KP> *pk = nullptr) {
+  static_assert(KP == 0 || PPT == 4, "prefetched keys cover 4 positions");
+  uint32_t ase[PPT];
+  if constexpr (KP > 0) {
+#pragma unroll
+    for (int e = 0; e < PPT; e++) ase[e] = pk->ase[e];
+  } else {
+    expand_gather_e<PPT>(ase, vi, m, p0, t);
   }
   uint64_t acc[2][PPT];
 #pragma unroll
@@ -1962,11 +1994,22 @@ __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const u
     const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + p0;
     const uint32_t *k1 = k0 + N;
     const uint32_t *tl = &tiles[tile0 + k][p0];
+    if (k < KP) {
+      const uint4 a = pk->k[k < KP ? k : 0][0], b = pk->k[k < KP ? k : 0][1];
+      const uint32_t ka[4] = {a.x, a.y, a.z, a.w}, kb[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int e = 0; e < PPT; e++) {
-      const uint32_t d = tl[e];
-      acc[0][e] += (uint64_t)d * k0[e];
-      acc[1][e] += (uint64_t)d * k1[e];
+      for (int e = 0; e < PPT; e++) {
+        const uint32_t d = tl[e];
+        acc[0][e] += (uint64_t)d * ka[e & 3];
+        acc[1][e] += (uint64_t)d * kb[e & 3];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < PPT; e++) {
+        const uint32_t d = tl[e];
+        acc[0][e] += (uint64_t)d * k0[e];
+        acc[1][e] += (uint64_t)d * k1[e];
+      }
     }
   }
   __syncthreads();  // tiles read and row 1 of residue m gathered by every thread
@@ -2167,6 +2210,10 @@ void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   for (int u = tid; u < 8 * DIG_WORDS / 4; u += 128) dst4[u] = src4[u];
 }
 
+#ifndef SP_UPD_KP
+#define SP_UPD_KP 3
+#endif
+
 // k_expand_update: one 8-wave block per (working step b, query, residue m):
 // wave k transforms digit k of the step for residue m; then every thread
 // updates residue m of v_i at its 4 positions (expand_update).
@@ -2186,6 +2233,10 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
   const uint32_t q = m ? mc1.q : mc0.q, mu1 = m ? mc1.mu1 : mc0.mu1, qinv = m ? mc1.qinv_neg : mc0.qinv_neg;
   const uint32_t *dg = dig + (size_t)blockIdx.y * dig_qstride + ((size_t)b * 8 + wave) * DIG_WORDS;
+  // keys of the first SP_UPD_KP digits and the e-term gather, loaded ahead of
+  // the digit transforms so that the multiply-accumulate waits for less
+  ExpandKeys<SP_UPD_KP> pk;
+  if constexpr (SP_UPD_KP > 0) expand_prefetch<SP_UPD_KP>(pk, vi, W, m, 4 * tid, t);
   uint32_t rr[32];
 #pragma unroll
   for (int i4 = 0; i4 < 8; i4++) {
@@ -2197,7 +2248,7 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   ntt_fwd_wave_t<8, true>(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
   store_p(tiles[wave], rr, lane);  // lazy (< 8q): expand_update reduces after the MAC
   __syncthreads();
-  expand_update<4>(vi, W, tiles, 0, m, 4 * tid, t, q, qinv);
+  expand_update<4, SP_UPD_KP>(vi, W, tiles, 0, m, 4 * tid, t, q, qinv, &pk);
   // Round r + 1's child v[i + 2^(r+1)] = v_neg1[r+1] (.) v_i (residue m, this
   // thread's positions, just written by this thread), unless i is skipped in
   // round r + 1; k_expand_digits of round r + 1 then reads it as formed.
