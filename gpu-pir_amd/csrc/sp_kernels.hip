@@ -2210,8 +2210,126 @@ void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   for (int u = tid; u < 8 * DIG_WORDS / 4; u += 128) dst4[u] = src4[u];
 }
 
+// k_expand_digits2: the digits of a working parent p < 2^r and of its child
+// c = p + 2^r in one 2-wave block.  The child is formed from p before round
+// r's updates, v_c = v_neg1[r] (.) v_p, so v_c.a = v_p.a * x^(-2^r): its
+// coefficients are p's shifted down by 2^r, those that wrap around negated
+// (Q - x; 0 stays 0, as the canonical inverse transform of v_c.a gives),
+// and one pair of inverse transforms serves both steps.  Grid x: [0, npair)
+// the working parents (below stop_round all p < 2^r; from stop_round on the
+// even ones, then the odd ones that work in the round), then `extra` blocks
+// that only form the v_neg1 children of skipped odd elements i = odd_start +
+// 2e + 1, as in k_expand_digits.  Digit slot of element i: its step index in
+// the update kernel's order (from stop_round on, even steps first).
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
+void k_expand_digits2(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int npair,
+                      int odd_start, int children_formed, const uint32_t *__restrict__ neg1_r,
+                      uint32_t *__restrict__ dig, size_t dig_qstride, const uint32_t *__restrict__ tw_base,
+                      ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t tiles[2][LDS_TILE];
+  const int b = blockIdx.x, half = 1 << r;
+  const int tid = threadIdx.x;
+  const int m = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const ModConst mc = m ? mc1 : mc0;
+  uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
+  const bool split = stop > 0 && r >= stop;
+  // wave m: rows 0 and 1 of residue m of element i = v_neg1 (.) v[i - 2^r]
+  auto form_child = [&](int i) {
+    const uint32_t *src = vq + (size_t)(i - half) * 4 * N;
+    uint32_t *dstv = vq + (size_t)i * 4 * N;
+    const uint4 *w4 = reinterpret_cast<const uint4 *>(neg1_r + (size_t)m * 2 * N);
+    const uint4 *wp4 = reinterpret_cast<const uint4 *>(neg1_r + (size_t)m * 2 * N + N);
+#pragma unroll
+    for (int row = 0; row < 2; row++) {
+      const uint4 *s4 = reinterpret_cast<const uint4 *>(src + (size_t)(row * 2 + m) * N);
+      uint4 *d4 = reinterpret_cast<uint4 *>(dstv + (size_t)(row * 2 + m) * N);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint4 x = s4[c * 64 + lane], w = w4[c * 64 + lane], wp = wp4[c * 64 + lane];
+        uint4 y;
+        y.x = canon(shoup_mul(x.x, w.x, wp.x, mc.q), mc.q);
+        y.y = canon(shoup_mul(x.y, w.y, wp.y, mc.q), mc.q);
+        y.z = canon(shoup_mul(x.z, w.z, wp.z, mc.q), mc.q);
+        y.w = canon(shoup_mul(x.w, w.w, wp.w, mc.q), mc.q);
+        d4[c * 64 + lane] = y;
+      }
+    }
+  };
+  if (b >= npair) {  // uniform per block
+    form_child(odd_start + 2 * (b - npair) + 1);
+    return;
+  }
+  const int p = split ? (b < half / 2 ? 2 * b : 2 * (b - half / 2) + 1) : b;
+  const int c = p + half;
+  const bool work_c = !expand_skipped(c, r, stop, max_bits);
+  if (work_c && !children_formed) form_child(c);  // round 0: nothing formed v_1 yet
+  auto slot = [&](int i) { return split ? ((i & 1) ? half + (i >> 1) : (i >> 1)) : i; };
+  uint32_t a[32];
+  load_p(a, vq + (size_t)p * 4 * N + (size_t)m * N, lane);
+  ntt_inv_wave_lazy(a, tw_base + (m * 2 + 1) * TW_WORDS, tiles[m], mc.q, mc.mu1, lane);
+#pragma unroll
+  for (int ii = 0; ii < 32; ii++) a[ii] = canon(shoup_mul(a[ii], mc.crt_c, mc.crt_cp, mc.q), mc.q);
+  // CRT exchange as in k_expand_digits: wave m combines the coefficients
+  // 64 (16 m + jj) + lane.
+  uint32_t own[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) {
+    own[jj] = m ? a[16 + jj] : a[jj];
+    tiles[m][jj * 64 + lane] = m ? a[jj] : a[16 + jj];
+  }
+  __syncthreads();
+  uint64_t xp[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) {
+    const uint32_t oth = tiles[1 - m][jj * 64 + lane];
+    const uint32_t x0 = m ? oth : own[jj], x1 = m ? own[jj] : oth;
+    uint64_t x = (uint64_t)x0 * Q1 + (uint64_t)x1 * Q0;
+    xp[jj] = (x >= QQ) ? x - QQ : x;
+  }
+  __syncthreads();  // the exchange tiles are dead
+  uint64_t xc[16];
+  if (work_c) {  // uniform per block: the child's coefficients through the tiles (2048 x 8 bytes)
+    uint64_t *xl = reinterpret_cast<uint64_t *>(&tiles[0][0]);
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) xl[64 * (16 * m + jj) + lane] = xp[jj];
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int s = 64 * (16 * m + jj) + lane + half;
+      const uint64_t y = xl[s & (N - 1)];
+      xc[jj] = s < N ? y : (y ? QQ - y : 0);
+    }
+    __syncthreads();
+  }
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  uint8_t *db = reinterpret_cast<uint8_t *>(&tiles[0][0]);  // 4096 words over tiles 0-1
+  auto digits_out = [&](const uint64_t (&x)[16], int i) {
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int z = 64 * (16 * m + jj) + lane;
+      const uint32_t zt = (uint32_t)z * t;
+      const uint32_t dst = zt & (N - 1);
+      const uint64_t y = ((zt >> LOGN) & 1) ? QQ - x[jj] : x[jj];
+      const uint32_t word = (dst >> 8) * 64 + (dst & 63), byte = (dst >> 6) & 3;
+#pragma unroll
+      for (int k = 0; k < 8; k++) db[((size_t)k * DIG_WORDS + word) * 4 + byte] = (uint8_t)((y >> (7 * k)) & 127u);
+    }
+    __syncthreads();
+    const uint4 *src4 = reinterpret_cast<const uint4 *>(&tiles[0][0]);
+    uint4 *dst4 =
+        reinterpret_cast<uint4 *>(dig + (size_t)blockIdx.y * dig_qstride + (size_t)slot(i) * 8 * DIG_WORDS);
+    for (int u = tid; u < 8 * DIG_WORDS / 4; u += 128) dst4[u] = src4[u];
+    __syncthreads();  // the tiles are free again
+  };
+  digits_out(xp, p);
+  if (work_c) digits_out(xc, c);
+}
+
 #ifndef SP_UPD_KP
 #define SP_UPD_KP 3
+#endif
+#ifndef SP_DIGITS_PAIR
+#define SP_DIGITS_PAIR 1
 #endif
 
 // k_expand_update: one 8-wave block per (working step b, query, residue m):
@@ -2421,8 +2539,22 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
   // formed the children of its working elements; the extra blocks still form
   // the children of skipped elements (and, at stop_round, recompute children
   // already formed: same parent, same value).
-  k_expand_digits<<<dim3(work + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, work, odd_start, r > 0 ? 1 : 0,
-                                                                neg1_r, dig, dig_qstride, t.tw, t.mc[0], t.mc[1]);
+  // Rounds whose one-step blocks exceed two full waves of the chip (16 two-wave
+  // blocks per CU) take the pair kernel, which saves the inverse transforms of
+  // every child (64 queries at (9,6): expansion 4.85 -> 4.69 ms, measured on
+  // one box); smaller rounds keep one step per block, whose shorter blocks
+  // finish sooner (one query: 598 against 602 us per answer with pairs).
+  if (SP_DIGITS_PAIR && (long)work * n_queries > 16L * t.n_cu) {
+    const int half = 1 << r;  // parents whose pair block forms the digits (see k_expand_digits2)
+    const int npair = (stop > 0 && r >= stop) ? half / 2 + (r == stop ? std::min(half / 2, max_bits) : 0) : half;
+    k_expand_digits2<<<dim3(npair + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, max_bits, npair,
+                                                                    odd_start, r > 0 ? 1 : 0, neg1_r, dig,
+                                                                    dig_qstride, t.tw, t.mc[0], t.mc[1]);
+  } else {
+    k_expand_digits<<<dim3(work + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, work, odd_start,
+                                                                  r > 0 ? 1 : 0, neg1_r, dig, dig_qstride, t.tw,
+                                                                  t.mc[0], t.mc[1]);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_expand_update<<<dim3(work, n_queries, 2), 512, 0, s>>>(v, q_stride, r, stop, max_bits, neg1_next, dig, dig_qstride,
