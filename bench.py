@@ -650,6 +650,8 @@ def main() -> int:
             "definition": "database as u32 NTT residues (2^(nu1+nu2) n^2 2 2048 4 B) + selectors + outputs, "
                           "per launch, over the mean launch duration (HIP events around K launches back to back "
                           "on the launch stream; db_mul_in_chain_ms has it inside whole answers)",
+            "note": "frac can reach 1.0 because the survey counts the database as u32 residues while the "
+                    "resident copy packs 8 residues in 7 words: the kernel moves the packed_layout bytes",
             "packed_layout": {"bytes": packed_bytes, "achieved": achieved_packed, "frac": achieved_packed / HBM_PEAK_GBPS,
                               "definition": "the bytes this kernel reads: the resident database packed 8 residues "
                                             "in 7 words + selectors + outputs (what the traffic counters measure)"},
