@@ -345,3 +345,34 @@ def test_mfma_products_to_fold_directly(gpu_ctx, force_dbmul, nu1, nu2, B):
         np.testing.assert_array_equal(c.decode(out[2][0][slot]), recs[idx])
     b.close()
     db.close()
+
+
+def test_expansion_pair_digits_below_and_at_stop_round(gpu_ctx, oracle):
+    """80 queries at (5, 5), t_GSW = 8 (g = 7, stop_round 6): rounds 5 and 6 have more
+    one-step digit blocks than 16 per CU (64 x 80 and 104 x 80 against 16 x 256), so
+    their digits come from the parent-and-child kernel, below stop_round (every
+    element works) and at it (even parents, then the odd ones that still work), the
+    children's coefficients by rotation of the parents'; the earlier rounds take one
+    step per block.  The expansion of the first, a middle and the last slot equals
+    the oracle's, element by element, and every record decodes."""
+    from spiral_pir.server import Batch, READ_EXPANDED, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD
+
+    s, recs, db = _setup(gpu_ctx, 5, 5, 970)
+    B = 80
+    qs = _clients_queries(s, B, 5, 4700)
+    b = Batch(db, B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    b.run(B, STAGE_EXPAND)
+    for slot in (0, 41, B - 1):
+        c, idx, q, left, right, vf = qs[slot]
+        np.testing.assert_array_equal(b.read(slot, READ_EXPANDED),
+                                      oracle.expand(q, s.nu1, s.nu2, s.t_gsw, s.t_exp, left, right),
+                                      err_msg=f"slot {slot}")
+    b.run(B, STAGE_DB_MUL)
+    b.run(B, STAGE_FOLD)
+    resp = b.responses(B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        np.testing.assert_array_equal(c.decode(resp[slot]), recs[idx], err_msg=f"slot {slot}")
+    b.close()
+    db.close()
