@@ -376,3 +376,44 @@ def test_expansion_pair_digits_below_and_at_stop_round(gpu_ctx, oracle):
         np.testing.assert_array_equal(c.decode(resp[slot]), recs[idx], err_msg=f"slot {slot}")
     b.close()
     db.close()
+
+
+def test_dbmul_mfma_two_row_tiles_against_oracle(gpu_ctx, oracle, force_dbmul):
+    """100 queries at (7, 5) = 200 selector rows (I = 224): the persistent MFMA kernel's
+    row tiles of 128 are a whole one and a partial one (rows 128-199 hold queries,
+    200-223 zero selectors, 224-255 are masked), 4 row blocks of 32; its
+    accumulators equal the VALU multiply's for slots in both
+    tiles and the oracle's for one slot of the second; the expansion of rounds 5-7
+    takes the parent-and-child digits kernel; every record decodes."""
+    from spiral_pir.server import Batch, READ_DB_MUL, READ_EXPANDED, STAGE_DB_MUL, STAGE_EXPAND, STAGE_FOLD
+
+    s, recs, db = _setup(gpu_ctx, 7, 5, 980)
+    B = 100
+    qs = _clients_queries(s, B, 5, 4800)
+    b = Batch(db, B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        b.load(slot, q, left, right, vf)
+    b.run(B, STAGE_EXPAND)
+    force_dbmul(1)
+    b.run(B, STAGE_DB_MUL)
+    slots = (0, 63, 64, 99)
+    ref = {k: b.read(k, READ_DB_MUL) for k in slots}
+    force_dbmul(2)
+    b.run(B, STAGE_DB_MUL)
+    assert b.last_run()["dbmul"] == "mfma"
+    for k in slots:
+        np.testing.assert_array_equal(b.read(k, READ_DB_MUL), ref[k], err_msg=f"slot {k}")
+    db_ntt = oracle.db_encode(recs, s.n2, s.nu1, s.nu2)
+    v = b.read(99, READ_EXPANDED)
+    c, idx, q, left, right, vf = qs[99]
+    np.testing.assert_array_equal(v, oracle.expand(q, s.nu1, s.nu2, s.t_gsw, s.t_exp, left, right))
+    sel = np.ascontiguousarray(v[0:2 * s.rows:2])
+    for t in range(s.n2):
+        np.testing.assert_array_equal(ref[99][t], oracle.db_mul(db_ntt[t], sel, s.nu1, s.nu2, nthreads=0),
+                                      err_msg=f"trial {t}")
+    b.run(B, STAGE_FOLD)
+    resp = b.responses(B)
+    for slot, (c, idx, q, left, right, vf) in enumerate(qs):
+        np.testing.assert_array_equal(c.decode(resp[slot]), recs[idx], err_msg=f"slot {slot}")
+    b.close()
+    db.close()
