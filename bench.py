@@ -198,6 +198,17 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
     rooflines["db_mul"] = {"bound": "mfma" if tuned["chosen"] == "mfma" else "valu", "kernel": tuned["chosen"],
                            "achieved": ops / 1e12, "peak": 5000.0, "unit": "int8 TOP/s (16 limb products per MAC)",
                            "frac": ops / 5e15}
+    # the batch's dominant kernels: the expansion rounds (digits + update
+    # launches, over half of the batch), in the single-query line's format
+    ex = rooflines["expand"]
+    roofline = {"kernel": "expansion rounds (k_expand_digits / k_expand_digits2 + k_expand_update, "
+                          f"{shape.g} rounds, {B} queries)",
+                "bound": "valu", "achieved": ex["achieved"], "peak": ex["peak"], "unit": ex["unit"],
+                "frac": ex["frac"], "traffic": None,
+                "share_of_batch": stage_ms["expand"] / sum(stage_ms.values()),
+                "definition": f"{ex['work']} ({expansion_steps(shape)} working steps x 18 transforms: 2 inverse "
+                              "and 16 digit transforms per step) over the stage's time from HIP events around one "
+                              "eager run; peak: VALU issue rate at 5 instructions per lazy butterfly"}
     resp = batch.responses(B)
     bad = [s for s in range(B) if not np.array_equal(clients[s % n_clients].decode(resp[s]), records[idx[s]])]
     ok = not bad
@@ -212,7 +223,7 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
     batch.close()
     return {"B": B, "ms_per_batch": ms, "queries_per_s": B * 1000.0 / ms, "steps": steps,
             "launch": "eager (one launch per kernel)" if eager else "hipGraph replay of the whole stage chain",
-            "stage_ms_eager": stage_ms, "stage_roofline": rooflines,
+            "stage_ms_eager": stage_ms, "roofline": roofline, "stage_roofline": rooflines,
             "db_mul_autotune": tuned, "decoded_all_correct": ok, "slots_decoded_wrong": bad}
 
 
