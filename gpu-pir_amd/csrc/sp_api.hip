@@ -226,7 +226,8 @@ extern "C" int sp_ctx_destroy(sp_ctx *ctx) {
 }
 
 extern "C" int sp_ctx_set_variant(sp_ctx *ctx, int stage, int variant) {
-  if (!ctx || stage < 0 || stage > 2 || variant < 0 || variant > 4)
+  const bool ok = variant >= 0 && (variant <= 4 || (stage == SP_STAGE_EXPAND && variant >= 10 && variant <= 10 + LOGN));
+  if (!ctx || stage < 0 || stage > 2 || !ok)
     return fail(SP_ERR_ARG, "sp_ctx_set_variant: stage %d variant %d", stage, variant);
   ctx->variant[stage] = variant;
   return SP_OK;
@@ -520,22 +521,41 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
   const int family = expansion_family(ctx, t_exp, dig);
   if (family == 4 && !dig) return fail(SP_ERR_STATE, "split expansion without digit scratch");
   // Latency form for the leading rounds whose one-launch grid gives each CU at
-  // most one block: at (9,6), one query, rounds 0-6 take 16.4-18.5 us each
-  // against 21-22 us for the two split launches; round 7 (384 blocks) is
-  // slower in one launch (26 vs 25 us).
-  int fused = 0;
-  if (family == 4 && snap.p && n_queries <= snap.queries && ctx->variant[SP_STAGE_EXPAND] == 0)
-    while (fused < rounds && 2L * expand_fused_blocks(fused, stop, t_gsw * nu2) * n_queries <= (long)ctx->tables.n_cu)
-      fused++;
+  // most one block: first the half-position form (four 4-wave blocks per
+  // step), then the 8-wave form (two blocks per step), then the split form.
+  // At (9,6), one query: rounds 0-5 in halves 4 us faster each than split,
+  // round 6 (448 half blocks) 0.7 us slower; round 6 in the 8-wave form (224
+  // blocks) faster than split; round 7 (384 blocks) slower in one launch.
+  // Variant 10 + R forces the half-position form for the first R rounds (those
+  // with N >> r >= 4), then the split form.
+  const int var = ctx->variant[SP_STAGE_EXPAND];
+  int halves = 0, fused = 0;  // rounds in the half-position form, in either latency form
+  if (family == 4 && snap.p && n_queries <= snap.queries) {
+    const long cu = ctx->tables.n_cu;
+    auto blocks = [&](int r) { return (long)expand_fused_blocks(r, stop, t_gsw * nu2) * n_queries; };
+    if (var == 0) {
+      while (halves < rounds && (N >> halves) >= 4 && 4 * blocks(halves) <= cu) halves++;
+      fused = halves;
+      while (fused < rounds && 2 * blocks(fused) <= cu) fused++;
+    } else if (var >= 10) {
+      while (halves < std::min(rounds, var - 10) && (N >> halves) >= 4) halves++;
+      fused = halves;
+    }
+  }
   // The queries (if given) go to v[0]; with the latency form one launch also readies round 0.
   if (fused > 0)
     HIP_TRY(launch_expand_prep(query, query_qstride, v, q_stride, n_queries, snap.p, snap.q_stride, ctx->tables, s));
   else if (query)
     HIP_TRY(launch_copy_queries(query, query_qstride, v, q_stride, n_queries, s));
-  for (int r = 0; r < fused; r++)
-    HIP_TRY(launch_expand_fused(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2,
-                                left_p + (size_t)r * key_level, right_p + (size_t)std::min(r, stop) * key_level,
-                                left_qstride, right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s));
+  for (int r = 0; r < fused; r++) {
+    const uint32_t *lr = left_p + (size_t)r * key_level, *rr = right_p + (size_t)std::min(r, stop) * key_level;
+    if (r < halves)
+      HIP_TRY(launch_expand_fused_h(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2, lr, rr, left_qstride,
+                                    right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s));
+    else
+      HIP_TRY(launch_expand_fused(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2, lr, rr, left_qstride,
+                                  right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s));
+  }
   for (int r = fused; r < rounds; r++) {
     const int num_in = 1 << r;
     const uint32_t *neg1_r = ctx->tables.neg1 + (size_t)r * 2 * 2 * N;
@@ -578,7 +598,8 @@ extern "C" int sp_expand(sp_ctx *ctx, uint32_t *dev_v, int nu1, int nu2, int t_g
   HIP_TRY(launch_nat_to_p(dev_right, kr.u32(), (size_t)(stop + 1) * key_level_polys, 2 * t_exp, 1, ctx->tables, s));
   DevBuf dig, snapbuf;
   ExpandSnap snap;
-  if (t_exp == 8 && (ctx->variant[SP_STAGE_EXPAND] == 0 || ctx->variant[SP_STAGE_EXPAND] == 4)) {
+  if (t_exp == 8 && (ctx->variant[SP_STAGE_EXPAND] == 0 || ctx->variant[SP_STAGE_EXPAND] == 4 ||
+                     ctx->variant[SP_STAGE_EXPAND] >= 10)) {
     HIP_TRY(dig.alloc((size_t)expansion_max_work(nu1, nu2, t_gsw) * 8 * 512 * 4));
     snap.q_stride = snap.set_stride = n_ct * 2 * N;
     snap.queries = 1;

@@ -203,18 +203,34 @@ __device__ __forceinline__ uint32_t tw_b(const uint32_t *__restrict__ tw, int sl
 // < 4q); with TAIL the remaining 11 - LAZY - 1 stages are lazy too and the
 // outputs are below (4 + 2 (10 - LAZY)) q, else Harvey's bound 4q holds.
 // mu1 = floor(2^32 / q).
-template <int LAZY, bool TAIL = false>
+//
+// HALF: the two half-branches of a "virtual" array.  After stage 1 (stride
+// 1024) the outputs of positions [0, 1024) (branch 0: x + t) and [1024, 2048)
+// (branch 1: x - t) depend only on their own half of the array, through the
+// same butterfly network with the twiddles of that half.  With HALF the
+// caller has done stage 1 of TWO polynomials A and B for one branch h, A's
+// branch in r[0..15] (indices 64 i + lane) and B's in r[16..31]; stages
+// 2-11 then run with branch h's twiddles in both halves (phase A: block blk
+// of stage s takes the twiddle of block (blk mod 2^(s-1)) + h 2^(s-1); phase
+// B: lane l takes the per-lane twiddles of lane (l mod 32) + 32 h).
+// Out: lanes 0-31 hold A's positions 1024 h + 32 lane + j, lanes 32-63 B's
+// positions 1024 h + 32 (lane - 32) + j.  One wave thus transforms two
+// polynomials' halves for the cost of one transform.
+template <int LAZY, bool TAIL = false, bool HALF = false>
 __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
-                                               uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane) {
+                                               uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane,
+                                               int h = 0) {
   static_assert(LAZY >= 0 && LAZY <= 8, "at most 8 lazy stages");
   static_assert(!TAIL || (LAZY > 0 && 4 + 2 * (10 - LAZY) <= 15), "lazy tail bound");
+  static_assert(!HALF || LAZY > 0, "the caller's stage 1 is lazy");
   // Phase A: strides 1024 .. 64 (index bits 10..6 live in the register index).
 #pragma unroll
-  for (int s = 0; s < 5; s++) {
+  for (int s = HALF ? 1 : 0; s < 5; s++) {
     const int half = 16 >> s;
 #pragma unroll
     for (int blk = 0; blk < (1 << s); blk++) {
-      const uint32_t w = tw_a(tw, (1 << s) + blk, 0), wp = tw_a(tw, (1 << s) + blk, 1);
+      const int ti = HALF ? (1 << s) + (blk & ((1 << (s - 1)) - 1)) + (h << (s - 1)) : (1 << s) + blk;
+      const uint32_t w = tw_a(tw, ti, 0), wp = tw_a(tw, ti, 1);
 #pragma unroll
       for (int i = 0; i < half; i++)
         ct_stage_bf<LAZY, TAIL>(1 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
@@ -229,13 +245,14 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
   for (int i = 0; i < 32; i++) r[i] = lds[tile_at(L, 2 * i + b)];
   wave_lds_sync();
   // Phase B: strides 32 .. 2 in registers.
+  const int tl = HALF ? ((lane & 31) | (h << 5)) : lane;  // lane whose per-lane twiddles apply
   int slot = 0;
 #pragma unroll
   for (int s = 0; s < 5; s++) {
     const int half = 16 >> s;
 #pragma unroll
     for (int blk = 0; blk < (1 << s); blk++) {
-      const uint32_t w = tw_b(tw, slot + blk, 0, lane), wp = tw_b(tw, slot + blk, 1, lane);
+      const uint32_t w = tw_b(tw, slot + blk, 0, tl), wp = tw_b(tw, slot + blk, 1, tl);
 #pragma unroll
       for (int i = 0; i < half; i++)
         ct_stage_bf<LAZY, TAIL>(6 + s, r[blk * 2 * half + i], r[blk * 2 * half + i + half], w, wp, q, mu1);
@@ -251,7 +268,7 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
     const uint32_t recv = swap_lane_pair(send);
     uint32_t x = b ? recv : r[k];
     uint32_t y = b ? r[16 + k] : recv;
-    ct_stage_bf<LAZY, TAIL>(11, x, y, tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q, mu1);
+    ct_stage_bf<LAZY, TAIL>(11, x, y, tw_b(tw, 31 + k, 0, tl), tw_b(tw, 31 + k, 1, tl), q, mu1);
     o[2 * k] = x;
     o[2 * k + 1] = y;
   }
@@ -262,6 +279,28 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
 __device__ __forceinline__ void ntt_fwd_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                              uint32_t *__restrict__ lds, uint32_t q, int lane) {
   ntt_fwd_wave_t<0>(r, tw, lds, q, 0u, lane);
+}
+
+// Stage 1 of two byte-packed digit polynomials A and B (byte b of word i4 is
+// coefficient 64 (4 i4 + b) + lane, values < 128), kept for branch H in the
+// form ntt_fwd_wave_t<LAZY, TAIL, true> takes: r[i] = A's branch value at
+// index 64 i + lane, r[16 + i] = B's (lazy: x + t for H = 0, x - t + 2q for
+// H = 1, with t = y w mod q in [0, 2q) for y the coefficient 1024 further on;
+// outputs < 128 + 2q).
+template <int H>
+__device__ __forceinline__ void digits_stage1_half(uint32_t (&r)[32], const uint32_t (&da)[8], const uint32_t (&db)[8],
+                                                   const uint32_t *__restrict__ tw, uint32_t q) {
+  const uint32_t nw = tw_a(tw, 1, 0), wp = tw_a(tw, 1, 1);
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const uint32_t wlo = p ? db[i >> 2] : da[i >> 2], whi = p ? db[4 + (i >> 2)] : da[4 + (i >> 2)];
+      const uint32_t x = (wlo >> (8 * (i & 3))) & 0xFFu, y = (whi >> (8 * (i & 3))) & 0xFFu;
+      const uint32_t nt = shoup_mul_neg(y, nw, wp, q);
+      r[16 * p + i] = H ? x + 2 * q + nt : x - nt;
+    }
+  }
 }
 
 // Inverse negacyclic NTT (without the factor N^-1) of one polynomial.

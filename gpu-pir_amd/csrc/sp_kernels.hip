@@ -1972,15 +1972,17 @@ __device__ __forceinline__ void expand_prefetch(ExpandKeys<KP> &pk, const uint32
   expand_gather_e<4>(pk.ase, vi, m, p0, t);
 }
 
-template <int PPT, int KP = 0>
-__device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
-                                              uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,
-                                              uint32_t q, uint32_t qinv, const ExpandKeys<KP> *pk = nullptr) {
+// tile_of(k): this thread's words of digit k's transform in LDS (PPT of them);
+// key_of(k, row): its key words of digit k and row `row` (global memory or LDS).
+template <int PPT, int KP = 0, class TileOf, class KeyOf>
+__device__ __forceinline__ void expand_update_tk(uint32_t *__restrict__ vi, TileOf tile_of, KeyOf key_of, int m, int p0,
+                                                 uint32_t t, uint32_t q, uint32_t qinv,
+                                                 const ExpandKeys<KP> *pk = nullptr) {
   static_assert(KP == 0 || PPT == 4, "prefetched keys cover 4 positions");
   uint32_t ase[PPT];
-  if constexpr (KP > 0) {
+  if (KP > 0 || pk) {  // gathered ahead by the caller (4 positions)
 #pragma unroll
-    for (int e = 0; e < PPT; e++) ase[e] = pk->ase[e];
+    for (int e = 0; e < PPT; e++) ase[e] = pk->ase[e & 3];
   } else {
     expand_gather_e<PPT>(ase, vi, m, p0, t);
   }
@@ -1991,9 +1993,8 @@ __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const u
     for (int e = 0; e < PPT; e++) acc[row][e] = 0;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + p0;
-    const uint32_t *k1 = k0 + N;
-    const uint32_t *tl = &tiles[tile0 + k][p0];
+    const uint32_t *k0 = key_of(k, 0), *k1 = key_of(k, 1);
+    const uint32_t *tl = tile_of(k);
     if (k < KP) {
       const uint4 a = pk->k[k < KP ? k : 0][0], b = pk->k[k < KP ? k : 0][1];
       const uint32_t ka[4] = {a.x, a.y, a.z, a.w}, kb[4] = {b.x, b.y, b.z, b.w};
@@ -2028,6 +2029,21 @@ __device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const u
       dst[e] = canon(x, q);
     }
   }
+}
+
+template <int PPT, int KP = 0, class TileOf>
+__device__ __forceinline__ void expand_update_t(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
+                                                TileOf tile_of, int m, int p0, uint32_t t, uint32_t q, uint32_t qinv,
+                                                const ExpandKeys<KP> *pk = nullptr) {
+  expand_update_tk<PPT, KP>(
+      vi, tile_of, [&](int k, int row) { return W + ((size_t)(m * 8 + k) * 2 + row) * N + p0; }, m, p0, t, q, qinv,
+      pk);
+}
+template <int PPT, int KP = 0>
+__device__ __forceinline__ void expand_update(uint32_t *__restrict__ vi, const uint32_t *__restrict__ W,
+                                              uint32_t (*tiles)[LDS_TILE], int tile0, int m, int p0, uint32_t t,
+                                              uint32_t q, uint32_t qinv, const ExpandKeys<KP> *pk = nullptr) {
+  expand_update_t<PPT, KP>(vi, W, [&](int k) { return &tiles[tile0 + k][p0]; }, m, p0, t, q, qinv, pk);
 }
 
 // Expansion step (gpu-pir/expansion.cuh:87-166) for element i of round r,
@@ -2473,6 +2489,83 @@ void k_expand_fused(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, 
   }
 }
 
+// Latency form in half-position blocks: one 4-wave block per (step, residue m,
+// position half h), as k_expand_fused but each block owns the NTT positions
+// 1024 h .. 1024 h + 1023 of residue m.  After stage 1 (stride 1024) a forward
+// transform's two halves are independent, so wave w runs the half-h branches
+// of digits w and w + 4 as one transform (ntt_fwd_wave_t<.., true>): the 8
+// digit transforms of a step take one transform per SIMD instead of two, and a
+// step's work spreads over twice the CUs.  The v_i.a inverse transforms, CRT
+// and digits are computed by every block of the step (free in a mostly idle
+// chip).  tau_t with t = 1 (mod 4) (every round with 2^r <= N / 4) maps each
+// half of the NTT positions onto itself, so the e-term gather of a block reads
+// only positions of its own half, which no other block writes; the caller
+// keeps rounds with t = 3 (mod 4) on the other forms.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_expand_fused_h(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work, int last,
+                      const uint32_t *__restrict__ neg1_r, const uint32_t *__restrict__ neg1_next,
+                      const uint32_t *__restrict__ s_in, uint32_t *__restrict__ s_out, size_t s_qstride,
+                      const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
+                      size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
+  __shared__ uint32_t tiles[6][LDS_TILE];
+  const int b = blockIdx.x, half = 1 << r, m = blockIdx.z & 1, h = blockIdx.z >> 1;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const ModConst &mcm = m ? mc1 : mc0;
+  // This thread's 4 words of half h in layout P: positions 1024 h + 32 l' + 4 c
+  // + e (l' = tid mod 32, c = tid / 32) at word 256 c + 4 (l' + 32 h) + e.
+  const int gp = 256 * (tid >> 5) + 4 * (tid & 31) + 128 * h;
+  uint32_t *vq = v + (size_t)blockIdx.y * q_stride;
+  if (b >= work) {  // v_neg1 child of a skipped (odd) element: i = half + 2 (b - work) + 1
+    const int i = half + 2 * (b - work) + 1;
+    neg1_rows4(vq + (size_t)(i - half) * 4 * N, vq + (size_t)i * 4 * N, nullptr, neg1_r, m, gp, mcm);
+    return;
+  }
+  const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  uint32_t *vi = vq + (size_t)i * 4 * N;
+  const uint32_t t = (uint32_t)(N >> r) + 1u;
+  const bool use_left = (r != 0) && ((i & 1) == 0);
+  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
+  const uint32_t *si = s_in + (size_t)blockIdx.y * s_qstride + (size_t)i * 2 * N;
+  expand_digits<256>(si, tiles, t, tw_base, mc0, mc1, wave, lane, tid);
+  uint32_t da[8], db[8];  // digits wave and wave + 4: byte b of word i4 is coefficient 64 (4 i4 + b) + lane
+  const uint32_t *dig = tiles[4];
+#pragma unroll
+  for (int i4 = 0; i4 < 8; i4++) {
+    da[i4] = dig[wave * DIG_WORDS + i4 * 64 + lane];
+    db[i4] = dig[(wave + 4) * DIG_WORDS + i4 * 64 + lane];
+  }
+  __syncthreads();  // the digit bytes are dead from here on
+  {
+    const uint32_t *twm = tw_base + (m * 2) * TW_WORDS;
+    uint32_t rr[32];
+    if (h)
+      digits_stage1_half<1>(rr, da, db, twm, mcm.q);
+    else
+      digits_stage1_half<0>(rr, da, db, twm, mcm.q);
+    // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
+    ntt_fwd_wave_t<8, true, true>(rr, twm, tiles[wave], mcm.q, mcm.mu1, lane, h);
+    // half tiles: digit `wave` at words [0, 1024) of tiles[wave], digit wave + 4
+    // at [1024, 2048); position 1024 h + 32 l' + 4 c + e at word 128 c + 4 l' + e
+    uint4 *t4 = reinterpret_cast<uint4 *>(tiles[wave] + (lane >> 5) * 1024) + (lane & 31);
+#pragma unroll
+    for (int c = 0; c < 8; c++) t4[c * 32] = make_uint4(rr[4 * c], rr[4 * c + 1], rr[4 * c + 2], rr[4 * c + 3]);
+  }
+  __syncthreads();
+  const int hp = 4 * tid;  // = 128 c + 4 l': this thread's words in every half tile
+  expand_update_t<4>(vi, W, [&](int k) { return &tiles[k & 3][(k >> 2) * 1024 + hp]; }, m, gp, t, mcm.q,
+                     mcm.qinv_neg);
+  if (last) return;
+  // Round r + 1: this thread's 4 positions of residue m, as just written.
+  uint32_t *so = s_out + (size_t)blockIdx.y * s_qstride;
+  *reinterpret_cast<uint4 *>(so + (size_t)i * 2 * N + (size_t)m * N + gp) =
+      *reinterpret_cast<const uint4 *>(vi + (size_t)m * N + gp);
+  if (!expand_skipped(i, r + 1, stop, max_bits)) {
+    const int c = i + 2 * half;
+    neg1_rows4(vi, vq + (size_t)c * 4 * N, so + (size_t)c * 2 * N, neg1_next, m, gp, mcm);
+  }
+}
+
 // Working steps of round r (grid of k_expand8 / k_expand_update), and the
 // skipped odd elements >= 2^r that still take the v_neg1 product.
 static void round_plan(int r, int stop, int max_bits, int &work, int &odd_start, int &extra) {
@@ -2528,6 +2621,23 @@ hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int 
   return hipGetLastError();
 }
 
+hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
+                                 const uint32_t *left, const uint32_t *right, size_t left_qstride,
+                                 size_t right_qstride, uint32_t *snap, size_t s_qstride, size_t s_round,
+                                 const DevTables &t, hipStream_t s) {
+  if ((N >> r) < 4) return hipErrorInvalidValue;  // tau_t must keep each position half (t = 1 mod 4)
+  int work, odd_start, extra;
+  round_plan(r, stop, max_bits, work, odd_start, extra);
+  const int neg1_only = (stop > 0 && r > stop) ? (1 << r) / 2 : 0;
+  const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
+  const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
+  const uint32_t *s_in = snap + (size_t)(r & 1) * s_round;
+  uint32_t *s_out = snap + (size_t)((r + 1) & 1) * s_round;
+  k_expand_fused_h<<<dim3(work + neg1_only, n_queries, 4), 256, 0, s>>>(
+      v, q_stride, r, stop, max_bits, work, r + 1 >= end, neg1_r, neg1_next, s_in, s_out, s_qstride, left, right,
+      left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
+  return hipGetLastError();
+}
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s) {

@@ -124,6 +124,12 @@ int expand_fused_blocks(int r, int stop, int max_bits);
 hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s);
+// The latency form in half-position blocks (4-wave blocks per step, residue
+// and half of the NTT positions); rounds with (N >> r) >= 4 only.
+hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
+                                 const uint32_t *left, const uint32_t *right, size_t left_qstride,
+                                 size_t right_qstride, uint32_t *snap, size_t s_qstride, size_t s_round,
+                                 const DevTables &t, hipStream_t s);
 // Round r of a run of `end` rounds in the split form (digits, then per-residue
 // transforms and update; the update also forms round r + 1's children).
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,

@@ -67,7 +67,10 @@ int sp_ctx_synchronize(sp_ctx *ctx);
  *     3 = 16-wave blocks (one per CU, half the passes), 4 = split in two
  *     launches (expansion: digits, then per-residue transforms and update;
  *     fold: partial products per residue and key-column half, then finish).
- *     Automatic: 4 for both
+ *     Automatic: 4 for both, the expansion with its leading rounds in one
+ *     launch each for runs of at most 2 queries.  SP_STAGE_EXPAND only:
+ *     10 + R = the first R rounds in one launch each with 4-wave blocks per
+ *     half of the NTT positions (rounds with 2048 >> r >= 4), then 4
  *   SP_STAGE_DB_MUL: 1 = VALU u64 multiply-accumulate (one query per database
  *     read), 2 = int8-limb MFMA, 3 / 4 = VALU with 2 / 4 queries per database
  *     read (runs of 2+ queries); a forced variant that does not apply to a run

@@ -147,7 +147,7 @@ def test_answer_nu10_7_stage_by_stage(gpu_ctx, oracle, dbs):
     one.close()
 
 
-@pytest.mark.parametrize("variant", [0, 4], ids=["auto", "split"])
+@pytest.mark.parametrize("variant", [0, 4, 21], ids=["auto", "split", "halves"])
 def test_expansion_nu10_7_random_keys(gpu_ctx, oracle, variant):
     """g = 11 rounds with random query and keys (not client keys: every key word
     arbitrary), element by element against the oracle."""

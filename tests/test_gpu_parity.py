@@ -213,8 +213,8 @@ def test_fold(gpu_ctx, oracle, nu2, general, variant):
     np.testing.assert_array_equal(got[0], exp[0])
 
 
-@pytest.mark.parametrize("variant", [(0, 1), (0, 2), (0, 3), (0, 4)], indirect=True,
-                         ids=["expand-4wave", "expand-8wave", "expand-16wave", "expand-split"])
+@pytest.mark.parametrize("variant", [(0, 1), (0, 2), (0, 3), (0, 4), (0, 21)], indirect=True,
+                         ids=["expand-4wave", "expand-8wave", "expand-16wave", "expand-split", "expand-halves"])
 @pytest.mark.parametrize("nu1,nu2", [(6, 2), (3, 1)])
 def test_expansion_random_keys(gpu_ctx, oracle, nu1, nu2, variant):
     from spiral_pir import ops
