@@ -562,6 +562,21 @@ def main() -> int:
     stop.record(stream)
     torch.cuda.synchronize()
     dbmul_ms = start.elapsed_time(stop) / K
+    # The streaming-read ceiling of this part: the library's read kernel (the
+    # multiply's non-temporal 16-byte loads, nothing else) over a buffer the size
+    # of the packed resident database, K launches back to back.
+    sbuf = torch.empty(shape.db_device_bytes, dtype=torch.uint8, device="cuda").fill_(3)
+    sink = torch.empty(Context.stream_sink_words(), dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        ctx.stream_read(sbuf.data_ptr(), sbuf.numel(), sink.data_ptr(), sptr)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for k in range(K):
+        ctx.stream_read(sbuf.data_ptr(), sbuf.numel(), sink.data_ptr(), sptr)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    stream_read_gbps = sbuf.numel() * K / (start.elapsed_time(stop) * 1e-3) / 1e9
+    del sbuf, sink
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(min(K, 20))]
     for a, b in ev:
         query.run(STAGE_EXPAND, sptr)
@@ -666,7 +681,20 @@ def main() -> int:
             "packed_layout": {"bytes": packed_bytes, "achieved": achieved_packed, "frac": achieved_packed / HBM_PEAK_GBPS,
                               "definition": "the bytes this kernel reads: the resident database packed 8 residues "
                                             "in 7 words + selectors + outputs (what the traffic counters measure)"},
+            "measured_peak": {"gbps": stream_read_gbps, "frac_packed": achieved_packed / stream_read_gbps,
+                              "definition": "the same part's streaming-read ceiling, measured in this run: the "
+                                            "library's read kernel (sp_stream_read: the multiply's non-temporal "
+                                            "16-byte loads, no arithmetic) over a buffer the size of the packed "
+                                            "database, K launches back to back; frac_packed = the multiply's "
+                                            "packed-byte rate over it"},
         },
+        "answer_roofline": {
+            "bound": "hbm",
+            "queries_per_s_at_peak": HBM_PEAK_GBPS * 1e9 / alg_bytes,
+            "frac": (1000.0 / ms_per_step) / (HBM_PEAK_GBPS * 1e9 / alg_bytes),
+            "definition": "queries/s over the rate at which the HBM peak moves one answer's algorithmic "
+                          "multiply bytes (the survey's u32 count); the rest of the answer, expansion and fold, "
+                          "is chains of dependent latency-bound launches that this bound leaves out"},
         "batched": None if batched is None else {k: v for k, v in batched.items() if k != "decoded_all_correct"},
         "group": group,
         "serving": serving,

@@ -1820,6 +1820,18 @@ extern "C" int sp_batch_synchronize(sp_batch *b) {
   return SP_OK;
 }
 
+// ---- measurement ----------------------------------------------------------------
+extern "C" size_t sp_stream_sink_words(void) { return stream_sink_words(); }
+
+extern "C" int sp_stream_read(sp_ctx *ctx, const void *dev_buf, size_t bytes, uint32_t *dev_sink, void *stream) {
+  if (!ctx || !dev_buf || !dev_sink) return fail(SP_ERR_ARG, "sp_stream_read: NULL");
+  if (bytes == 0 || bytes % 16 || reinterpret_cast<uintptr_t>(dev_buf) % 16)
+    return fail(SP_ERR_ARG, "sp_stream_read: %zu bytes (a multiple of 16, 16-byte aligned)", bytes);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_stream_read(dev_buf, bytes, dev_sink, pick(ctx, stream)));
+  return SP_OK;
+}
+
 // ---- single query = batch of capacity 1 -------------------------------------
 extern "C" int sp_query_create(sp_db *db, int t_gsw, int t_exp, sp_query **out) {
   if (!out) return fail(SP_ERR_ARG, "sp_query_create: NULL");

@@ -3597,4 +3597,35 @@ hipError_t launch_dbmul_mfma(const uint32_t *v, size_t v_q, int nq, const uint8_
   return launch_outp_to_acc(outp, T, I, C, nq, acc, acc_q, t, s);
 }
 
+// ---- measurement: HBM streaming-read ceiling ----------------------------------
+// Reads `n16` 16-byte words with non-temporal loads (the database multiply's
+// load form), SR words in flight per thread, and xors them into one sink word
+// per thread so that nothing is optimised away: the rate a pure streaming read
+// reaches on this part, the practical ceiling of the HBM-bound multiply.
+constexpr int STREAM_BLOCKS = 1024, STREAM_UNROLL = 8;
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_read(const v4u *__restrict__ src, size_t n16,
+                                                     uint32_t *__restrict__ sink) {
+  uint32_t acc = 0;
+  const size_t stride = (size_t)gridDim.x * 256 * STREAM_UNROLL;
+  for (size_t base = (size_t)blockIdx.x * 256 * STREAM_UNROLL + threadIdx.x; base < n16; base += stride) {
+    v4u w[STREAM_UNROLL];
+#pragma unroll
+    for (int u = 0; u < STREAM_UNROLL; u++) {
+      const size_t k = base + (size_t)u * 256;
+      w[u] = k < n16 ? __builtin_nontemporal_load(src + k) : v4u{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int u = 0; u < STREAM_UNROLL; u++) acc ^= w[u].x ^ w[u].y ^ w[u].z ^ w[u].w;
+  }
+  sink[(size_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+size_t stream_sink_words() { return (size_t)STREAM_BLOCKS * 256; }
+
+hipError_t launch_stream_read(const void *src, size_t bytes, uint32_t *sink, hipStream_t s) {
+  k_stream_read<<<STREAM_BLOCKS, 256, 0, s>>>(reinterpret_cast<const v4u *>(src), bytes / 16, sink);
+  return hipGetLastError();
+}
+
 }  // namespace sp

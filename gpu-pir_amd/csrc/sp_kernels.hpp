@@ -160,4 +160,9 @@ hipError_t launch_outp_to_acc(const uint32_t *outp, int T, int I, int C, int nq,
 hipError_t launch_from_ntt_pm(const uint32_t *outp, int T, int I, int C, int nq, uint32_t *acc, size_t acc_q,
                               int nc, const DevTables &t, hipStream_t s);
 
+// HBM streaming-read ceiling: reads bytes / 16 16-byte words of src (bytes a
+// multiple of 16) into stream_sink_words() sink words.
+size_t stream_sink_words();
+hipError_t launch_stream_read(const void *src, size_t bytes, uint32_t *sink, hipStream_t s);
+
 }  // namespace sp

@@ -49,6 +49,7 @@ SIGNATURES = {
     "sp_db_mul": [P, P, P, P, I, I, P],
     "sp_fold": [P, P, I, I, P, P, P],
     "sp_fold_noalloc": [P, P, I, I, P, P, P],
+    "sp_stream_read": [P, P, SZ, P, P],
     "sp_expansion_depth": [I, I, I, PI, PI],
     "sp_expand": [P, P, I, I, I, I, P, P, I, P],
     "sp_regev_to_gsw": [P, P, SZ, I, I, I, P, P, P],
@@ -115,6 +116,7 @@ _RET = {
     "sp_version": (ctypes.c_char_p, []),
     "sp_client_last_error": (ctypes.c_char_p, []),
     "sp_db_device_bytes": (SZ, [P]),
+    "sp_stream_sink_words": (SZ, []),
 }
 
 _lib = None

@@ -80,6 +80,16 @@ class Context:
     def synchronize(self) -> None:
         _lib.call("sp_ctx_synchronize", self._h)
 
+    def stream_read(self, buf_ptr: int, nbytes: int, sink_ptr: int, stream=None) -> None:
+        """One launch of the HBM streaming-read ceiling kernel over nbytes of device
+        memory at buf_ptr (sink: stream_sink_words() u32 words of device memory)."""
+        _lib.call("sp_stream_read", self._h, ctypes.c_void_p(buf_ptr), nbytes, ctypes.c_void_p(sink_ptr),
+                  ctypes.c_void_p(stream or 0))
+
+    @staticmethod
+    def stream_sink_words() -> int:
+        return int(_lib.load().sp_stream_sink_words())
+
     def set_variant(self, stage: int, variant: int) -> None:
         """Kernel choice per stage (STAGE_* index 0/1/2 = expand/fold/db_mul); 0 = automatic."""
         _lib.call("sp_ctx_set_variant", self._h, stage, variant)

@@ -413,6 +413,15 @@ int sp_client_decode_packed(sp_client *c, const uint8_t *host_response, size_t b
 int sp_client_secret(const sp_client *c, int32_t *host_s);
 const char *sp_client_last_error(void);
 
+/* ---- measurement -----------------------------------------------------------
+ * No counterpart in the reference.  The HBM streaming-read ceiling that the
+ * database multiply is measured against: one launch reads `bytes` of device
+ * memory (a multiple of 16, 16-byte aligned) with the multiply's
+ * non-temporal 16-byte loads and writes sp_stream_sink_words() u32 words to
+ * dev_sink.  Asynchronous on `stream`. */
+size_t sp_stream_sink_words(void);
+int sp_stream_read(sp_ctx *ctx, const void *dev_buf, size_t bytes, uint32_t *dev_sink, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

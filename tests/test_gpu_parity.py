@@ -347,3 +347,18 @@ def test_query_rerun_is_identical(gpu_ctx):
     np.testing.assert_array_equal(c.decode(r1), recs[77])
     q.close()
     db.close()
+
+
+@pytest.mark.parametrize("words", [4, 1 << 20, (1 << 22) + 12])
+def test_stream_read_reads_every_word(gpu_ctx, words):
+    """The streaming-read ceiling kernel (bench measurement) touches every word:
+    the xor of its sink equals the xor of the buffer, for sizes below one pass
+    of the grid, a whole number of passes and a ragged tail."""
+    import torch
+
+    a = rng(990 + words % 97).integers(0, 2**32, size=words, dtype=np.uint32)
+    d = dev(a)
+    sink = torch.full((gpu_ctx.stream_sink_words(),), -1, dtype=torch.int32, device="cuda")
+    gpu_ctx.stream_read(d.data_ptr(), 4 * words, sink.data_ptr())
+    got = int(np.bitwise_xor.reduce(host(sink)))
+    assert got == int(np.bitwise_xor.reduce(a))
