@@ -227,6 +227,111 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
             "db_mul_autotune": tuned, "decoded_all_correct": ok, "slots_decoded_wrong": bad}
 
 
+def run_pipeline(torch, mods, db, shape, records, B: int, steps: int) -> dict:
+    """Host<->device pipelining (the overlap of uploads, kernels and readouts for
+    arriving queries that gpu-pir/README.md:169-194 and optplan.md:69-78 plan):
+    `steps` batches of B queries with fold vectors (3 MiB each at (9,6)) from
+    pinned host memory, resident client keys, through two batch buffers.
+    Serialized: upload, graph launch, readout, wait, batch after batch on one
+    buffer.  Pipelined: batch k on buffer k mod 2, whose upload is queued on the
+    buffer's io stream behind its previous run and whose run and readout go on
+    the buffer's own stream, so one buffer's transfers overlap the other's
+    kernels.  The parts are also timed alone."""
+    from spiral_pir.serving import pinned_u32
+
+    Batch, Client, Keys = mods["Batch"], mods["Client"], mods["Keys"]
+    n_clients = min(8, B)
+    clients = [Client(shape, 2500 + k) for k in range(n_clients)]
+    keys = [Keys(db, *c.keygen()) for c in clients]
+    idx = [int(x) for x in np.random.default_rng(2600).integers(0, shape.records, size=B)]
+    qs, vfs = [], []
+    for j in range(B):
+        q, vf = clients[j % n_clients].query(idx[j])
+        pq, pv = pinned_u32(q.size), pinned_u32(vf.size)
+        pq[:] = q.reshape(-1)
+        pv[:] = vf.reshape(-1)
+        qs.append(pq)
+        vfs.append(pv)
+    slot_keys = [keys[j % n_clients] for j in range(B)]
+    bufs = [Batch(db, B) for _ in range(2)]
+    streams = [torch.cuda.Stream() for _ in range(2)]  # default priority
+    outs = [pinned_u32(B * shape.response_words) for _ in range(2)]
+    for k, b in enumerate(bufs):  # one-time work: autotuning and graph capture, outside the timings
+        b.load_many(0, qs, slot_keys, vfs)
+        if k == 0:
+            b.run(B, 1, streams[0].cuda_stream)
+            chosen = b.tune(B)["chosen"]
+        else:
+            b.set_dbmul(B, chosen)
+        b.graph_launch(B, streams[k].cuda_stream)
+        b.synchronize()
+
+    def wall(f) -> float:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / steps
+
+    b0, s0 = bufs[0], streams[0].cuda_stream
+    upload_ms = wall(lambda: b0.load_many(0, qs, slot_keys, vfs))
+    compute_ms = wall(lambda: b0.graph_launch(B, s0))
+
+    def readout():
+        b0.read_responses_async(B, outs[0], s0)
+        b0.synchronize()
+
+    readout_ms = wall(readout)
+    ref = outs[0].copy()
+
+    def serialized():
+        b0.load_many(0, qs, slot_keys, vfs)
+        b0.graph_launch(B, s0)
+        b0.read_responses_async(B, outs[0], s0)
+        b0.synchronize()
+
+    serialized_ms = wall(serialized)
+    ser_ok = bool(np.array_equal(outs[0], ref))
+    state = {"k": 0}
+    pipe_ok = []
+
+    def pipelined():
+        k = state["k"]
+        b, s, out = bufs[k & 1], streams[k & 1].cuda_stream, outs[k & 1]
+        if k >= 2:  # this buffer's previous batch: its responses are in `out`
+            b.synchronize()
+            pipe_ok.append(bool(np.array_equal(out, ref)))
+        b.load_many(0, qs, slot_keys, vfs, asynchronous=True)
+        b.graph_launch(B, s)
+        b.read_responses_async(B, out, s)
+        state["k"] = k + 1
+
+    for b in bufs:
+        b.synchronize()
+    pipelined_ms = wall(pipelined)
+    for k in range(2):
+        bufs[k].synchronize()
+        pipe_ok.append(bool(np.array_equal(outs[k], ref)))
+    for b in bufs:
+        b.close()
+    for k in keys:
+        k.close()
+    parts = upload_ms + compute_ms + readout_ms
+    hidden = (parts - pipelined_ms) / (upload_ms + readout_ms)
+    return {"workload": f"{steps} batches of {B} queries with fold vectors (query 32 KiB + fold vectors "
+                        f"{shape.fold_words * 4 / 2**20:.1f} MiB up, response {shape.response_words * 4 // 1024} KiB "
+                        "down per query) from pinned host memory, resident keys, hipGraph launches",
+            "upload_ms": upload_ms, "compute_ms": compute_ms, "readout_ms": readout_ms,
+            "upload_gbps": B * (shape.query_words + shape.fold_words) * 4 / (upload_ms * 1e-3) / 1e9,
+            "serialized_ms_per_batch": serialized_ms, "pipelined_ms_per_batch": pipelined_ms,
+            "pipelined_queries_per_s": B * 1000.0 / pipelined_ms,
+            "transfer_hidden": hidden,
+            "definition": "ms per batch from the host clock; transfer_hidden = (upload + compute + readout alone "
+                          "- pipelined) / (upload + readout): the share of the transfer time that overlaps kernels",
+            "responses_equal": ser_ok and all(pipe_ok)}
+
+
 def run_full_path(torch, mods, db, shape, records, B: int, steps: int, stream, q2_bits: int = 20) -> dict:
     """Spiral's whole server path: packed queries (fold vectors converted on the
     device from the expanded query), expansion, database multiply, fold, response
@@ -477,6 +582,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-form", choices=("packed", "vectors"), default="packed",
                     help="serving requests: packed queries with compact responses, or fold-vector queries")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="skip the host<->device pipelining leg (two buffers, transfers overlapping kernels)")
     ap.add_argument("--no-full-path", action="store_true",
                     help="skip the packed-query / compact-response leg (conversion and response packing)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -612,6 +719,10 @@ def main() -> int:
         correct = correct and batched["decoded_all_correct"]
     mods = {"Batch": Batch, "Client": Client, "Database": Database, "Group": Group, "Shape": Shape, "Query": Query,
             "Keys": Keys, "STAGE_EXPAND": STAGE_EXPAND}
+    pipeline = None
+    if args.batch > 0 and not args.no_pipeline:
+        pipeline = run_pipeline(torch, mods, db, shape, records, args.batch, args.batch_steps)
+        correct = correct and pipeline["responses_equal"]
     full_path = None
     if not args.no_full_path:
         full_path = run_full_path(torch, mods, db, shape, records, args.batch, args.batch_steps, stream)
@@ -699,6 +810,7 @@ def main() -> int:
         "group": group,
         "serving": serving,
         "full_path": full_path,
+        "pipeline": pipeline,
         "ingest_s": t_ingest,
         "library": version(),
     }
