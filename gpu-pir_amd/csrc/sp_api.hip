@@ -107,7 +107,7 @@ static void build_tw(uint32_t *out, int m, int inv) {
     out[2 * idx + 1] = shoup_companion(T[idx], q);
   }
   for (int lane = 0; lane < 64; lane++) {
-    const int L = lane >> 1, b = lane & 1;
+    const int L = lane & 31, b = lane >> 5;  // phase-B lane L + 32 b (sp_device.hpp)
     for (int s = 0; s < TW_B_SLOTS; s++) {
       int idx;
       if (s == 0) idx = 32 + L;

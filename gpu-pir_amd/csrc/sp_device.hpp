@@ -9,20 +9,22 @@
 //  * One wavefront (64 lanes) owns one 2048-point transform; each lane holds
 //    32 coefficients in VGPRs.  No workgroup barrier is needed inside a
 //    transform: the single index transpose goes through a wave-private LDS
-//    tile (2112 words: 2 pad words per 64 keep both access patterns free of
-//    bank conflicts with one base address per lane, see tile_at) and the one
-//    cross-lane butterfly stage uses a DPP neighbour swap.
+//    tile (rows of 65 words keep both access patterns free of bank conflicts
+//    with one base address per lane, see tile_at) and the one cross-lane
+//    butterfly stage exchanges register halves between lanes l and l + 32
+//    with v_permlane32_swap (one instruction per register pair, no selects).
 //  * Butterflies are Harvey's lazy Cooley-Tukey / Gentleman-Sande forms with
 //    Shoup multiplication (w, w' = floor(w * 2^32 / q)): 3 integer multiplies
 //    per butterfly.  Forward values live in [0, 4q), inverse in [0, 2q).
 //
 // Index bookkeeping (N = 2048, index bits b10..b0):
 //   phase A   lane l, register i  <->  coefficient 64*i + l
-//   phase B   lane l = 2L + b, register i <-> index 64*L + 2*i + b
-//   NTT side  lane l, register r  <->  NTT position 32*l + r
-// Internal NTT-domain layout "P" stores position z = 32*l + 4*c + e at word
-// 256*c + 4*l + e, so that chunk c of a lane's 32 positions is one coalesced
-// 16-byte load across the wave.
+//   phase B   lane l = L + 32*b, register i <-> index 64*L + 2*i + b
+//   NTT side  lane l, register r  <->  NTT position 32*vlane(l) + r,
+//             vlane(l) = 2*(l mod 32) + l / 32
+// Internal NTT-domain layout "P" stores position z = 32*v + 4*c + e at word
+// 256*c + 4*v + e, so that chunk c of a lane's 32 positions is one coalesced
+// 16-byte load across the wave (lane l takes v = vlane(l)).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -175,16 +177,24 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ uint32_t swap_lane_pair(uint32_t v) {
-  // Value of lane (l ^ 1): DPP quad_perm [1,0,3,2].
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+// The NTT-side lane index: lane l of a wave transform holds NTT positions
+// 32 vlane(l) .. 32 vlane(l) + 31.
+__device__ __forceinline__ int vlane(int lane) { return 2 * (lane & 31) + (lane >> 5); }
+
+// x of lanes 0-31 and y of lanes 32-63 stay; lanes 32-63 of x receive y of
+// lanes 0-31 and lanes 0-31 of y receive x of lanes 32-63 (v_permlane32_swap).
+__device__ __forceinline__ void swap_halves(uint32_t &x, uint32_t &y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+  x = r[0];
+  y = r[1];
 }
 
 // Transpose tile slot of coefficient 64*i + l (phase-A register i, lane l):
-// rows of 66 words.  A phase-A access (fixed i, all lanes) and a phase-B
-// access (lane 2L+b reading row L, column 2i+b) both touch 64 distinct banks,
-// and both are one per-lane base plus a constant offset per register.
-__device__ __forceinline__ int tile_at(int i, int l) { return 66 * i + l; }
+// rows of 65 words.  A phase-A access (fixed i, lanes 0-31 or 32-63) and a
+// phase-B access (lane L + 32b reading row L, column 2i + b) both touch 32
+// distinct banks in each half wave, and both are one per-lane base plus a
+// constant offset per register.
+__device__ __forceinline__ int tile_at(int i, int l) { return 65 * i + l; }
 
 __device__ __forceinline__ uint32_t tw_a(const uint32_t *__restrict__ tw, int idx, int k) {
   return tw[idx * 2 + k];
@@ -212,10 +222,12 @@ __device__ __forceinline__ uint32_t tw_b(const uint32_t *__restrict__ tw, int sl
 // branch in r[0..15] (indices 64 i + lane) and B's in r[16..31]; stages
 // 2-11 then run with branch h's twiddles in both halves (phase A: block blk
 // of stage s takes the twiddle of block (blk mod 2^(s-1)) + h 2^(s-1); phase
-// B: lane l takes the per-lane twiddles of lane (l mod 32) + 32 h).
-// Out: lanes 0-31 hold A's positions 1024 h + 32 lane + j, lanes 32-63 B's
-// positions 1024 h + 32 (lane - 32) + j.  One wave thus transforms two
-// polynomials' halves for the cost of one transform.
+// B: lane l takes the per-lane twiddles of lane l with bit 4 set to h, the
+// lane whose L = l mod 32 lies in half h).
+// Out: lane l = L + 32 b holds virtual positions 64 L + 32 b + j: A's
+// positions 1024 h + 64 L + 32 b + j for L < 16, B's 1024 h + 64 (L - 16) +
+// 32 b + j for L >= 16.  One wave thus transforms two polynomials' halves
+// for the cost of one transform.
 template <int LAZY, bool TAIL = false, bool HALF = false>
 __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                                uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane,
@@ -240,12 +252,12 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
 #pragma unroll
   for (int i = 0; i < 32; i++) lds[tile_at(i, lane)] = r[i];
   wave_lds_sync();
-  const int L = lane >> 1, b = lane & 1;
+  const int L = lane & 31, b = lane >> 5;
 #pragma unroll
   for (int i = 0; i < 32; i++) r[i] = lds[tile_at(L, 2 * i + b)];
   wave_lds_sync();
   // Phase B: strides 32 .. 2 in registers.
-  const int tl = HALF ? ((lane & 31) | (h << 5)) : lane;  // lane whose per-lane twiddles apply
+  const int tl = HALF ? ((lane & ~16) | (h << 4)) : lane;  // lane whose per-lane twiddles apply
   int slot = 0;
 #pragma unroll
   for (int s = 0; s < 5; s++) {
@@ -259,15 +271,15 @@ __device__ __forceinline__ void ntt_fwd_wave_t(uint32_t (&r)[32], const uint32_t
     }
     slot += (1 << s);
   }
-  // Stride 1 crosses the lane pair (2L, 2L+1): each lane keeps 16 of the 32
-  // butterflies after a half exchange and ends holding 32 consecutive outputs.
+  // Stride 1 pairs lanes L and L + 32: after the swap of r[k] (lanes 32-63)
+  // with r[16 + k] (lanes 0-31), lane L + 32 b holds the pair of index
+  // 64 L + 2 (16 b + k) in (r[k], r[16 + k]) and ends holding the 32
+  // consecutive outputs 64 L + 32 b + j.
   uint32_t o[32];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const uint32_t send = b ? r[k] : r[16 + k];
-    const uint32_t recv = swap_lane_pair(send);
-    uint32_t x = b ? recv : r[k];
-    uint32_t y = b ? r[16 + k] : recv;
+    uint32_t x = r[k], y = r[16 + k];
+    swap_halves(x, y);
     ct_stage_bf<LAZY, TAIL>(11, x, y, tw_b(tw, 31 + k, 0, tl), tw_b(tw, 31 + k, 1, tl), q, mu1);
     o[2 * k] = x;
     o[2 * k + 1] = y;
@@ -308,17 +320,18 @@ __device__ __forceinline__ void digits_stage1_half(uint32_t (&r)[32], const uint
 // Out: r[i] = N * coefficient (64*i + lane) mod q, values in [0, 2q).
 __device__ __forceinline__ void ntt_inv_wave(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                              uint32_t *__restrict__ lds, uint32_t q, int lane) {
-  const int L = lane >> 1, b = lane & 1;
+  const int L = lane & 31, b = lane >> 5;
 #pragma unroll
   for (int k = 0; k < 16; k++)
     gs_bf(r[2 * k], r[2 * k + 1], tw_b(tw, 31 + k, 0, lane), tw_b(tw, 31 + k, 1, lane), q);
+  // to phase B: lane L + 32 b' takes index 64 L + 2 i + b' in n[i]
   uint32_t n[32];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const uint32_t send = b ? r[2 * k] : r[2 * k + 1];
-    const uint32_t recv = swap_lane_pair(send);
-    n[k] = b ? recv : r[2 * k];
-    n[16 + k] = b ? r[2 * k + 1] : recv;
+    uint32_t x = r[2 * k], y = r[2 * k + 1];
+    swap_halves(x, y);
+    n[k] = x;
+    n[16 + k] = y;
   }
   // Strides 2 .. 32 in registers.
 #pragma unroll
@@ -392,7 +405,7 @@ __device__ __forceinline__ void gs_stage_lazy(uint32_t &x, uint32_t &y, uint32_t
 template <bool MAD = true>
 __device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint32_t *__restrict__ tw,
                                                   uint32_t *__restrict__ lds, uint32_t q, uint32_t mu1, int lane) {
-  const int L = lane >> 1, b = lane & 1;
+  const int L = lane & 31, b = lane >> 5;
   const uint32_t nq = MAD ? neg_q(q) : 0u;
 #pragma unroll
   for (int k = 0; k < 16; k++)
@@ -400,10 +413,10 @@ __device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint3
   uint32_t n[32];
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const uint32_t send = b ? r[2 * k] : r[2 * k + 1];
-    const uint32_t recv = swap_lane_pair(send);
-    n[k] = b ? recv : r[2 * k];
-    n[16 + k] = b ? r[2 * k + 1] : recv;
+    uint32_t x = r[2 * k], y = r[2 * k + 1];
+    swap_halves(x, y);
+    n[k] = x;
+    n[16 + k] = y;
   }
   // Strides 2 .. 32 in registers (stages 2-6).
   static_for<5>([&](auto ss) {
@@ -440,16 +453,18 @@ __device__ __forceinline__ void ntt_inv_wave_lazy(uint32_t (&r)[32], const uint3
 // ---- internal NTT-domain layout P --------------------------------------
 __device__ __forceinline__ void load_p(uint32_t (&r)[32], const uint32_t *__restrict__ base, int lane) {
   const uint4 *b4 = reinterpret_cast<const uint4 *>(base);
+  const int vl = vlane(lane);
 #pragma unroll
   for (int c = 0; c < 8; c++) {
-    const uint4 v = b4[c * 64 + lane];
+    const uint4 v = b4[c * 64 + vl];
     r[4 * c] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
   }
 }
 __device__ __forceinline__ void store_p(uint32_t *__restrict__ base, const uint32_t (&r)[32], int lane) {
   uint4 *b4 = reinterpret_cast<uint4 *>(base);
+  const int vl = vlane(lane);
 #pragma unroll
-  for (int c = 0; c < 8; c++) b4[c * 64 + lane] = make_uint4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+  for (int c = 0; c < 8; c++) b4[c * 64 + vl] = make_uint4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
 }
 __host__ __device__ __forceinline__ int p_of_z(int z) {
   // natural NTT position -> internal word

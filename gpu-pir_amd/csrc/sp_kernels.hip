@@ -28,13 +28,13 @@ __global__ __launch_bounds__(256) void k_ntt_natural(uint32_t *__restrict__ poly
     if (!INV) {
       load_coeffs(r, p, lane);
       ntt_fwd_wave(r, tw, lds, q, lane);
-      uint4 *o = reinterpret_cast<uint4 *>(p + 32 * lane);
+      uint4 *o = reinterpret_cast<uint4 *>(p + 32 * vlane(lane));
 #pragma unroll
       for (int c = 0; c < 8; c++)
         o[c] = make_uint4(canon4(r[4 * c], q), canon4(r[4 * c + 1], q), canon4(r[4 * c + 2], q),
                           canon4(r[4 * c + 3], q));
     } else {
-      const uint4 *in = reinterpret_cast<const uint4 *>(p + 32 * lane);
+      const uint4 *in = reinterpret_cast<const uint4 *>(p + 32 * vlane(lane));
 #pragma unroll
       for (int c = 0; c < 8; c++) {
         const uint4 v = in[c];
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(128) void k_from_ntt(const uint32_t *__restrict__ n
     if (LP) {
       load_p(r, src, lane);
     } else {
-      const uint4 *in = reinterpret_cast<const uint4 *>(src + 32 * lane);
+      const uint4 *in = reinterpret_cast<const uint4 *>(src + 32 * vlane(lane));
 #pragma unroll
       for (int c = 0; c < 8; c++) {
         const uint4 v = in[c];
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(128) void k_to_ntt(const uint32_t *__restrict__ raw
     for (int i = 0; i < 32; i++) r[i] = raw_mod(src[64 * i + lane], src[N + 64 * i + lane], mc);
     __syncthreads();  // both waves read both planes before either overwrites one
     ntt_fwd_wave(r, tw, lds, mc.q, lane);
-    uint4 *o = reinterpret_cast<uint4 *>(ntt + pi * 2 * N + m * N + 32 * lane);
+    uint4 *o = reinterpret_cast<uint4 *>(ntt + pi * 2 * N + m * N + 32 * vlane(lane));
 #pragma unroll
     for (int c = 0; c < 8; c++)
       o[c] = make_uint4(canon4(r[4 * c], mc.q), canon4(r[4 * c + 1], mc.q), canon4(r[4 * c + 2], mc.q),
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_extprod(const uint32_t *__restrict__ in
 #pragma unroll
       for (int i = 0; i < 32; i++) r[i] = digit_of(hi[64 * i + lane], lo[64 * i + lane], k * bits, mask);
       ntt_fwd_wave(r, tw, lds, q, lane);
-      const uint32_t *k0 = keys + it * key_stride + ((size_t)(m * mx + col) * 2 + 0) * N + 32 * lane;
+      const uint32_t *k0 = keys + it * key_stride + ((size_t)(m * mx + col) * 2 + 0) * N + 32 * vlane(lane);
       const uint32_t *k1 = k0 + N;
 #pragma unroll
       for (int jj = 0; jj < 32; jj++) {
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_extprod(const uint32_t *__restrict__ in
       lds_all[wave * N + 32 * lane + jj] = canon(h ? v0 : v1, q);
     }
     __syncthreads();
-    uint32_t *dst = acc + it * acc_stride + (size_t)(h * 2 + m) * N + 32 * lane;
+    uint32_t *dst = acc + it * acc_stride + (size_t)(h * 2 + m) * N + 32 * vlane(lane);
     const uint32_t *other = lds_all + (wave ^ 1) * N + 32 * lane;
 #pragma unroll
     for (int jj = 0; jj < 32; jj++) {
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(256) void k_expand_action(uint32_t *__restrict__ v,
     if (h) {
       // NTT-domain automorphism: position z of tau(f) is position z' of f,
       // 2*brv(z')+1 = t*(2*brv(z)+1) mod 2N.
-      const uint32_t z = 32 * lane + jj;
+      const uint32_t z = 32 * vlane(lane) + jj;
       const uint32_t e = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
       const uint32_t zs = brv11((e - 1) >> 1);
       x += as_e[m][p_of_z((int)zs)];
@@ -1101,7 +1101,7 @@ __device__ __forceinline__ void store_tile_p_full(uint32_t *__restrict__ tile, c
     uint32_t v[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) v[e] = r[4 * c + e] - __umulhi(r[4 * c + e], mu1) * q;
-    t4[c * 64 + lane] = make_uint4(v[0], v[1], v[2], v[3]);
+    t4[c * 64 + vlane(lane)] = make_uint4(v[0], v[1], v[2], v[3]);
   }
 }
 
@@ -2551,8 +2551,10 @@ void k_expand_fused_h(uint32_t *__restrict__ v, size_t q_stride, int r, int stop
     // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
     ntt_fwd_wave_t<8, true, true>(rr, twm, tiles[wave], mcm.q, mcm.mu1, lane, h);
     // half tiles: digit `wave` at words [0, 1024) of tiles[wave], digit wave + 4
-    // at [1024, 2048); position 1024 h + 32 l' + 4 c + e at word 128 c + 4 l' + e
-    uint4 *t4 = reinterpret_cast<uint4 *>(tiles[wave] + (lane >> 5) * 1024) + (lane & 31);
+    // at [1024, 2048); position 1024 h + 32 l' + 4 c + e at word 128 c + 4 l' + e.
+    // Lane L + 32 b holds virtual positions 64 L + 32 b + j: digit wave + 4 for
+    // L >= 16, at l' = 2 (L mod 16) + b.
+    uint4 *t4 = reinterpret_cast<uint4 *>(tiles[wave] + ((lane >> 4) & 1) * 1024) + (2 * (lane & 15) + (lane >> 5));
 #pragma unroll
     for (int c = 0; c < 8; c++) t4[c * 32] = make_uint4(rr[4 * c], rr[4 * c + 1], rr[4 * c + 2], rr[4 * c + 3]);
   }
@@ -3483,9 +3485,10 @@ __global__ __launch_bounds__(NCB * 128) void k_from_ntt_pm(const uint32_t *__res
   const ModConst &mc = m ? mc1 : mc0;
   const uint4 *src = reinterpret_cast<const uint4 *>(outp + outp_word(m, 0, t, i, c, T, I, C));
   uint32_t r[32];
+  const int vl = vlane(lane);
 #pragma unroll
-  for (int k = 0; k < 8; k++) {  // position chunk 64 k + lane: words 4 (64 k + lane) .. + 3
-    const uint4 v = src[(size_t)(64 * k + lane) * C];
+  for (int k = 0; k < 8; k++) {  // position chunk 64 k + vl: words 4 (64 k + vl) .. + 3
+    const uint4 v = src[(size_t)(64 * k + vl) * C];
     r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
   }
   uint32_t *xm = tiles[w];
