@@ -9,7 +9,7 @@
 //  * One wavefront (64 lanes) owns one 2048-point transform; each lane holds
 //    32 coefficients in VGPRs.  No workgroup barrier is needed inside a
 //    transform: the single index transpose goes through a wave-private LDS
-//    tile (rows of 65 words keep both access patterns free of bank conflicts
+//    tile (rows of 66 words keep both access patterns free of bank conflicts
 //    with one base address per lane, see tile_at) and the one cross-lane
 //    butterfly stage exchanges register halves between lanes l and l + 32
 //    with v_permlane32_swap (one instruction per register pair, no selects).
@@ -190,11 +190,12 @@ __device__ __forceinline__ void swap_halves(uint32_t &x, uint32_t &y) {
 }
 
 // Transpose tile slot of coefficient 64*i + l (phase-A register i, lane l):
-// rows of 65 words.  A phase-A access (fixed i, lanes 0-31 or 32-63) and a
-// phase-B access (lane L + 32b reading row L, column 2i + b) both touch 32
-// distinct banks in each half wave, and both are one per-lane base plus a
-// constant offset per register.
-__device__ __forceinline__ int tile_at(int i, int l) { return 65 * i + l; }
+// rows of 66 words.  A phase-A access (fixed i: 66 i + l) and a phase-B access
+// (lane L + 32b reading row L, column 2i + b: 66 L + 2 i + b) both touch 64
+// distinct banks (2 L + b distinct mod 64; a 65-word row gives L + b, and the
+// counters then show 1.3 conflict cycles per LDS instruction), and both are
+// one per-lane base plus a constant offset per register.
+__device__ __forceinline__ int tile_at(int i, int l) { return 66 * i + l; }
 
 __device__ __forceinline__ uint32_t tw_a(const uint32_t *__restrict__ tw, int idx, int k) {
   return tw[idx * 2 + k];
