@@ -30,6 +30,9 @@ HBM_PEAK_GBPS = 8000.0
 # --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy kernel's readings (FETCH
 # x2.000, WRITE x1.000); profiles/db_mul_hbm_traffic_nu9_6.md.  Measured shapes only.
 DB_MUL_TRAFFIC_BYTES = {(9, 6): (1851516 + 8192) * 1024}
+# HBM bytes per 64-query batch of the expansion stage (k_expand_digits, k_expand_digits2,
+# k_expand_update), from the same two counter runs; profiles/db_mul_hbm_traffic_nu9_6.md.
+EXPANSION_TRAFFIC_BYTES = {(9, 6, 64): 12.36e9}
 
 
 def db_mul_algorithmic_bytes(shape, packed: bool = False) -> int:
@@ -204,7 +207,8 @@ def run_batched(torch, Batch, Client, db, shape, records, B: int, steps: int, st
     roofline = {"kernel": "expansion rounds (k_expand_digits / k_expand_digits2 + k_expand_update, "
                           f"{shape.g} rounds, {B} queries)",
                 "bound": "valu", "achieved": ex["achieved"], "peak": ex["peak"], "unit": ex["unit"],
-                "frac": ex["frac"], "traffic": None,
+                "frac": ex["frac"], "traffic": EXPANSION_TRAFFIC_BYTES.get((shape.nu1, shape.nu2, B)),
+                "traffic_unit": "HBM bytes per batch (counters), the expansion array rewritten every round",
                 "share_of_batch": stage_ms["expand"] / sum(stage_ms.values()),
                 "definition": f"{ex['work']} ({expansion_steps(shape)} working steps x 18 transforms: 2 inverse "
                               "and 16 digit transforms per step) over the stage's time from HIP events around one "
