@@ -2612,12 +2612,13 @@ int expand_fused_blocks(int r, int stop, int max_bits) {
 
 hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
-                               uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s) {
+                               uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s,
+                               bool live_only) {
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
   // Children of skipped elements only: at stop_round the round plan's extra
   // elements are children of working elements, made by the previous epilogue.
-  const int neg1_only = (stop > 0 && r > stop) ? (1 << r) / 2 : 0;
+  const int neg1_only = (stop > 0 && r > stop && !live_only) ? (1 << r) / 2 : 0;
   const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
   const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
   const uint32_t *s_in = snap + (size_t)(r & 1) * s_round;
@@ -2631,11 +2632,11 @@ hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int 
 hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                  const uint32_t *left, const uint32_t *right, size_t left_qstride,
                                  size_t right_qstride, uint32_t *snap, size_t s_qstride, size_t s_round,
-                                 const DevTables &t, hipStream_t s) {
+                                 const DevTables &t, hipStream_t s, bool live_only) {
   if ((N >> r) < 4) return hipErrorInvalidValue;  // tau_t must keep each position half (t = 1 mod 4)
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
-  const int neg1_only = (stop > 0 && r > stop) ? (1 << r) / 2 : 0;
+  const int neg1_only = (stop > 0 && r > stop && !live_only) ? (1 << r) / 2 : 0;
   const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
   const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
   const uint32_t *s_in = snap + (size_t)(r & 1) * s_round;
@@ -2647,9 +2648,11 @@ hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, in
 }
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
-                               uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s) {
+                               uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s,
+                               bool live_only) {
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
+  if (live_only) extra = 0;  // the children of skipped elements feed nothing the answer reads
   const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
   const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
   // From round 1 on, the previous round's update (or latency-form) launch has

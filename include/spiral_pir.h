@@ -227,6 +227,13 @@ int sp_db_build_limbs(sp_db *db, size_t *limb_bytes);
  *                   2 = database multiply, 4 = from_ntt + fold.  Stages can
  *                   run one at a time; stage 1 restarts from the stored query
  *                   each time, so a run can be repeated without reloading.
+ *                   A run of stage 1 alone forms every element of the
+ *                   expanded v (expansion.cuh:87-166); a run of stages 1 and
+ *                   2 together leaves out elements nothing after the
+ *                   expansion reads: the odd i >= 2^(stop_round + 1) (v_neg1
+ *                   products of elements skipped past stop_round; every such
+ *                   i has i / 2 >= nu2 * t_gsw), which keep their previous
+ *                   contents.
  *   sp_query_read   synchronizes, then copies out: what = 0 the n2 response
  *                   ciphertexts (raw), 1 the expanded v (NTT, natural order),
  *                   2 the database-multiply accumulators (NTT, natural order,

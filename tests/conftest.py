@@ -92,3 +92,14 @@ def rand_raw(g: np.random.Generator, n: int, include_q: bool = False) -> np.ndar
     out[:, 0] = (x >> np.uint64(32)).astype(np.uint32)
     out[:, 1] = (x & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     return out
+
+
+def live_elements(oracle, nu1: int, nu2: int, t_gsw: int) -> np.ndarray:
+    """Mask of the expanded elements a whole answer forms: every element but the odd
+    i >= 2^(stop_round + 1), the v_neg1 products of elements skipped past stop_round
+    (gpu-pir/expansion.cuh:92-97), which no step, selector or GSW slot reads."""
+    g, st = oracle.expansion_g(nu1, nu2, t_gsw), oracle.expansion_stop_round(nu2, t_gsw)
+    i = np.arange(1 << g)
+    dead = (i % 2 == 1) & (i >= (2 << st))
+    assert np.all((i[dead] >> 1) >= nu2 * t_gsw)  # no GSW slot among them
+    return ~dead
