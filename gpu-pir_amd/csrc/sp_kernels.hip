@@ -2350,10 +2350,12 @@ void k_expand_digits2(uint32_t *__restrict__ v, size_t q_stride, int r, int stop
 
 // k_expand_update: one 8-wave block per (working step b, query, residue m):
 // wave k transforms digit k of the step for residue m; then every thread
-// updates residue m of v_i at its 4 positions (expand_update).  Grid x holds
-// (b, m) for 16-block groups of 8 steps, residue 0 then 1: the two blocks of a
-// step are 8 apart, so they run on the same XCD at nearly the same time and
-// the second reads the step's digit bytes from that XCD's L2 instead of HBM.
+// updates residue m of v_i at its 4 positions (expand_update).  Grid x (2 work
+// blocks) holds (b, m) for 16-block groups of 8 steps, residue 0 then 1: the
+// two blocks of a step are 8 apart, so they run on the same XCD at nearly the
+// same time and the second reads the step's digit bytes from that XCD's L2
+// instead of HBM.  Rounds of fewer than 8 steps (work 2 or 4) take the two
+// residues of a step side by side, so their few blocks spread over all XCDs.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work,
                      const uint32_t *__restrict__ neg1_next, const uint32_t *__restrict__ dig, size_t dig_qstride,
@@ -2361,8 +2363,8 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
                      size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
   __shared__ uint32_t tiles[8][LDS_TILE];
   const int id = blockIdx.x, half = 1 << r;
-  const int b = (id & 7) | ((id >> 4) << 3), m = (id >> 3) & 1;
-  if (b >= work) return;  // uniform per block: the grid rounds the steps up to a multiple of 8
+  const bool groups = (work & 7) == 0;
+  const int b = groups ? (id & 7) | ((id >> 4) << 3) : id >> 1, m = groups ? (id >> 3) & 1 : id & 1;
   const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -2677,7 +2679,7 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_expand_update<<<dim3(2 * ((work + 7) / 8 * 8), n_queries), 512, 0, s>>>(
+  k_expand_update<<<dim3(2 * work, n_queries), 512, 0, s>>>(
       v, q_stride, r, stop, max_bits, work, neg1_next, dig, dig_qstride, left, right, left_qstride, right_qstride, t.tw,
       t.mc[0], t.mc[1]);
   return hipGetLastError();
