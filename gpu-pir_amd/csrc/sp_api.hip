@@ -512,7 +512,8 @@ struct ExpandSnap {
 static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_queries, int nu1, int nu2, int t_gsw,
                          int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t left_qstride,
                          size_t right_qstride, int rounds, uint32_t *dig, size_t dig_qstride, const ExpandSnap &snap,
-                         const uint32_t *query, size_t query_qstride, hipStream_t s, bool live_only = false) {
+                         const uint32_t *query, size_t query_qstride, hipStream_t s, bool live_only = false,
+                         const int *odd_live = nullptr) {
   int g = 0, stop = 0;
   int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
   if (rc) return rc;
@@ -552,10 +553,11 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
     if (r < halves)
       HIP_TRY(launch_expand_fused_h(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2, lr, rr, left_qstride,
                                     right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s,
-                                    live_only));
+                                    live_only, odd_live));
     else
       HIP_TRY(launch_expand_fused(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2, lr, rr, left_qstride,
-                                  right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s, live_only));
+                                  right_qstride, snap.p, snap.q_stride, snap.set_stride, ctx->tables, s, live_only,
+                                  odd_live));
   }
   for (int r = fused; r < rounds; r++) {
     const int num_in = 1 << r;
@@ -564,7 +566,7 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
     const uint32_t *rr = right_p + (size_t)std::min(r, stop) * key_level;
     if (family == 4) {
       HIP_TRY(launch_expand_split(v, q_stride, n_queries, r, rounds, stop, t_gsw * nu2, lr, rr, left_qstride,
-                                  right_qstride, dig, dig_qstride, ctx->tables, s, live_only));
+                                  right_qstride, dig, dig_qstride, ctx->tables, s, live_only, odd_live));
       continue;
     }
     HIP_TRY(launch_expand_neg1(v, q_stride, n_queries, num_in, neg1_r, ctx->tables, s));
@@ -1352,7 +1354,8 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
     snap.q_stride = ((size_t)1 << b->g) * 2 * N;
     snap.set_stride = (size_t)snap.queries * snap.q_stride;
     int rc = run_expansion(ctx, b->v, b->v_q, n, db->nu1, db->nu2, b->t_gsw, b->t_exp, b->left, b->right, b->l_q,
-                           b->r_q, -1, b->dig, b->dig_q, snap, b->query_ct, 4 * N, s, (stages & 2) != 0);
+                           b->r_q, -1, b->dig, b->dig_q, snap, b->query_ct, 4 * N, s, (stages & 2) != 0,
+                           b->d_packed);
     if (rc) return rc;
     if (packed)
       HIP_TRY(launch_regev_to_gsw(true, b->v, b->v_q, n, db->nu2, b->t_gsw, b->conv, b->conv_q, b->fkeys, b->f_q,

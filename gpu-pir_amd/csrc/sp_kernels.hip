@@ -2149,6 +2149,7 @@ __device__ __forceinline__ void neg1_rows4(const uint32_t *__restrict__ src, uin
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
                                                        int work, int odd_start, int children_formed,
+                                                       const int *__restrict__ odd_live,
                                                        const uint32_t *__restrict__ neg1_r,
                                                        uint32_t *__restrict__ dig, size_t dig_qstride,
                                                        const uint32_t *__restrict__ tw_base, ModConst mc0,
@@ -2158,6 +2159,7 @@ void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   const bool step = b < work;
   const int i = !step ? odd_start + 2 * (b - work) + 1
                       : ((stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b);
+  if (odd_live && (i & 1) && !odd_live[blockIdx.y]) return;  // uniform per block: an odd step nothing reads
   const int tid = threadIdx.x;
   const int m = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const ModConst mc = m ? mc1 : mc0;
@@ -2239,7 +2241,8 @@ void k_expand_digits(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
 // the update kernel's order (from stop_round on, even steps first).
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_digits2(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int npair,
-                      int odd_start, int children_formed, const uint32_t *__restrict__ neg1_r,
+                      int odd_start, int children_formed, const int *__restrict__ odd_live,
+                      const uint32_t *__restrict__ neg1_r,
                       uint32_t *__restrict__ dig, size_t dig_qstride, const uint32_t *__restrict__ tw_base,
                       ModConst mc0, ModConst mc1) {
   __shared__ uint32_t tiles[2][LDS_TILE];
@@ -2276,6 +2279,7 @@ void k_expand_digits2(uint32_t *__restrict__ v, size_t q_stride, int r, int stop
     return;
   }
   const int p = split ? (b < half / 2 ? 2 * b : 2 * (b - half / 2) + 1) : b;
+  if (odd_live && (p & 1) && !odd_live[blockIdx.y]) return;  // an odd parent and child nothing reads
   const int c = p + half;
   const bool work_c = !expand_skipped(c, r, stop, max_bits);
   if (work_c && !children_formed) form_child(c);  // round 0: nothing formed v_1 yet
@@ -2358,7 +2362,8 @@ void k_expand_digits2(uint32_t *__restrict__ v, size_t q_stride, int r, int stop
 // residues of a step side by side, so their few blocks spread over all XCDs.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work,
-                     const uint32_t *__restrict__ neg1_next, const uint32_t *__restrict__ dig, size_t dig_qstride,
+                     const int *__restrict__ odd_live, const uint32_t *__restrict__ neg1_next,
+                     const uint32_t *__restrict__ dig, size_t dig_qstride,
                      const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
                      size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
   __shared__ uint32_t tiles[8][LDS_TILE];
@@ -2366,6 +2371,7 @@ void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop,
   const bool groups = (work & 7) == 0;
   const int b = groups ? (id & 7) | ((id >> 4) << 3) : id >> 1, m = groups ? (id >> 3) & 1 : id & 1;
   const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  if (odd_live && (i & 1) && !odd_live[blockIdx.y]) return;  // uniform per block: an odd step nothing reads
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
@@ -2444,7 +2450,8 @@ __global__ __launch_bounds__(256) void k_expand_prep(const uint32_t *__restrict_
 
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_fused(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work, int last,
-                    const uint32_t *__restrict__ neg1_r, const uint32_t *__restrict__ neg1_next,
+                    const int *__restrict__ odd_live, const uint32_t *__restrict__ neg1_r,
+                    const uint32_t *__restrict__ neg1_next,
                     const uint32_t *__restrict__ s_in, uint32_t *__restrict__ s_out, size_t s_qstride,
                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
                     size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
@@ -2461,6 +2468,7 @@ void k_expand_fused(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, 
     return;
   }
   const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  if (odd_live && (i & 1) && !odd_live[blockIdx.y]) return;  // uniform per block: an odd step nothing reads
   uint32_t *vi = vq + (size_t)i * 4 * N;
   const uint32_t t = (uint32_t)(N >> r) + 1u;
   const bool use_left = (r != 0) && ((i & 1) == 0);
@@ -2510,7 +2518,8 @@ void k_expand_fused(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, 
 // keeps rounds with t = 3 (mod 4) on the other forms.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_fused_h(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work, int last,
-                      const uint32_t *__restrict__ neg1_r, const uint32_t *__restrict__ neg1_next,
+                      const int *__restrict__ odd_live, const uint32_t *__restrict__ neg1_r,
+                      const uint32_t *__restrict__ neg1_next,
                       const uint32_t *__restrict__ s_in, uint32_t *__restrict__ s_out, size_t s_qstride,
                       const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
                       size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
@@ -2529,6 +2538,7 @@ void k_expand_fused_h(uint32_t *__restrict__ v, size_t q_stride, int r, int stop
     return;
   }
   const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  if (odd_live && (i & 1) && !odd_live[blockIdx.y]) return;  // uniform per block: an odd step nothing reads
   uint32_t *vi = vq + (size_t)i * 4 * N;
   const uint32_t t = (uint32_t)(N >> r) + 1u;
   const bool use_left = (r != 0) && ((i & 1) == 0);
@@ -2615,7 +2625,7 @@ int expand_fused_blocks(int r, int stop, int max_bits) {
 hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s,
-                               bool live_only) {
+                               bool live_only, const int *odd_live) {
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
   // Children of skipped elements only: at stop_round the round plan's extra
@@ -2626,7 +2636,7 @@ hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int 
   const uint32_t *s_in = snap + (size_t)(r & 1) * s_round;
   uint32_t *s_out = snap + (size_t)((r + 1) & 1) * s_round;
   k_expand_fused<<<dim3(work + neg1_only, n_queries, 2), 512, 0, s>>>(
-      v, q_stride, r, stop, max_bits, work, r + 1 >= end, neg1_r, neg1_next, s_in, s_out, s_qstride, left, right,
+      v, q_stride, r, stop, max_bits, work, r + 1 >= end, live_only ? odd_live : nullptr, neg1_r, neg1_next, s_in, s_out, s_qstride, left, right,
       left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
@@ -2634,7 +2644,7 @@ hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int 
 hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                  const uint32_t *left, const uint32_t *right, size_t left_qstride,
                                  size_t right_qstride, uint32_t *snap, size_t s_qstride, size_t s_round,
-                                 const DevTables &t, hipStream_t s, bool live_only) {
+                                 const DevTables &t, hipStream_t s, bool live_only, const int *odd_live) {
   if ((N >> r) < 4) return hipErrorInvalidValue;  // tau_t must keep each position half (t = 1 mod 4)
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
@@ -2644,17 +2654,18 @@ hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, in
   const uint32_t *s_in = snap + (size_t)(r & 1) * s_round;
   uint32_t *s_out = snap + (size_t)((r + 1) & 1) * s_round;
   k_expand_fused_h<<<dim3(work + neg1_only, n_queries, 4), 256, 0, s>>>(
-      v, q_stride, r, stop, max_bits, work, r + 1 >= end, neg1_r, neg1_next, s_in, s_out, s_qstride, left, right,
+      v, q_stride, r, stop, max_bits, work, r + 1 >= end, live_only ? odd_live : nullptr, neg1_r, neg1_next, s_in, s_out, s_qstride, left, right,
       left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s,
-                               bool live_only) {
+                               bool live_only, const int *odd_live) {
   int work, odd_start, extra;
   round_plan(r, stop, max_bits, work, odd_start, extra);
   if (live_only) extra = 0;  // the children of skipped elements feed nothing the answer reads
+  if (!live_only) odd_live = nullptr;
   const uint32_t *neg1_r = t.neg1 + (size_t)r * 2 * 2 * N;
   const uint32_t *neg1_next = r + 1 < end ? t.neg1 + (size_t)(r + 1) * 2 * 2 * N : nullptr;
   // From round 1 on, the previous round's update (or latency-form) launch has
@@ -2670,17 +2681,17 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
     const int half = 1 << r;  // parents whose pair block forms the digits (see k_expand_digits2)
     const int npair = (stop > 0 && r >= stop) ? half / 2 + (r == stop ? std::min(half / 2, max_bits) : 0) : half;
     k_expand_digits2<<<dim3(npair + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, max_bits, npair,
-                                                                    odd_start, r > 0 ? 1 : 0, neg1_r, dig,
-                                                                    dig_qstride, t.tw, t.mc[0], t.mc[1]);
+                                                                    odd_start, r > 0 ? 1 : 0, odd_live, neg1_r,
+                                                                    dig, dig_qstride, t.tw, t.mc[0], t.mc[1]);
   } else {
     k_expand_digits<<<dim3(work + extra, n_queries), 128, 0, s>>>(v, q_stride, r, stop, work, odd_start,
-                                                                  r > 0 ? 1 : 0, neg1_r, dig, dig_qstride, t.tw,
-                                                                  t.mc[0], t.mc[1]);
+                                                                  r > 0 ? 1 : 0, odd_live, neg1_r, dig, dig_qstride,
+                                                                  t.tw, t.mc[0], t.mc[1]);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_expand_update<<<dim3(2 * work, n_queries), 512, 0, s>>>(
-      v, q_stride, r, stop, max_bits, work, neg1_next, dig, dig_qstride, left, right, left_qstride, right_qstride, t.tw,
+      v, q_stride, r, stop, max_bits, work, odd_live, neg1_next, dig, dig_qstride, left, right, left_qstride, right_qstride, t.tw,
       t.mc[0], t.mc[1]);
   return hipGetLastError();
 }

@@ -124,22 +124,25 @@ int expand_fused_blocks(int r, int stop, int max_bits);
 hipError_t launch_expand_fused(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *snap, size_t s_qstride, size_t s_round, const DevTables &t, hipStream_t s,
-                               bool live_only = false);
+                               bool live_only = false, const int *odd_live = nullptr);
 // The latency form in half-position blocks (4-wave blocks per step, residue
 // and half of the NTT positions); rounds with (N >> r) >= 4 only.
 hipError_t launch_expand_fused_h(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                  const uint32_t *left, const uint32_t *right, size_t left_qstride,
                                  size_t right_qstride, uint32_t *snap, size_t s_qstride, size_t s_round,
-                                 const DevTables &t, hipStream_t s, bool live_only = false);
+                                 const DevTables &t, hipStream_t s, bool live_only = false,
+                                 const int *odd_live = nullptr);
 // Round r of a run of `end` rounds in the split form (digits, then per-residue
 // transforms and update; the update also forms round r + 1's children).
 // live_only (all three forms): the odd elements past stop_round that no later
 // step, selector or GSW slot reads (the v_neg1 children of skipped elements)
-// are not formed; they stay as they were.
+// are not formed; they stay as they were.  With live_only, odd_live (one int
+// per query, may be NULL) marks the queries whose odd elements are read (the
+// GSW slots of packed queries): the odd steps of the others are not run.
 hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int r, int end, int stop, int max_bits,
                                const uint32_t *left, const uint32_t *right, size_t left_qstride, size_t right_qstride,
                                uint32_t *dig, size_t dig_qstride, const DevTables &t, hipStream_t s,
-                               bool live_only = false);
+                               bool live_only = false, const int *odd_live = nullptr);
 
 // int8-limb MFMA database multiply (C >= 32, R multiple of 32).
 //   D: database limbs [t][m][p][R/32][4][C][32] (k_db_to_limbs, one-time)

@@ -233,7 +233,9 @@ int sp_db_build_limbs(sp_db *db, size_t *limb_bytes);
  *                   expansion reads: the odd i >= 2^(stop_round + 1) (v_neg1
  *                   products of elements skipped past stop_round; every such
  *                   i has i / 2 >= nu2 * t_gsw), which keep their previous
- *                   contents.
+ *                   contents, and for a query loaded with fold vectors every
+ *                   odd element (only the conversion of packed queries reads
+ *                   them), whose contents are then unspecified.
  *   sp_query_read   synchronizes, then copies out: what = 0 the n2 response
  *                   ciphertexts (raw), 1 the expanded v (NTT, natural order),
  *                   2 the database-multiply accumulators (NTT, natural order,

@@ -421,15 +421,16 @@ def test_dbmul_mfma_two_row_tiles_against_oracle(gpu_ctx, oracle, force_dbmul):
 
 @pytest.mark.parametrize("nu1,nu2,B", [(6, 2, 2), (6, 2, 20), (7, 5, 40)])
 def test_full_answer_forms_only_live_elements(gpu_ctx, oracle, nu1, nu2, B):
-    """A whole answer (all stages in one run) forms every expanded element that the
-    multiply, the conversion or a later step reads, and leaves the odd i >=
-    2^(stop_round + 1) (v_neg1 products of elements skipped past stop_round) as
-    they were: after an expansion-only run of other queries, a full run of new
-    ones holds the oracle's expansion of the new queries on the live elements and
-    the earlier values on the rest; every response decodes.  B = 2 takes the
-    latency forms for the leading rounds, B = 20 the split form throughout, B = 40
-    at (7, 5) the parent-and-child digits kernel in the last round."""
-    from conftest import live_elements
+    """A whole answer (all stages in one run) of queries with fold vectors forms
+    every even expanded element (the selectors and their ancestors) exactly as the
+    oracle does, leaves the odd i >= 2^(stop_round + 1) (v_neg1 products of elements
+    skipped past stop_round) as they were, and does not run the odd steps (only the
+    conversion of packed queries reads odd elements): after an expansion-only run
+    of other queries, a full run of new ones holds the oracle's expansion of the new
+    queries on the even elements and the earlier values on those odd ones, and every
+    response decodes.  B = 2 takes the latency forms for the leading rounds, B = 20
+    the split form throughout, B = 40 at (7, 5) the parent-and-child digits kernel
+    in the last round."""
     from spiral_pir.server import Batch, READ_EXPANDED, STAGE_EXPAND, STAGES_ALL
 
     s, recs, db = _setup(gpu_ctx, nu1, nu2, 1500 + nu1 + B)
@@ -444,14 +445,16 @@ def test_full_answer_forms_only_live_elements(gpu_ctx, oracle, nu1, nu2, B):
     for slot, (c, idx, q, left, right, vf) in enumerate(new):
         b.load(slot, q, left, right, vf)
     b.run(B, STAGES_ALL)
-    live = live_elements(oracle, nu1, nu2, s.t_gsw)
-    assert (~live).any()
+    i = np.arange(1 << s.g)
+    even = i % 2 == 0
+    untouched = ~even & (i >= (2 << oracle.expansion_stop_round(nu2, s.t_gsw)))
+    assert untouched.any()
     for slot in slots:
         c, idx, q, left, right, vf = new[slot]
         v = b.read(slot, READ_EXPANDED)
         exp = oracle.expand(q, nu1, nu2, s.t_gsw, s.t_exp, left, right)
-        np.testing.assert_array_equal(v[live], exp[live], err_msg=f"slot {slot} live elements")
-        np.testing.assert_array_equal(v[~live], prev[slot][~live], err_msg=f"slot {slot} elements left as they were")
+        np.testing.assert_array_equal(v[even], exp[even], err_msg=f"slot {slot} even elements")
+        np.testing.assert_array_equal(v[untouched], prev[slot][untouched], err_msg=f"slot {slot} elements left as they were")
     resp = b.responses(B)
     for slot, (c, idx, q, left, right, vf) in enumerate(new):
         np.testing.assert_array_equal(c.decode(resp[slot]), recs[idx], err_msg=f"slot {slot}")
