@@ -513,7 +513,7 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
                          int t_exp, const uint32_t *left_p, const uint32_t *right_p, size_t left_qstride,
                          size_t right_qstride, int rounds, uint32_t *dig, size_t dig_qstride, const ExpandSnap &snap,
                          const uint32_t *query, size_t query_qstride, hipStream_t s, bool live_only = false,
-                         const int *odd_live = nullptr) {
+                         const int *odd_live = nullptr, bool even_only = false) {
   int g = 0, stop = 0;
   int rc = sp_expansion_depth(nu1, nu2, t_gsw, &g, &stop);
   if (rc) return rc;
@@ -528,14 +528,26 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
   // round 6 (448 half blocks) 0.7 us slower; round 6 in the 8-wave form (224
   // blocks) faster than split; round 7 (384 blocks) slower in one launch.
   // Variant 10 + R forces the half-position form for the first R rounds (those
-  // with N >> r >= 4), then the split form.
+  // with N >> r >= 4), then the split form.  For the 8-wave form the blocks
+  // counted are those that work: a live-only run forms no children of skipped
+  // elements, and one whose queries all carry fold vectors (even_only) runs
+  // only the even steps, 2^r per round.  At (9,6), one query with fold vectors:
+  // rounds 0-5 in halves, rounds 6-7 in the 8-wave form, rounds 8-9 split
+  // (583.4 us per answer, against 585.6 us with round 7 split and 587.0 us
+  // with rounds 6 (whose grid half returns at once) in halves as well).
   const int var = ctx->variant[SP_STAGE_EXPAND];
   int halves = 0, fused = 0;  // rounds in the half-position form, in either latency form
   if (family == 4 && snap.p && n_queries <= snap.queries) {
     const long cu = ctx->tables.n_cu;
-    auto blocks = [&](int r) { return (long)expand_fused_blocks(r, stop, t_gsw * nu2) * n_queries; };
+    auto blocks = [&](int r) {
+      const long steps = even_only   ? (1L << r)
+                         : live_only ? expand_split_work(r, stop, t_gsw * nu2)
+                                     : expand_fused_blocks(r, stop, t_gsw * nu2);
+      return steps * n_queries;
+    };
+    auto blocks_all = [&](int r) { return (long)expand_fused_blocks(r, stop, t_gsw * nu2) * n_queries; };
     if (var == 0) {
-      while (halves < rounds && (N >> halves) >= 4 && 4 * blocks(halves) <= cu) halves++;
+      while (halves < rounds && (N >> halves) >= 4 && 4 * blocks_all(halves) <= cu) halves++;
       fused = halves;
       while (fused < rounds && 2 * blocks(fused) <= cu) fused++;
     } else if (var >= 10) {
@@ -1355,7 +1367,7 @@ static int batch_enqueue(sp_batch *b, int n, int stages, hipStream_t s) {
     snap.set_stride = (size_t)snap.queries * snap.q_stride;
     int rc = run_expansion(ctx, b->v, b->v_q, n, db->nu1, db->nu2, b->t_gsw, b->t_exp, b->left, b->right, b->l_q,
                            b->r_q, -1, b->dig, b->dig_q, snap, b->query_ct, 4 * N, s, (stages & 2) != 0,
-                           b->d_packed);
+                           b->d_packed, (stages & 2) != 0 && !packed);
     if (rc) return rc;
     if (packed)
       HIP_TRY(launch_regev_to_gsw(true, b->v, b->v_q, n, db->nu2, b->t_gsw, b->conv, b->conv_q, b->fkeys, b->f_q,
