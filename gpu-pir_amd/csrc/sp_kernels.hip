@@ -1941,35 +1941,35 @@ __device__ __forceinline__ void expand_digits(const uint32_t *__restrict__ vi, u
 // of v_i (< q) and, on row 1, the gathered e term (< q) are added to it (< 7q
 // < 8q), and one reduction (conditional subtractions of 4q, 2q, q) brings the
 // sum to [0, q).  Precondition: v_i, and so the gathered e term, canonical.
-// The e-term gather of expand_update: tau(as_e) at this thread's positions.
+// The e-term gather of expand_update: tau(as_e) at this thread's positions,
+// from row 1 of residue m of v_i (`row`, layout P, in global memory or LDS).
 template <int PPT>
-__device__ __forceinline__ void expand_gather_e(uint32_t (&ase)[PPT], const uint32_t *__restrict__ vi, int m, int p0,
-                                                uint32_t t) {
+__device__ __forceinline__ void expand_gather_e(uint32_t (&ase)[PPT], const uint32_t *row, int p0, uint32_t t) {
 #pragma unroll
   for (int e = 0; e < PPT; e++) {
     const uint32_t z = (uint32_t)z_of_p(p0 + e);
     const uint32_t ev = ((2 * brv11(z) + 1) * t) & (2 * N - 1);
-    ase[e] = vi[(size_t)(2 + m) * N + p_of_z((int)brv11((ev - 1) >> 1))];
+    ase[e] = row[p_of_z((int)brv11((ev - 1) >> 1))];
   }
 }
 
 // Key words of digits k < KP (rows 0 and 1) at this thread's 4 positions,
-// loaded ahead of the digit transforms by callers that have the registers.
+// loaded ahead of the digit transforms by callers that have the registers,
+// and the gathered e term (filled by the caller).
 template <int KP>
 struct ExpandKeys {
   uint4 k[KP > 0 ? KP : 1][2];
   uint32_t ase[4];
 };
 template <int KP>
-__device__ __forceinline__ void expand_prefetch(ExpandKeys<KP> &pk, const uint32_t *__restrict__ vi,
-                                                const uint32_t *__restrict__ W, int m, int p0, uint32_t t) {
+__device__ __forceinline__ void expand_prefetch_keys(ExpandKeys<KP> &pk, const uint32_t *__restrict__ W, int m,
+                                                     int p0) {
 #pragma unroll
   for (int k = 0; k < KP; k++) {
     const uint32_t *k0 = W + ((size_t)(m * 8 + k) * 2) * N + p0;
     pk.k[k][0] = *reinterpret_cast<const uint4 *>(k0);
     pk.k[k][1] = *reinterpret_cast<const uint4 *>(k0 + N);
   }
-  expand_gather_e<4>(pk.ase, vi, m, p0, t);
 }
 
 // tile_of(k): this thread's words of digit k's transform in LDS (PPT of them);
@@ -1980,11 +1980,11 @@ __device__ __forceinline__ void expand_update_tk(uint32_t *__restrict__ vi, Tile
                                                  const ExpandKeys<KP> *pk = nullptr) {
   static_assert(KP == 0 || PPT == 4, "prefetched keys cover 4 positions");
   uint32_t ase[PPT];
-  if (KP > 0 || pk) {  // gathered ahead by the caller (4 positions)
+  if (pk) {  // gathered ahead by the caller (4 positions)
 #pragma unroll
     for (int e = 0; e < PPT; e++) ase[e] = pk->ase[e & 3];
   } else {
-    expand_gather_e<PPT>(ase, vi, m, p0, t);
+    expand_gather_e<PPT>(ase, vi + (size_t)(2 + m) * N, p0, t);
   }
   uint64_t acc[2][PPT];
 #pragma unroll
@@ -2360,37 +2360,54 @@ void k_expand_digits2(uint32_t *__restrict__ v, size_t q_stride, int r, int stop
 // same time and the second reads the step's digit bytes from that XCD's L2
 // instead of HBM.  Rounds of fewer than 8 steps (work 2 or 4) take the two
 // residues of a step side by side, so their few blocks spread over all XCDs.
+// The digit bytes (and row 1 for the e term) are requested first and the
+// keys only once those are in: a wave's loads complete in issue order, so the
+// transforms wait for no key load.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 void k_expand_update(uint32_t *__restrict__ v, size_t q_stride, int r, int stop, int max_bits, int work,
                      const int *__restrict__ odd_live, const uint32_t *__restrict__ neg1_next,
                      const uint32_t *__restrict__ dig, size_t dig_qstride,
                      const uint32_t *__restrict__ left, const uint32_t *__restrict__ right, size_t left_qstride,
                      size_t right_qstride, const uint32_t *__restrict__ tw_base, ModConst mc0, ModConst mc1) {
-  __shared__ uint32_t tiles[8][LDS_TILE];
-  const int id = blockIdx.x, half = 1 << r;
+  __shared__ __align__(16) uint32_t tiles[8][LDS_TILE];
+  const int x = blockIdx.x, y = blockIdx.y;
+  const int half = 1 << r;
   const bool groups = (work & 7) == 0;
-  const int b = groups ? (id & 7) | ((id >> 4) << 3) : id >> 1, m = groups ? (id >> 3) & 1 : id & 1;
-  const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
-  if (odd_live && (i & 1) && !odd_live[blockIdx.y]) return;  // uniform per block: an odd step nothing reads
+  const int b = groups ? (x & 7) | ((x >> 4) << 3) : x >> 1, m = groups ? (x >> 3) & 1 : x & 1;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  uint32_t *vi = v + (size_t)blockIdx.y * q_stride + (size_t)i * 4 * N;
+  const int i = (stop > 0 && r >= stop) ? (b < half ? 2 * b : 2 * (b - half) + 1) : b;
+  uint32_t *vi = v + (size_t)y * q_stride + (size_t)i * 4 * N;
+  uint32_t dw[8];
+  {
+    const uint32_t *dg = dig + (size_t)y * dig_qstride + ((size_t)b * 8 + wave) * DIG_WORDS;
+#pragma unroll
+    for (int i4 = 0; i4 < 8; i4++) dw[i4] = dg[i4 * 64 + lane];
+  }
+  const uint4 row1 = reinterpret_cast<const uint4 *>(vi + (size_t)(2 + m) * N)[tid];
+  // The loads above go out before the flag's load below: otherwise the
+  // compiler moves them behind it and its wait.
+  asm volatile("" ::: "memory");
+  if (odd_live && (i & 1) && !odd_live[y]) return;  // uniform per block: an odd step nothing reads
   const uint32_t t = (uint32_t)(N >> r) + 1u;
   const bool use_left = (r != 0) && ((i & 1) == 0);
-  const uint32_t *W = use_left ? left + (size_t)blockIdx.y * left_qstride : right + (size_t)blockIdx.y * right_qstride;
+  const uint32_t *W = use_left ? left + (size_t)y * left_qstride : right + (size_t)y * right_qstride;
   const uint32_t q = m ? mc1.q : mc0.q, mu1 = m ? mc1.mu1 : mc0.mu1, qinv = m ? mc1.qinv_neg : mc0.qinv_neg;
-  const uint32_t *dg = dig + (size_t)blockIdx.y * dig_qstride + ((size_t)b * 8 + wave) * DIG_WORDS;
-  // keys of the first SP_UPD_KP digits and the e-term gather, loaded ahead of
-  // the digit transforms so that the multiply-accumulate waits for less
   ExpandKeys<SP_UPD_KP> pk;
-  if constexpr (SP_UPD_KP > 0) expand_prefetch<SP_UPD_KP>(pk, vi, W, m, 4 * tid, t);
+  // The e term: row 1 of residue m of v_i (old values) passes through LDS, one
+  // coalesced 16-byte load per thread instead of 4 scattered 4-byte ones.
+  reinterpret_cast<uint4 *>(tiles[0])[tid] = row1;
+  __syncthreads();
+  expand_gather_e<4>(pk.ase, tiles[0], 4 * tid, t);
+  __syncthreads();  // tiles[0] is wave 0's transform tile from here on
   uint32_t rr[32];
 #pragma unroll
-  for (int i4 = 0; i4 < 8; i4++) {
-    const uint32_t w = dg[i4 * 64 + lane];
+  for (int i4 = 0; i4 < 8; i4++)
 #pragma unroll
-    for (int bb = 0; bb < 4; bb++) rr[4 * i4 + bb] = (w >> (8 * bb)) & 0xFFu;
-  }
+    for (int bb = 0; bb < 4; bb++) rr[4 * i4 + bb] = (dw[i4] >> (8 * bb)) & 0xFFu;
+  // keys of the first SP_UPD_KP digits, loaded ahead of the digit transforms
+  // so that the multiply-accumulate waits for less
+  if constexpr (SP_UPD_KP > 0) expand_prefetch_keys<SP_UPD_KP>(pk, W, m, 4 * tid);
   // digits < 128: stages 1-8 and 10-11 skip the reduction (outputs < 8q)
   ntt_fwd_wave_t<8, true>(rr, tw_base + (m * 2) * TW_WORDS, tiles[wave], q, mu1, lane);
   store_p(tiles[wave], rr, lane);  // lazy (< 8q): expand_update reduces after the MAC
@@ -2690,9 +2707,8 @@ hipError_t launch_expand_split(uint32_t *v, size_t q_stride, int n_queries, int 
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_expand_update<<<dim3(2 * work, n_queries), 512, 0, s>>>(
-      v, q_stride, r, stop, max_bits, work, odd_live, neg1_next, dig, dig_qstride, left, right, left_qstride, right_qstride, t.tw,
-      t.mc[0], t.mc[1]);
+  k_expand_update<<<dim3(2 * work, n_queries), 512, 0, s>>>(v, q_stride, r, stop, max_bits, work, odd_live, neg1_next, dig, dig_qstride,
+                                       left, right, left_qstride, right_qstride, t.tw, t.mc[0], t.mc[1]);
   return hipGetLastError();
 }
 
