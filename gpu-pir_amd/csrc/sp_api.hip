@@ -521,20 +521,20 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
   const size_t key_level = (size_t)2 * t_exp * 2 * N;
   const int family = expansion_family(ctx, t_exp, dig);
   if (family == 4 && !dig) return fail(SP_ERR_STATE, "split expansion without digit scratch");
-  // Latency form for the leading rounds whose one-launch grid gives each CU at
-  // most one block: first the half-position form (four 4-wave blocks per
-  // step), then the 8-wave form (two blocks per step), then the split form.
+  // Latency form for the leading rounds: first the half-position form (four
+  // 4-wave blocks per step) while its grid gives each CU at most one block,
+  // then the 8-wave form (two blocks per step) while its grid gives each CU at
+  // most four (two resident at a time), then the split form.
   // At (9,6), one query: rounds 0-5 in halves 4 us faster each than split,
   // round 6 (448 half blocks) 0.7 us slower; round 6 in the 8-wave form (224
-  // blocks) faster than split; round 7 (384 blocks) slower in one launch.
-  // Variant 10 + R forces the half-position form for the first R rounds (those
-  // with N >> r >= 4), then the split form.  For the 8-wave form the blocks
-  // counted are those that work: a live-only run forms no children of skipped
-  // elements, and one whose queries all carry fold vectors (even_only) runs
-  // only the even steps, 2^r per round.  At (9,6), one query with fold vectors:
-  // rounds 0-5 in halves, rounds 6-7 in the 8-wave form, rounds 8-9 split
-  // (583.4 us per answer, against 585.6 us with round 7 split and 587.0 us
-  // with rounds 6 (whose grid half returns at once) in halves as well).
+  // blocks) faster than split.  Variant 10 + R forces the half-position form
+  // for the first R rounds (those with N >> r >= 4), then the split form.  For
+  // the 8-wave form the blocks counted are those that work: a live-only run
+  // forms no children of skipped elements, and one whose queries all carry
+  // fold vectors (even_only) runs only the even steps, 2^r per round.  At
+  // (9,6), one query with fold vectors: rounds 0-5 in halves, rounds 6-9 in
+  // the 8-wave form (580.4-580.9 us per answer, against 583.2-583.6 us with
+  // rounds 6-8 and 583.6-584.7 us with rounds 6-7 in it, the rest split).
   const int var = ctx->variant[SP_STAGE_EXPAND];
   int halves = 0, fused = 0;  // rounds in the half-position form, in either latency form
   if (family == 4 && snap.p && n_queries <= snap.queries) {
@@ -549,7 +549,7 @@ static int run_expansion(sp_ctx *ctx, uint32_t *v, size_t q_stride, int n_querie
     if (var == 0) {
       while (halves < rounds && (N >> halves) >= 4 && 4 * blocks_all(halves) <= cu) halves++;
       fused = halves;
-      while (fused < rounds && 2 * blocks(fused) <= cu) fused++;
+      while (fused < rounds && 2 * blocks(fused) <= 4 * cu) fused++;
     } else if (var >= 10) {
       while (halves < std::min(rounds, var - 10) && (N >> halves) >= 4) halves++;
       fused = halves;
